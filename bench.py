@@ -1,0 +1,268 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X matrix-free Laplacian apply (one GPU).
+
+Default (the contract line): 3D unit cube, 64^3 hexahedra with interior
+vertices perturbed by U[-0.2h, 0.2h] (seed 1), Q4 (5 nodes/dir, 16,581,375
+free dofs), 6 Gauss points/dir, FP64 apply y = A u with u ~ U[-1,1] (seed 0)
+resident in HBM.  A step is one y = A u over all cells (zero y + one apply
+kernel).  Prints one JSON line with GDOF/s, the roofline of the apply kernel
+(HIP-event timed on its stream) and the CPU oracle timed on the host cores.
+
+Other modes (one JSON line each): --mode sweep (Q2..Q7 at ~16M dofs, FP64 and
+FP32), --mode cg (Q7 32^3: FP64 CG vs mixed-precision refinement to 1e-10),
+--mode adjoint (Q5 48^3 kappa-gradient kernel).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
+FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "apply3d_f64_q4_64cube_traffic.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def alg_bytes(n_free, n_cells, n, q, D, s_vec=8, s_g=8):
+    """Algorithmic HBM bytes of one apply: read x + write y, the int32 cell
+    dofs, the stored geometric factor."""
+    return ((s_vec + s_vec) * n_free + 4 * n ** D * n_cells
+            + s_g * (D * (D + 1) // 2) * q ** D * n_cells)
+
+
+def read_traffic():
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def build_problem(pkg, D, m, p, q, perturb=0.2, kappa=None):
+    model = pkg.CartesianModel(D, m, perturb=perturb, seed=1)
+    space = pkg.FESpace(model, p)
+    mp = pkg.SumFactorizationMap.with_points(D, p, q)
+    A = pkg.LazyMatrix(mp, space, kappa=kappa)
+    return model, space, mp, A
+
+
+def time_apply(torch, A, x, y, steps, warmup, reduced=None):
+    """Wall time per step (zero y + apply) and mean apply-kernel time (HIP
+    events on the launch stream)."""
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        y.zero_()
+        if reduced:
+            A.mul_reduced(y, x, reduced)
+        else:
+            A.mul_add(y, x)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        if reduced:
+            ev[i][0].record(stream)
+            A.mul_reduced(y, x, reduced)  # includes its own zero fill
+            ev[i][1].record(stream)
+        else:
+            y.zero_()
+            ev[i][0].record(stream)
+            A.mul_add(y, x)
+            ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    kern = sum(a.elapsed_time(b) for a, b in ev) / steps / 1e3
+    return wall, kern
+
+
+def mode_apply(args, torch, pkg):
+    import numpy as np
+    D, m, p, q = 3, 64, 4, 6
+    log("setting up 3D 64^3 Q4 q=6 perturbed ...")
+    model, space, mp, A = build_problem(pkg, D, m, p, q)
+    n = p + 1
+    xh = pkg.uniform(0, space.n_free, -1.0, 1.0)
+    x = torch.from_numpy(xh).cuda()
+    y = torch.empty_like(x)
+    log("timing %d steps after %d warmup ..." % (args.steps, args.warmup))
+    wall, kern = time_apply(torch, A, x, y, args.steps, args.warmup)
+    Balg = alg_bytes(space.n_free, space.n_cells, n, q, D)
+    Falg = pkg.count_manual_flops_poisson_matrix_free(space.n_cells, D, (q,) * D, (n,) * D,
+                                                      count_only_fma_ops=False,
+                                                      structured_mesh_opt=False)
+    achieved = Balg / kern / 1e9
+    out = {
+        "metric": "Laplacian apply GDOF/s (3D Q4-Q7, FP64 vs mixed CG@1e-10), % of roofline",
+        "value": space.n_free / wall / 1e9,
+        "unit": "GDOF/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded perturbed mesh, u ~ U[-1,1])",
+        "config": {"workload": "3D unit cube 64^3 hexes perturbed +-0.2h, Q4 (5 nodes/dir), "
+                               "6 Gauss pts/dir, FP64 apply y = A u",
+                   "n_free": space.n_free, "n_cells": space.n_cells, "order": p, "nquad": q},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
+                     "kernel": "k_apply3d<5,6,double,double>", "kernel_ms": kern * 1e3,
+                     "alg_bytes": Balg, "alg_flops": Falg,
+                     "achieved_tflops": Falg / kern / 1e12},
+    }
+    # parity + CPU baseline from the same oracle run
+    yd = y.cpu().numpy()
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        X = model.cell_vertices()
+        log("CPU oracle (%d threads) ..." % threads)
+        y_ref = oracle.apply(D, n, q, space.cell_dofs, X, xh, mp.tables, nthreads=threads)
+        reps = 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            y_ref = oracle.apply(D, n, q, space.cell_dofs, X, xh, mp.tables, nthreads=threads)
+        tc = (time.perf_counter() - t0) / reps
+        out["cpu_baseline"] = {"value": space.n_free / tc / 1e9, "unit": "GDOF/s",
+                               "cores": threads, "kind": "port",
+                               "sample": "full 64^3 Q4 q=6 apply (the same workload), "
+                                         "mean of %d after 1 warm-up, OpenMP C oracle" % reps}
+        out["parity_rel_l2"] = float(np.linalg.norm(yd - y_ref) / np.linalg.norm(y_ref))
+    A.free()
+    return out
+
+
+def mode_sweep(args, torch, pkg):
+    rows = []
+    for p in range(2, 8):
+        q = p + 2
+        m = int(round(256 / p))
+        model, space, mp, A = build_problem(pkg, 3, m, p, q)
+        n = p + 1
+        x = torch.from_numpy(pkg.uniform(0, space.n_free, -1, 1)).cuda()
+        y = torch.empty_like(x)
+        wall, kern = time_apply(torch, A, x, y, args.steps, args.warmup)
+        B64 = alg_bytes(space.n_free, space.n_cells, n, q, 3)
+        F = pkg.count_manual_flops_poisson_matrix_free(space.n_cells, 3, (q,) * 3, (n,) * 3,
+                                                       count_only_fma_ops=False,
+                                                       structured_mesh_opt=False)
+        troof = max(B64 / (HBM_PEAK_GBS * 1e9), F / (FP64_PEAK_TFLOPS * 1e12))
+        row = {"order": p, "nquad": q, "m": m, "n_free": space.n_free,
+               "f64_gdofs": space.n_free / wall / 1e9, "f64_kernel_ms": kern * 1e3,
+               "f64_roofline_frac": troof / kern, "f64_GBs": B64 / kern / 1e9}
+        for prec in ("f32", "f32_gbf16"):
+            x32 = x.float()
+            y32 = torch.empty_like(x32)
+            A.prepare(prec)
+            w32, k32 = time_apply(torch, A, x32, y32, args.steps, args.warmup, reduced=prec)
+            sg = 4 if prec == "f32" else 2
+            B32 = alg_bytes(space.n_free, space.n_cells, n, q, 3, s_vec=4, s_g=sg)
+            row[prec + "_gdofs"] = space.n_free / w32 / 1e9
+            row[prec + "_kernel_ms"] = k32 * 1e3
+            row[prec + "_GBs"] = B32 / k32 / 1e9
+        log(json.dumps(row))
+        rows.append(row)
+        A.free()
+        del x, y
+        torch.cuda.empty_cache()
+    return {"metric": "Q2-Q7 apply sweep at ~16M dofs (3D perturbed, q = p + 2)", "rows": rows,
+            "unit": "GDOF/s", "n_gpus": 1}
+
+
+def mode_cg(args, torch, pkg):
+    import numpy as np
+    D, m, p, q = 3, 32, 7, 9
+    log("setting up 3D 32^3 Q7 q=9 perturbed ...")
+    model, space, mp, A = build_problem(pkg, D, m, p, q)
+    b = torch.from_numpy(pkg.normal(2, space.n_free)).cuda()
+    res = {"metric": "CG to 1e-10: FP64 vs mixed-precision refinement, 3D 32^3 Q7 q=9",
+           "n_free": space.n_free, "n_gpus": 1}
+    x64 = torch.zeros_like(b)
+    st = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=args.maxit)
+    res["f64"] = st
+    log("f64", st)
+    for inner, irt in [("f32", 1e-3), ("f32", 1e-2), ("f32_gbf16", 1e-2)]:
+        x = torch.zeros_like(b)
+        st = pkg.cg_ir_(x, A, b, rtol=1e-10, inner=inner, inner_rtol=irt,
+                        inner_maxiter=args.maxit)
+        st["err_vs_f64"] = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
+        st["speedup_vs_f64"] = res["f64"]["seconds"] / st["seconds"]
+        res["%s_inner%g" % (inner, irt)] = st
+        log(inner, irt, st)
+    _ = np
+    A.free()
+    return res
+
+
+def mode_adjoint(args, torch, pkg):
+    import numpy as np
+    D, m, p, q = 3, 48, 5, 7
+    log("setting up 3D 48^3 Q5 q=7 perturbed with kappa ...")
+    model = pkg.CartesianModel(D, m, perturb=0.2, seed=1)
+    space = pkg.FESpace(model, p)
+    mp = pkg.SumFactorizationMap.with_points(D, p, q)
+    nq = q ** D
+    kappa = np.exp(pkg.normal(3, space.n_cells * nq))
+    A = pkg.LazyMatrix(mp, space, kappa=kappa)
+    u = torch.from_numpy(pkg.uniform(0, space.n_free, -1, 1)).cuda()
+    d = torch.from_numpy(pkg.uniform(4, space.n_free, -1, 1)).cuda()
+    y = torch.empty_like(u)
+    A.mul(y, u)
+    r = y - d
+    kb = torch.empty(space.n_cells * nq, dtype=torch.float64, device="cuda")
+    ub = torch.empty_like(u)
+    import ctypes
+    for _ in range(args.warmup):
+        pkg._lib.check(A._L.sf_adjoint(A.plan, ctypes.c_void_p(u.data_ptr()),
+                                       ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(kb.data_ptr()),
+                                       ctypes.c_void_p(ub.data_ptr()), None))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pkg._lib.check(A._L.sf_adjoint(A.plan, ctypes.c_void_p(u.data_ptr()),
+                                       ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(kb.data_ptr()),
+                                       ctypes.c_void_p(ub.data_ptr()), None))
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    n = p + 1
+    Badj = (8 * 3 * space.n_free + 4 * n ** 3 * space.n_cells
+            + 8 * (6 + 1 + 1) * nq * space.n_cells)
+    A.free()
+    return {"metric": "adjoint (dJ/dkappa + dJ/du) 3D 48^3 Q5 q=7", "ms": t * 1e3,
+            "GB/s": Badj / t / 1e9, "frac_hbm": Badj / t / 1e9 / HBM_PEAK_GBS,
+            "n_free": space.n_free, "n_gpus": 1}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mode", default="apply", choices=["apply", "sweep", "cg", "adjoint"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--maxit", type=int, default=20000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (--gpus %d requested); the apply "
+              "would shard by cell slabs with a halo sum of interface dofs, which is not "
+              "built here." % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import __graft_entry__
+    pkg = __graft_entry__.import_package()
+    torch.cuda.set_device(0)
+    fn = {"apply": mode_apply, "sweep": mode_sweep, "cg": mode_cg, "adjoint": mode_adjoint}
+    out = fn[args.mode](args, torch, pkg)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

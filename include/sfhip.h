@@ -1,0 +1,222 @@
+/*
+ * sfhip.h -- C ABI of the MI355X (gfx950) matrix-free, sum-factorized
+ * high-order finite-element Laplacian  y = A u.
+ *
+ * Every entry point takes plain C types (pointers, sizes, ints), returns an
+ * int status (SF_OK = 0) and never lets a C++ exception cross the boundary.
+ * sf_last_error() returns a thread-local message for the last failure.
+ *
+ * Conventions
+ *   - D in {2, 3}; n = p + 1 nodes and q Gauss points per direction
+ *     (isotropic, as the reference's GPU path requires:
+ *     src/SumFactorizationMaps.jl:25-32 packs one q x n matrix per direction).
+ *   - 1D tables are row-major q x n:  B[i*n + j] = phi_j(xi_i),
+ *     Dm[i*n + j] = phi_j'(xi_i)  (reference: A0 / B0 in
+ *     src/SumFactorizationMaps.jl:54-55).
+ *   - Cell-local arrays are tensor-lexicographic with direction 1 (x) fastest,
+ *     the same order as the reference's Z tensors and its linear quadrature
+ *     index (src/SumFactorizationKernels.jl:174-216).
+ *   - Geometric factor per quadrature point: the symmetric
+ *     G = w_q |det J| (J^T J)^{-1}, stored as D(D+1)/2 entries
+ *     (2D: G00 G01 G11; 3D: G00 G01 G02 G11 G12 G22), layout
+ *     [cell][entry][quadrature point].  This is the product the reference
+ *     forms on the fly in _sumfac_apply_weights! from wq, jq = J^{-T} and
+ *     djq = |det J| (src/SumFactorizationKernels.jl:191-210).
+ *   - Device-pointer entry points are stream ordered on `stream`
+ *     (a hipStream_t; NULL = the null stream) and return before the work ends.
+ *   - A plan is not re-entrant: one host thread per plan at a time.
+ */
+#ifndef SFHIP_H
+#define SFHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum sf_status {
+  SF_OK = 0,
+  SF_ERR_ARG = 1,           /* bad dimension / order / size / null pointer   */
+  SF_ERR_UNSUPPORTED = 2,   /* (n, q) pair or precision without a kernel     */
+  SF_ERR_DOF_RANGE = 3,     /* a cell dof id outside the free-dof range      */
+  SF_ERR_JACOBIAN = 4,      /* det J <= 0 (or non-finite) at a quad. point   */
+  SF_ERR_HIP = 5,           /* HIP runtime error                             */
+  SF_ERR_NOT_READY = 6,     /* plan has no cells or no geometry yet          */
+  SF_ERR_NOT_CONVERGED = 7, /* solver hit its iteration limit                */
+  SF_ERR_NONFINITE = 8      /* NaN / Inf met in a solver                     */
+};
+
+/* Arithmetic of an operator application. */
+enum sf_precision {
+  SF_PREC_F64 = 0,       /* FP64 everywhere                                       */
+  SF_PREC_F32 = 1,       /* FP32 vectors, tables, G and contractions (vector ALU) */
+  SF_PREC_F32_GBF16 = 2, /* as F32, but G stored in bfloat16 (half the G bytes)   */
+  SF_PREC_MFMA_BF16 = 3  /* contractions as bf16 MFMA GEMMs, FP32 accumulation    */
+};
+
+typedef struct sf_plan sf_plan; /* opaque */
+
+typedef struct sf_plan_info {
+  int dim, order, n1d, q1d;
+  int64_t n_cells, n_free;
+  int64_t n_qp;            /* n_cells * q^D                                   */
+  int has_cells, has_geometry, has_kappa;
+  int threads_per_block;   /* launch shape of the FP64 apply kernel           */
+  int cells_per_block;
+  int64_t device_bytes;    /* device memory the plan holds                   */
+} sf_plan_info;
+
+typedef struct sf_solve_stats {
+  int iterations;        /* CG iterations (IR: total inner iterations)      */
+  int outer_iterations;  /* IR outer (FP64 residual) steps; 0 for plain CG  */
+  int converged;         /* 1 when ||b - A x|| / ||b|| <= rtol               */
+  double rel_residual;   /* final FP64 ||b - A x|| / ||b||                   */
+  double seconds;        /* wall time of the solve (device synchronised)     */
+} sf_solve_stats;
+
+/* ---- diagnostics ------------------------------------------------------- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* sf_last_error(void);
+
+/* Library version string. */
+const char* sf_version(void);
+
+/* ---- 1D tables (host only, no device needed) -------------------------- */
+
+/* Lagrange basis of order p on the 1D nodes in the reference's node order
+ * [0, 1, 1/p, 2/p, ..., (p-1)/p] evaluated at the q Gauss-Legendre points of
+ * [0, 1] (ascending), plus the points and weights.  Any output may be NULL.
+ * Replaces the table construction of SumFactorizationMap(D, fe_orders,
+ * quad_orders), src/SumFactorizationMaps.jl:47-65 (A0 :54, B0 :55). */
+int sf_tables_1d(int order, int nquad, double* B, double* Dm, double* points,
+                 double* weights);
+
+/* Number of Gauss points Gridap uses for a quadrature degree:
+ * ceil((degree + 1) / 2) (Quadrature(SEGMENT, degree), used at
+ * src/SumFactorizationMaps.jl:51). */
+int sf_gauss_points_for_degree(int degree);
+
+/* Analytic FLOP model of one apply, restating
+ * count_manual_flops_poisson_matrix_free(ncells, D, SQ, SB;
+ * count_only_fma_ops, structured_mesh_opt), src/Utils.jl:33-72. */
+int64_t sf_flops_model(int64_t n_cells, int dim, int nquad, int n1d,
+                       int count_only_fma_ops, int structured_mesh_opt);
+
+/* ---- plan life cycle ---------------------------------------------------- */
+
+/* Create a plan on the current HIP device.  Replaces the setup of
+ * LazyMatrix(m, U, V, dOmega), src/LazyMatrices.jl:30-50, together with the
+ * descriptor SumFactorizationMap{D,SB,SQ}, src/SumFactorizationMaps.jl:2-10.
+ * Default tables (sf_tables_1d) are installed; flags is reserved (0). */
+int sf_plan_create(sf_plan** out, int dim, int order, int nquad,
+                   int64_t n_cells, int64_t n_free, int flags);
+
+/* Release every device buffer of the plan.  NULL is accepted. */
+int sf_plan_destroy(sf_plan* plan);
+
+int sf_plan_get_info(const sf_plan* plan, sf_plan_info* info);
+
+/* Replace the 1D tables (host arrays, q x n row-major) -- e.g. Gridap's own
+ * A0 / B0 so the device uses bit-identical tables.
+ * Reference: m.mats = [evalMats, gradMats], src/SumFactorizationMaps.jl:57-58. */
+int sf_plan_set_tables(sf_plan* plan, const double* B, const double* Dm,
+                       const double* weights);
+
+/* Cell-to-dof map, host array of n_cells * n^D int32.
+ * index_base = 1: Gridap convention (free ids 1..n_free, ids <= 0 are
+ *                 Dirichlet), i.e. get_cell_dof_ids(U).data as used by
+ *                 src/LazyMatrices.jl:47 and test/Demo_StepByStep.jl:57-69;
+ * index_base = 0: free ids 0..n_free-1, negative ids are Dirichlet.
+ * local_to_tensor (may be NULL): n^D entries, local dof k of every cell goes
+ * to tensor-lexicographic slot local_to_tensor[k] (0-based) -- the
+ * reference's dof_map (_get_dof_map, src/SumFactorizationMaps.jl:40-45)
+ * linearised.  NULL means the ids are already tensor-lexicographic.
+ * Ids are range-checked; SF_ERR_DOF_RANGE on failure. */
+int sf_plan_set_cells(sf_plan* plan, const int32_t* cell_dofs, int index_base,
+                      const int32_t* local_to_tensor);
+
+/* Geometry from cell vertices: host array [cell][2^D vertices][D coords],
+ * vertices tensor-lexicographic (x fastest), trilinear (bilinear) map.
+ * Computes G on the device in FP64 once (J, det J, (J^T J)^{-1}).
+ * Replaces the lazy cell_jq / cell_djq of src/LazyMatrices.jl:37-41.
+ * SF_ERR_JACOBIAN if det J <= 0 anywhere. */
+int sf_plan_set_vertices(sf_plan* plan, const double* vertices);
+
+/* Geometry given directly: host array [cell][D(D+1)/2][q^D] (see header). */
+int sf_plan_set_geometry(sf_plan* plan, const double* G);
+
+/* Copy the plan's FP64 geometric factor back to a host array of the layout
+ * above (for checking). */
+int sf_plan_get_geometry(const sf_plan* plan, double* G);
+
+/* Variable diffusion coefficient kappa at quadrature points, host array
+ * [cell][q^D] (positive).  Afterwards every apply uses kappa * G.
+ * No reference counterpart (the reference's operator has kappa = 1). */
+int sf_plan_set_kappa(sf_plan* plan, const double* kappa);
+
+/* Build the reduced-precision operator data (FP32 / bf16 G, FP32 tables)
+ * ahead of time; otherwise done on the first call that needs it. */
+int sf_plan_prepare(sf_plan* plan, int precision);
+
+/* ---- the hot path ------------------------------------------------------- */
+
+/* y = A x on free dofs, FP64, device pointers, y overwritten (not
+ * accumulated).  Replaces LinearAlgebra.mul!(y, A::LazyMatrix, x),
+ * src/LazyMatrices.jl:52-61, whose per-cell work is
+ * Arrays.evaluate!(cache, m::SumFactorizationMap, x, wq, jq, djq),
+ * src/SumFactorizationMaps.jl:78-89 (gather, dof->quad contractions,
+ * weights + Jacobian, quad->dof contractions, scatter). */
+int sf_apply(sf_plan* plan, double* y, const double* x, void* stream);
+
+/* y += A x (FP64, device pointers): the accumulate convention of the
+ * reference's GPU kernels, whose scatter adds into a y the caller zeroed
+ * (gpu_mul_v0! .. gpu_mul_v4!, src/GPUKernels.jl:77-89; launched as
+ * kernel(args...; threads, blocks, shmem), test/GPU_v3.jl:77-84).  One
+ * kernel launch, no fill. */
+int sf_apply_add(sf_plan* plan, double* y, const double* x, void* stream);
+
+/* Same, with host pointers: stages x / y through device buffers and returns
+ * when y is complete.  The host-vector form of the reference's mul!. */
+int sf_apply_host(sf_plan* plan, double* y, const double* x);
+
+/* y = A x with FP32 vectors (device pointers).  precision is SF_PREC_F32,
+ * SF_PREC_F32_GBF16 or SF_PREC_MFMA_BF16.  No reference counterpart: the
+ * reduced-precision inner operator of the mixed-precision solver. */
+int sf_apply_f32(sf_plan* plan, float* y, const float* x, int precision,
+                 void* stream);
+
+/* ---- differentiation ---------------------------------------------------- */
+
+/* Pullback of y = A(kappa) u for a cotangent lam (device pointers, FP64):
+ *   dJdkappa[cell][qp] = grad(lam)_q . (G_q grad(u)_q)      (kappa-free G),
+ *   dJdu               = A(kappa)^T lam = A(kappa) lam       (overwritten).
+ * Gradients at quadrature points are recomputed, never stored.  Either
+ * output may be NULL.  No reference counterpart (new kernel). */
+int sf_adjoint(sf_plan* plan, const double* u, const double* lam,
+               double* dJdkappa, double* dJdu, void* stream);
+
+/* ---- solvers (device pointers, FP64 b and x) --------------------------- */
+
+/* Unpreconditioned CG on the FP64 operator, x used as the initial guess,
+ * stop when ||r|| <= rtol ||b||.  Replaces IterativeSolvers.cg!(x, A, b)
+ * as called by compute_solution, profiling/drivers.jl:85-95. */
+int sf_cg(sf_plan* plan, const double* b, double* x, double rtol, int maxit,
+          sf_solve_stats* stats, void* stream);
+
+/* Mixed-precision iterative refinement: FP64 outer residual r = b - A x,
+ * FP64 norms and dot products, inner CG with the reduced-precision operator
+ * `inner_precision` (FP32 vectors) to relative tolerance inner_rtol or
+ * inner_maxit iterations, FP64 correction x += d.  Stops when the FP64
+ * ||b - A x|| <= rtol ||b|| or after max_outer steps.  x is the initial
+ * guess.  No reference counterpart. */
+int sf_cg_ir(sf_plan* plan, const double* b, double* x, double rtol,
+             int max_outer, int inner_precision, double inner_rtol,
+             int inner_maxit, sf_solve_stats* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFHIP_H */

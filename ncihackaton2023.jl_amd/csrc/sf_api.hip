@@ -1,0 +1,945 @@
+// sf_api.hip -- the C ABI (include/sfhip.h): plans, tables, set-up, apply,
+// adjoint and the device-resident solvers.  Host code only; kernels live in
+// the other translation units.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sf_common.hpp"
+#include "sf_kernels.h"
+#include "sfhip.h"
+
+using namespace sf;
+
+// ---------------------------------------------------------------------------
+// errors
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define SF_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return fail(SF_ERR_HIP, "%s failed: %s (%s:%d)", #call,               \
+                  hipGetErrorString(e_), __FILE__, __LINE__);               \
+  } while (0)
+
+#define SF_TRY try {
+#define SF_CATCH                                                       \
+  }                                                                    \
+  catch (const std::bad_alloc&) {                                      \
+    return fail(SF_ERR_ARG, "host allocation failed");                 \
+  }                                                                    \
+  catch (...) {                                                        \
+    return fail(SF_ERR_ARG, "unexpected C++ exception");               \
+  }
+
+// ---------------------------------------------------------------------------
+// 1D tables: Gauss-Legendre on [0, 1] and Lagrange polynomials on the
+// reference's node order.
+
+// Legendre P_q(z) and P_{q-1}(z) by the three-term recurrence.
+void legendre_pair(int q, double z, double* pq, double* pqm1) {
+  double pn = z, pm = 1.0;
+  for (int k = 2; k <= q; ++k) {
+    const double pk = ((2.0 * k - 1.0) * z * pn - (k - 1.0) * pm) / k;
+    pm = pn;
+    pn = pk;
+  }
+  *pq = pn;
+  *pqm1 = pm;
+}
+
+// Gauss-Legendre points / weights on [-1, 1] by Newton iteration on P_q,
+// ascending; w = 2 / ((1 - z^2) P_q'(z)^2).
+void gauss_legendre(int q, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(q, 0.0);
+  w.assign(q, 0.0);
+  const double pi = 3.14159265358979323846;
+  for (int i = 0; i < (q + 1) / 2; ++i) {
+    double z = cos(pi * (i + 0.75) / (q + 0.5));  // i-th root from the top
+    double pn, pm;
+    for (int it = 0; it < 100; ++it) {
+      legendre_pair(q, z, &pn, &pm);
+      const double dp = q * (z * pn - pm) / (z * z - 1.0);
+      const double dz = pn / dp;
+      z -= dz;
+      if (fabs(dz) <= 1e-15) break;
+    }
+    legendre_pair(q, z, &pn, &pm);
+    const double dp = q * (z * pn - pm) / (z * z - 1.0);
+    const double wi = 2.0 / ((1.0 - z * z) * dp * dp);
+    x[q - 1 - i] = z;
+    x[i] = -z;
+    w[i] = wi;
+    w[q - 1 - i] = wi;
+  }
+  if (q % 2 == 1) x[q / 2] = 0.0;
+}
+
+// nodes in the reference's 1D order: [0, 1, 1/p, ..., (p-1)/p]
+// (Gridap's SEGMENT node order: vertices first, then interior nodes).
+std::vector<double> lagrange_nodes(int p) {
+  std::vector<double> t;
+  if (p == 0) {
+    t.push_back(0.5);
+    return t;
+  }
+  t.push_back(0.0);
+  t.push_back(1.0);
+  for (int k = 1; k < p; ++k) t.push_back((double)k / p);
+  return t;
+}
+
+int build_tables(int p, int q, double* B, double* Dm, double* pts,
+                 double* wts) {
+  if (p < 1 || p > kMaxN - 1) return fail(SF_ERR_ARG, "order %d outside 1..%d", p, kMaxN - 1);
+  if (q < 1 || q > 64) return fail(SF_ERR_ARG, "nquad %d outside 1..64", q);
+  const int n = p + 1;
+  std::vector<double> gx, gw;
+  gauss_legendre(q, gx, gw);
+  const std::vector<double> t = lagrange_nodes(p);
+  for (int i = 0; i < q; ++i) {
+    const double xi = 0.5 * (gx[i] + 1.0);
+    if (pts) pts[i] = xi;
+    if (wts) wts[i] = 0.5 * gw[i];
+    for (int j = 0; j < n; ++j) {
+      // phi_j(x) = prod_{m != j} (x - t_m) / (t_j - t_m)
+      double den = 1.0, val = 1.0;
+      for (int m = 0; m < n; ++m)
+        if (m != j) {
+          den *= (t[j] - t[m]);
+          val *= (xi - t[m]);
+        }
+      // phi_j'(x) = sum_k prod_{m != j, k} (x - t_m) / den
+      double der = 0.0;
+      for (int k = 0; k < n; ++k) {
+        if (k == j) continue;
+        double pr = 1.0;
+        for (int m = 0; m < n; ++m)
+          if (m != j && m != k) pr *= (xi - t[m]);
+        der += pr;
+      }
+      if (B) B[i * n + j] = val / den;
+      if (Dm) Dm[i * n + j] = der / den;
+    }
+  }
+  return SF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device buffer helper
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  int64_t n = 0;
+  hipError_t alloc(int64_t count) {
+    if (p && n == count) return hipSuccess;
+    release();
+    if (count <= 0) return hipSuccess;
+    hipError_t e = hipMalloc((void**)&p, sizeof(T) * (size_t)count);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    n = count;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int64_t bytes() const { return n * (int64_t)sizeof(T); }
+};
+
+struct DeviceGuard {
+  int prev = -1;
+  bool changed = false;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) == hipSuccess && prev != dev) {
+      changed = (hipSetDevice(dev) == hipSuccess);
+    }
+  }
+  ~DeviceGuard() {
+    if (changed) (void)hipSetDevice(prev);
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// plan
+
+struct sf_plan {
+  int dim = 0, order = 0, n = 0, q = 0, ng = 0;
+  int64_t ncell = 0, nfree = 0, nloc = 0, nqc = 0;
+  int device = 0;
+  std::vector<double> hB, hD, hW, hPts;
+  DevBuf<double> dB, dD, dW, dPts;
+  DevBuf<float> dB32, dD32;
+  DevBuf<int32_t> cells;
+  DevBuf<double> G, Gk, kappa;
+  DevBuf<float> G32;
+  DevBuf<bf16s> Gbf;
+  bool has_cells = false, has_geom = false, has_kappa = false;
+  bool g32_valid = false, gbf_valid = false;
+  DevBuf<int> flag;
+  // staging of the host-pointer apply
+  DevBuf<double> xs, ys;
+  // solver workspace
+  DevBuf<double> w_r, w_p, w_q, w_t;
+  DevBuf<float> w_r32, w_s32, w_d32, w_p32, w_q32;
+  DevBuf<double> red_part;
+  DevBuf<unsigned> red_counter;
+  DevBuf<CGScalars> scal;
+  DevBuf<double> dscal;  // dot results
+  hipStream_t own_stream = nullptr;
+
+  const double* active_G() const { return has_kappa ? Gk.p : G.p; }
+
+  void release_all() {
+    dB.release(); dD.release(); dW.release(); dPts.release();
+    dB32.release(); dD32.release();
+    cells.release(); G.release(); Gk.release(); kappa.release();
+    G32.release(); Gbf.release(); flag.release();
+    xs.release(); ys.release();
+    w_r.release(); w_p.release(); w_q.release(); w_t.release();
+    w_r32.release(); w_s32.release(); w_d32.release(); w_p32.release(); w_q32.release();
+    red_part.release(); red_counter.release(); scal.release(); dscal.release();
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+    own_stream = nullptr;
+  }
+
+  int64_t device_bytes() const {
+    return dB.bytes() + dD.bytes() + dW.bytes() + dPts.bytes() + dB32.bytes() +
+           dD32.bytes() + cells.bytes() + G.bytes() + Gk.bytes() +
+           kappa.bytes() + G32.bytes() + Gbf.bytes() + flag.bytes() +
+           xs.bytes() + ys.bytes() + w_r.bytes() + w_p.bytes() + w_q.bytes() +
+           w_t.bytes() + w_r32.bytes() + w_s32.bytes() + w_d32.bytes() + w_p32.bytes() +
+           w_q32.bytes() + red_part.bytes() + red_counter.bytes() +
+           scal.bytes() + dscal.bytes();
+  }
+};
+
+namespace {
+
+int upload_tables(sf_plan* P) {
+  const int64_t tn = (int64_t)P->q * P->n;
+  SF_HIP(P->dB.alloc(tn));
+  SF_HIP(P->dD.alloc(tn));
+  SF_HIP(P->dW.alloc(P->q));
+  SF_HIP(P->dPts.alloc(P->q));
+  SF_HIP(P->dB32.alloc(tn));
+  SF_HIP(P->dD32.alloc(tn));
+  SF_HIP(hipMemcpy(P->dB.p, P->hB.data(), tn * 8, hipMemcpyHostToDevice));
+  SF_HIP(hipMemcpy(P->dD.p, P->hD.data(), tn * 8, hipMemcpyHostToDevice));
+  SF_HIP(hipMemcpy(P->dW.p, P->hW.data(), P->q * 8, hipMemcpyHostToDevice));
+  SF_HIP(hipMemcpy(P->dPts.p, P->hPts.data(), P->q * 8, hipMemcpyHostToDevice));
+  std::vector<float> b32(tn), d32(tn);
+  for (int64_t i = 0; i < tn; ++i) {
+    b32[i] = (float)P->hB[i];
+    d32[i] = (float)P->hD[i];
+  }
+  SF_HIP(hipMemcpy(P->dB32.p, b32.data(), tn * 4, hipMemcpyHostToDevice));
+  SF_HIP(hipMemcpy(P->dD32.p, d32.data(), tn * 4, hipMemcpyHostToDevice));
+  return SF_OK;
+}
+
+int check_plan(const sf_plan* P) {
+  if (!P) return fail(SF_ERR_ARG, "null plan");
+  return SF_OK;
+}
+
+int check_ready(const sf_plan* P) {
+  if (!P) return fail(SF_ERR_ARG, "null plan");
+  if (!P->has_cells) return fail(SF_ERR_NOT_READY, "plan has no cell dofs (sf_plan_set_cells)");
+  if (!P->has_geom)
+    return fail(SF_ERR_NOT_READY, "plan has no geometry (sf_plan_set_vertices / sf_plan_set_geometry)");
+  return SF_OK;
+}
+
+int read_flag(sf_plan* P, int* out) {
+  int h = 0;
+  SF_HIP(hipMemcpy(&h, P->flag.p, sizeof(int), hipMemcpyDeviceToHost));
+  *out = h;
+  return SF_OK;
+}
+
+int clear_flag(sf_plan* P) {
+  SF_HIP(P->flag.alloc(1));
+  SF_HIP(hipMemset(P->flag.p, 0, sizeof(int)));
+  return SF_OK;
+}
+
+// y = A x (FP64) on stream s, y overwritten.
+int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s) {
+  SF_HIP(launch_fill<double>(y, P->nfree, 0.0, s));
+  hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
+                           P->dB.p, P->dD.p, P->ncell, s);
+  if (e == hipErrorNotSupported)
+    return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
+  SF_HIP(e);
+  return SF_OK;
+}
+
+int prepare(sf_plan* P, int precision) {
+  const int64_t ng = P->ncell * P->ng * P->nqc;
+  if (precision == SF_PREC_F64) return SF_OK;
+  if (precision == SF_PREC_F32 || precision == SF_PREC_MFMA_BF16) {
+    if (!P->g32_valid) {
+      SF_HIP(P->G32.alloc(ng));
+      SF_HIP(launch_to_f32(P->active_G(), ng, P->G32.p, nullptr));
+      SF_HIP(hipDeviceSynchronize());
+      P->g32_valid = true;
+    }
+    return SF_OK;
+  }
+  if (precision == SF_PREC_F32_GBF16) {
+    if (!P->gbf_valid) {
+      SF_HIP(P->Gbf.alloc(ng));
+      SF_HIP(launch_to_bf16(P->active_G(), ng, P->Gbf.p, nullptr));
+      SF_HIP(hipDeviceSynchronize());
+      P->gbf_valid = true;
+    }
+    return SF_OK;
+  }
+  return fail(SF_ERR_UNSUPPORTED, "unknown precision %d", precision);
+}
+
+// y = A x in reduced precision (FP32 vectors), y overwritten.
+int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
+                 hipStream_t s) {
+  SF_HIP(launch_fill<float>(y, P->nfree, 0.0f, s));
+  hipError_t e;
+  if (precision == SF_PREC_F32) {
+    e = apply_f32(P->dim, P->n, P->q, x, y, P->cells.p, P->G32.p, P->dB32.p,
+                  P->dD32.p, P->ncell, s);
+  } else if (precision == SF_PREC_F32_GBF16) {
+    e = apply_f32_gbf16(P->dim, P->n, P->q, x, y, P->cells.p, P->Gbf.p,
+                        P->dB32.p, P->dD32.p, P->ncell, s);
+  } else {
+    return fail(SF_ERR_UNSUPPORTED, "precision %d has no FP32 apply yet", precision);
+  }
+  if (e == hipErrorNotSupported)
+    return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
+  SF_HIP(e);
+  return SF_OK;
+}
+
+int ensure_solver_ws(sf_plan* P, bool mixed) {
+  SF_HIP(P->w_r.alloc(P->nfree));
+  SF_HIP(P->w_p.alloc(P->nfree));
+  SF_HIP(P->w_q.alloc(P->nfree));
+  SF_HIP(P->w_t.alloc(P->nfree));
+  if (mixed) {
+    SF_HIP(P->w_r32.alloc(P->nfree));
+    SF_HIP(P->w_s32.alloc(P->nfree));
+    SF_HIP(P->w_d32.alloc(P->nfree));
+    SF_HIP(P->w_p32.alloc(P->nfree));
+    SF_HIP(P->w_q32.alloc(P->nfree));
+  }
+  if (!P->red_part.p) {
+    SF_HIP(P->red_part.alloc(kRedBlocks));
+    SF_HIP(P->red_counter.alloc(1));
+    SF_HIP(hipMemset(P->red_counter.p, 0, sizeof(unsigned)));
+    SF_HIP(P->scal.alloc(1));
+    SF_HIP(P->dscal.alloc(4));
+  }
+  return SF_OK;
+}
+
+int read_scalars(sf_plan* P, CGScalars* h, hipStream_t s) {
+  SF_HIP(hipMemcpyAsync(h, P->scal.p, sizeof(CGScalars), hipMemcpyDeviceToHost, s));
+  SF_HIP(hipStreamSynchronize(s));
+  return SF_OK;
+}
+
+int dot_host(sf_plan* P, const double* a, const double* b, double* out,
+             hipStream_t s) {
+  Reducer red{P->red_part.p, P->red_counter.p};
+  SF_HIP(launch_dot<double>(a, b, P->nfree, red, P->dscal.p, s));
+  SF_HIP(hipMemcpyAsync(out, P->dscal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  SF_HIP(hipStreamSynchronize(s));
+  return SF_OK;
+}
+
+// Plain CG on the FP64 operator, device vectors.  Chunks of `chunk`
+// iterations are queued between host checks of the device scalars; once
+// converged the remaining queued updates are no-ops.
+int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
+           hipStream_t s, int* iters_out, int* nonfinite_out) {
+  Reducer red{P->red_part.p, P->red_counter.p};
+  double bb = 0.0;
+  int rc = dot_host(P, b, b, &bb, s);
+  if (rc) return rc;
+  const double stop_rr = rtol * rtol * bb;
+  rc = do_apply_f64(P, P->w_t.p, x, s);
+  if (rc) return rc;
+  SF_HIP(launch_cg_start<double>(b, P->w_t.p, P->w_r.p, P->w_p.p, P->w_q.p,
+                                 P->nfree, red, P->scal.p, stop_rr, s));
+  CGScalars h{};
+  rc = read_scalars(P, &h, s);
+  if (rc) return rc;
+  const int chunk = 8;
+  while (!h.done && h.iters < maxit) {
+    const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
+    for (int k = 0; k < todo; ++k) {
+      hipError_t e = apply_f64(P->dim, P->n, P->q, P->w_p.p, P->w_q.p,
+                               P->cells.p, P->active_G(), P->dB.p, P->dD.p,
+                               P->ncell, s);
+      SF_HIP(e);
+      SF_HIP(launch_cg_alpha<double>(P->w_p.p, P->w_q.p, P->nfree, red, P->scal.p, s));
+      SF_HIP(launch_cg_update<double>(x, P->w_r.p, P->w_p.p, P->w_q.p, P->nfree, red, P->scal.p, s));
+      SF_HIP(launch_cg_dir<double>(P->w_p.p, P->w_r.p, P->w_q.p, P->nfree, P->scal.p, s));
+    }
+    rc = read_scalars(P, &h, s);
+    if (rc) return rc;
+  }
+  *iters_out = h.iters;
+  *nonfinite_out = h.nonfinite;
+  return SF_OK;
+}
+
+// Inner CG for IR: solve A32 d = b32 (FP32 vectors, FP64 scalars) with
+// b32 = w_r32 of unit norm, d = 0 initially, to ||r|| <= inner_rtol or
+// maxit iterations.
+int cg_f32_inner(sf_plan* P, int precision, double inner_rtol, int maxit,
+                 hipStream_t s, int* iters_out, int* nonfinite_out) {
+  Reducer red{P->red_part.p, P->red_counter.p};
+  const float* b32 = P->w_r32.p;
+  float* r = P->w_s32.p;
+  float* d = P->w_d32.p;
+  float* p = P->w_p32.p;
+  float* q = P->w_q32.p;
+  SF_HIP(launch_fill<float>(d, P->nfree, 0.0f, s));
+  SF_HIP(launch_cg_start<float>(b32, d, r, p, q, P->nfree, red, P->scal.p,
+                                inner_rtol * inner_rtol, s));
+  CGScalars h{};
+  int rc = read_scalars(P, &h, s);
+  if (rc) return rc;
+  const int chunk = 8;
+  while (!h.done && h.iters < maxit) {
+    const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
+    for (int k = 0; k < todo; ++k) {
+      hipError_t e;
+      if (precision == SF_PREC_F32)
+        e = apply_f32(P->dim, P->n, P->q, p, q, P->cells.p, P->G32.p, P->dB32.p,
+                      P->dD32.p, P->ncell, s);
+      else
+        e = apply_f32_gbf16(P->dim, P->n, P->q, p, q, P->cells.p, P->Gbf.p,
+                            P->dB32.p, P->dD32.p, P->ncell, s);
+      SF_HIP(e);
+      SF_HIP(launch_cg_alpha<float>(p, q, P->nfree, red, P->scal.p, s));
+      SF_HIP(launch_cg_update<float>(d, r, p, q, P->nfree, red, P->scal.p, s));
+      SF_HIP(launch_cg_dir<float>(p, r, q, P->nfree, P->scal.p, s));
+    }
+    rc = read_scalars(P, &h, s);
+    if (rc) return rc;
+  }
+  *iters_out = h.iters;
+  *nonfinite_out = h.nonfinite;
+  return SF_OK;
+}
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char* sf_last_error(void) { return g_err.c_str(); }
+
+const char* sf_version(void) { return "sfhip 0.1 (gfx950)"; }
+
+int sf_tables_1d(int order, int nquad, double* B, double* Dm, double* points,
+                 double* weights) {
+  SF_TRY
+  return build_tables(order, nquad, B, Dm, points, weights);
+  SF_CATCH
+}
+
+int sf_gauss_points_for_degree(int degree) {
+  if (degree < 0) return 1;
+  return (degree + 2) / 2;  // ceil((degree + 1) / 2)
+}
+
+int64_t sf_flops_model(int64_t n_cells, int dim, int nquad, int n1d,
+                       int count_only_fma_ops, int structured_mesh_opt) {
+  // restates count_manual_flops_poisson_matrix_free, src/Utils.jl:33-72
+  const int D = dim;
+  int64_t b2 = 0;
+  for (int d = 1; d <= D; ++d) {
+    int64_t t = D;
+    for (int k = 1; k <= d; ++k) t *= nquad;
+    for (int k = d; k <= D; ++k) t *= n1d;
+    b2 += t;
+  }
+  if (!count_only_fma_ops) b2 *= 2;
+  int64_t DD = 1;
+  for (int k = 0; k < D; ++k) DD *= D;
+  int64_t b3 = 0;
+  if (!structured_mesh_opt) {
+    b3 = (D - 1) * DD + (int64_t)D * D;
+    if (!count_only_fma_ops) b3 = 2 * b3 + DD + 2 * D;
+  } else {
+    if (!count_only_fma_ops) b3 = D;
+  }
+  int64_t qd = 1;
+  for (int k = 0; k < D; ++k) qd *= nquad;
+  b3 *= qd;
+  const int64_t b4 = b2;
+  return n_cells * (b2 + b3 + b4);
+}
+
+int sf_plan_create(sf_plan** out, int dim, int order, int nquad,
+                   int64_t n_cells, int64_t n_free, int flags) {
+  SF_TRY
+  if (!out) return fail(SF_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (dim != 2 && dim != 3) return fail(SF_ERR_ARG, "dim must be 2 or 3, got %d", dim);
+  if (order < 1 || order > kMaxN - 1) return fail(SF_ERR_ARG, "order %d outside 1..%d", order, kMaxN - 1);
+  if (nquad < 1) return fail(SF_ERR_ARG, "nquad must be >= 1");
+  if (n_cells < 0 || n_free < 0) return fail(SF_ERR_ARG, "negative sizes");
+  if (n_free > 0x7fffffffLL) return fail(SF_ERR_ARG, "n_free above int32 range");
+  if (flags != 0) return fail(SF_ERR_ARG, "flags must be 0");
+  int nt = 0, eb = 0;
+  apply_shape_f64(dim, order + 1, nquad, &nt, &eb);
+  if (nt == 0)
+    return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d (q must be in n-1..n+3)", dim, order + 1, nquad);
+  sf_plan* P = new sf_plan();
+  P->dim = dim;
+  P->order = order;
+  P->n = order + 1;
+  P->q = nquad;
+  P->ng = dim * (dim + 1) / 2;
+  P->ncell = n_cells;
+  P->nfree = n_free;
+  P->nloc = dim == 3 ? (int64_t)P->n * P->n * P->n : (int64_t)P->n * P->n;
+  P->nqc = dim == 3 ? (int64_t)P->q * P->q * P->q : (int64_t)P->q * P->q;
+  if (hipGetDevice(&P->device) != hipSuccess) {
+    delete P;
+    return fail(SF_ERR_HIP, "no HIP device");
+  }
+  P->hB.resize(P->q * P->n);
+  P->hD.resize(P->q * P->n);
+  P->hW.resize(P->q);
+  P->hPts.resize(P->q);
+  int rc = build_tables(order, nquad, P->hB.data(), P->hD.data(), P->hPts.data(), P->hW.data());
+  if (rc == SF_OK) rc = upload_tables(P);
+  if (rc == SF_OK) rc = clear_flag(P);
+  if (rc != SF_OK) {
+    P->release_all();
+    delete P;
+    return rc;
+  }
+  *out = P;
+  return SF_OK;
+  SF_CATCH
+}
+
+int sf_plan_destroy(sf_plan* plan) {
+  SF_TRY
+  if (!plan) return SF_OK;
+  {
+    DeviceGuard g(plan->device);
+    plan->release_all();
+  }
+  delete plan;
+  return SF_OK;
+  SF_CATCH
+}
+
+int sf_plan_get_info(const sf_plan* P, sf_plan_info* info) {
+  SF_TRY
+  if (check_plan(P)) return SF_ERR_ARG;
+  if (!info) return fail(SF_ERR_ARG, "null info");
+  info->dim = P->dim;
+  info->order = P->order;
+  info->n1d = P->n;
+  info->q1d = P->q;
+  info->n_cells = P->ncell;
+  info->n_free = P->nfree;
+  info->n_qp = P->ncell * P->nqc;
+  info->has_cells = P->has_cells;
+  info->has_geometry = P->has_geom;
+  info->has_kappa = P->has_kappa;
+  apply_shape_f64(P->dim, P->n, P->q, &info->threads_per_block, &info->cells_per_block);
+  info->device_bytes = P->device_bytes();
+  return SF_OK;
+  SF_CATCH
+}
+
+int sf_plan_set_tables(sf_plan* P, const double* B, const double* Dm,
+                       const double* weights) {
+  SF_TRY
+  if (check_plan(P)) return SF_ERR_ARG;
+  if (!B || !Dm) return fail(SF_ERR_ARG, "null table");
+  DeviceGuard g(P->device);
+  const int64_t tn = (int64_t)P->q * P->n;
+  for (int64_t i = 0; i < tn; ++i)
+    if (!isfinite(B[i]) || !isfinite(Dm[i])) return fail(SF_ERR_ARG, "non-finite table entry");
+  P->hB.assign(B, B + tn);
+  P->hD.assign(Dm, Dm + tn);
+  if (weights) P->hW.assign(weights, weights + P->q);
+  return upload_tables(P);
+  SF_CATCH
+}
+
+int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
+                      const int32_t* local_to_tensor) {
+  SF_TRY
+  if (check_plan(P)) return SF_ERR_ARG;
+  if (!cell_dofs && P->ncell > 0) return fail(SF_ERR_ARG, "null cell_dofs");
+  if (index_base != 0 && index_base != 1) return fail(SF_ERR_ARG, "index_base must be 0 or 1");
+  DeviceGuard g(P->device);
+  const int64_t nl = P->nloc;
+  std::vector<int32_t> perm(nl);
+  if (local_to_tensor) {
+    std::vector<char> seen(nl, 0);
+    for (int64_t k = 0; k < nl; ++k) {
+      const int32_t t = local_to_tensor[k];
+      if (t < 0 || t >= nl || seen[t])
+        return fail(SF_ERR_ARG, "local_to_tensor is not a permutation of 0..%lld", (long long)nl - 1);
+      seen[t] = 1;
+      perm[k] = t;
+    }
+  } else {
+    for (int64_t k = 0; k < nl; ++k) perm[k] = (int32_t)k;
+  }
+  std::vector<int32_t> h((size_t)(P->ncell * nl));
+  const int64_t nf = P->nfree;
+  for (int64_t c = 0; c < P->ncell; ++c) {
+    const int32_t* src = cell_dofs + c * nl;
+    int32_t* dst = h.data() + c * nl;
+    for (int64_t k = 0; k < nl; ++k) {
+      const int64_t id = (int64_t)src[k];
+      int64_t v;
+      if (index_base == 1) {
+        v = id >= 1 ? id - 1 : -1;
+      } else {
+        v = id >= 0 ? id : -1;
+      }
+      if (v >= nf)
+        return fail(SF_ERR_DOF_RANGE, "cell %lld local dof %lld: id %lld outside the %lld free dofs",
+                    (long long)c, (long long)k, (long long)id, (long long)nf);
+      dst[perm[k]] = (int32_t)v;
+    }
+  }
+  SF_HIP(P->cells.alloc(P->ncell * nl));
+  if (!h.empty())
+    SF_HIP(hipMemcpy(P->cells.p, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  P->has_cells = true;
+  return SF_OK;
+  SF_CATCH
+}
+
+static int finish_geometry(sf_plan* P) {
+  P->g32_valid = P->gbf_valid = false;
+  if (P->has_kappa) {
+    SF_HIP(P->Gk.alloc(P->ncell * P->ng * P->nqc));
+    SF_HIP(launch_scale_kappa(P->G.p, P->kappa.p, P->ng, P->nqc, P->ncell, P->Gk.p, nullptr));
+    SF_HIP(hipDeviceSynchronize());
+  }
+  return SF_OK;
+}
+
+int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
+  SF_TRY
+  if (check_plan(P)) return SF_ERR_ARG;
+  if (!vertices && P->ncell > 0) return fail(SF_ERR_ARG, "null vertices");
+  DeviceGuard g(P->device);
+  const int64_t nv = P->ncell * (P->dim == 3 ? 24 : 8);
+  DevBuf<double> X;
+  SF_HIP(X.alloc(nv));
+  int rc = SF_OK;
+  do {
+    if (nv > 0) {
+      hipError_t e = hipMemcpy(X.p, vertices, nv * 8, hipMemcpyHostToDevice);
+      if (e != hipSuccess) { rc = fail(SF_ERR_HIP, "vertex upload: %s", hipGetErrorString(e)); break; }
+    }
+    hipError_t e = P->G.alloc(P->ncell * P->ng * P->nqc);
+    if (e != hipSuccess) { rc = fail(SF_ERR_HIP, "G alloc: %s", hipGetErrorString(e)); break; }
+    rc = clear_flag(P);
+    if (rc) break;
+    e = launch_geometry(P->dim, X.p, P->dPts.p, P->dW.p, P->q, P->ncell, P->G.p, P->flag.p, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) { rc = fail(SF_ERR_HIP, "geometry kernel: %s", hipGetErrorString(e)); break; }
+    int bad = 0;
+    rc = read_flag(P, &bad);
+    if (rc) break;
+    if (bad) {
+      P->has_geom = false;
+      rc = fail(SF_ERR_JACOBIAN, "det J <= 0 or non-finite at some quadrature point");
+      break;
+    }
+    P->has_geom = true;
+    rc = finish_geometry(P);
+  } while (0);
+  X.release();
+  return rc;
+  SF_CATCH
+}
+
+int sf_plan_set_geometry(sf_plan* P, const double* G) {
+  SF_TRY
+  if (check_plan(P)) return SF_ERR_ARG;
+  if (!G && P->ncell > 0) return fail(SF_ERR_ARG, "null G");
+  DeviceGuard g(P->device);
+  const int64_t n = P->ncell * P->ng * P->nqc;
+  for (int64_t i = 0; i < n; ++i)
+    if (!isfinite(G[i])) return fail(SF_ERR_ARG, "non-finite G entry at %lld", (long long)i);
+  SF_HIP(P->G.alloc(n));
+  if (n > 0) SF_HIP(hipMemcpy(P->G.p, G, n * 8, hipMemcpyHostToDevice));
+  P->has_geom = true;
+  return finish_geometry(P);
+  SF_CATCH
+}
+
+int sf_plan_get_geometry(const sf_plan* P, double* G) {
+  SF_TRY
+  if (check_plan(P)) return SF_ERR_ARG;
+  if (!P->has_geom) return fail(SF_ERR_NOT_READY, "plan has no geometry");
+  if (!G) return fail(SF_ERR_ARG, "null G");
+  DeviceGuard g(P->device);
+  const int64_t n = P->ncell * P->ng * P->nqc;
+  if (n > 0) SF_HIP(hipMemcpy(G, P->G.p, n * 8, hipMemcpyDeviceToHost));
+  return SF_OK;
+  SF_CATCH
+}
+
+int sf_plan_set_kappa(sf_plan* P, const double* kappa) {
+  SF_TRY
+  if (check_plan(P)) return SF_ERR_ARG;
+  if (!kappa && P->ncell > 0) return fail(SF_ERR_ARG, "null kappa");
+  DeviceGuard g(P->device);
+  const int64_t n = P->ncell * P->nqc;
+  for (int64_t i = 0; i < n; ++i)
+    if (!(kappa[i] > 0.0) || !isfinite(kappa[i]))
+      return fail(SF_ERR_ARG, "kappa[%lld] = %g is not positive and finite", (long long)i, kappa[i]);
+  SF_HIP(P->kappa.alloc(n));
+  if (n > 0) SF_HIP(hipMemcpy(P->kappa.p, kappa, n * 8, hipMemcpyHostToDevice));
+  P->has_kappa = true;
+  if (P->has_geom) return finish_geometry(P);
+  P->g32_valid = P->gbf_valid = false;
+  return SF_OK;
+  SF_CATCH
+}
+
+int sf_plan_prepare(sf_plan* P, int precision) {
+  SF_TRY
+  int rc = check_ready(P);
+  if (rc) return rc;
+  DeviceGuard g(P->device);
+  return prepare(P, precision);
+  SF_CATCH
+}
+
+int sf_apply(sf_plan* P, double* y, const double* x, void* stream) {
+  SF_TRY
+  int rc = check_ready(P);
+  if (rc) return rc;
+  if ((!x || !y) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
+  DeviceGuard g(P->device);
+  return do_apply_f64(P, y, x, (hipStream_t)stream);
+  SF_CATCH
+}
+
+int sf_apply_add(sf_plan* P, double* y, const double* x, void* stream) {
+  SF_TRY
+  int rc = check_ready(P);
+  if (rc) return rc;
+  if ((!x || !y) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
+  DeviceGuard g(P->device);
+  hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
+                           P->dB.p, P->dD.p, P->ncell, (hipStream_t)stream);
+  if (e == hipErrorNotSupported)
+    return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
+  SF_HIP(e);
+  return SF_OK;
+  SF_CATCH
+}
+
+int sf_apply_host(sf_plan* P, double* y, const double* x) {
+  SF_TRY
+  int rc = check_ready(P);
+  if (rc) return rc;
+  if ((!x || !y) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
+  DeviceGuard g(P->device);
+  SF_HIP(P->xs.alloc(P->nfree));
+  SF_HIP(P->ys.alloc(P->nfree));
+  if (P->nfree > 0)
+    SF_HIP(hipMemcpy(P->xs.p, x, P->nfree * 8, hipMemcpyHostToDevice));
+  rc = do_apply_f64(P, P->ys.p, P->xs.p, nullptr);
+  if (rc) return rc;
+  if (P->nfree > 0)
+    SF_HIP(hipMemcpy(y, P->ys.p, P->nfree * 8, hipMemcpyDeviceToHost));
+  SF_HIP(hipDeviceSynchronize());
+  return SF_OK;
+  SF_CATCH
+}
+
+int sf_apply_f32(sf_plan* P, float* y, const float* x, int precision,
+                 void* stream) {
+  SF_TRY
+  int rc = check_ready(P);
+  if (rc) return rc;
+  if ((!x || !y) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
+  if (precision != SF_PREC_F32 && precision != SF_PREC_F32_GBF16 && precision != SF_PREC_MFMA_BF16)
+    return fail(SF_ERR_ARG, "precision %d is not a reduced-precision operator", precision);
+  DeviceGuard g(P->device);
+  rc = prepare(P, precision);
+  if (rc) return rc;
+  return do_apply_f32(P, y, x, precision, (hipStream_t)stream);
+  SF_CATCH
+}
+
+int sf_adjoint(sf_plan* P, const double* u, const double* lam,
+               double* dJdkappa, double* dJdu, void* stream) {
+  SF_TRY
+  int rc = check_ready(P);
+  if (rc) return rc;
+  if ((!u || !lam) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
+  DeviceGuard g(P->device);
+  hipStream_t s = (hipStream_t)stream;
+  if (dJdu) SF_HIP(launch_fill<double>(dJdu, P->nfree, 0.0, s));
+  hipError_t e = adjoint_f64(P->dim, P->n, P->q, u, lam, dJdkappa, dJdu,
+                             P->cells.p, P->G.p, P->has_kappa ? P->kappa.p : nullptr,
+                             P->dB.p, P->dD.p, P->ncell, s);
+  if (e == hipErrorNotSupported)
+    return fail(SF_ERR_UNSUPPORTED, "no adjoint kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
+  SF_HIP(e);
+  return SF_OK;
+  SF_CATCH
+}
+
+int sf_cg(sf_plan* P, const double* b, double* x, double rtol, int maxit,
+          sf_solve_stats* stats, void* stream) {
+  SF_TRY
+  int rc = check_ready(P);
+  if (rc) return rc;
+  if ((!b || !x) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
+  if (!(rtol >= 0.0) || maxit < 0) return fail(SF_ERR_ARG, "bad rtol / maxit");
+  DeviceGuard g(P->device);
+  hipStream_t s = (hipStream_t)stream;
+  rc = ensure_solver_ws(P, false);
+  if (rc) return rc;
+  SF_HIP(hipStreamSynchronize(s));
+  const double t0 = now_s();
+  int iters = 0, nonfinite = 0;
+  rc = cg_f64(P, b, x, rtol, maxit, s, &iters, &nonfinite);
+  if (rc) return rc;
+  // true residual
+  rc = do_apply_f64(P, P->w_t.p, x, s);
+  if (rc) return rc;
+  SF_HIP(launch_residual(b, P->w_t.p, P->w_r.p, P->nfree, s));
+  double rr = 0.0, bb = 0.0;
+  rc = dot_host(P, P->w_r.p, P->w_r.p, &rr, s);
+  if (rc) return rc;
+  rc = dot_host(P, b, b, &bb, s);
+  if (rc) return rc;
+  const double t1 = now_s();
+  const double rel = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
+  if (stats) {
+    stats->iterations = iters;
+    stats->outer_iterations = 0;
+    stats->rel_residual = rel;
+    stats->converged = (rel <= rtol) ? 1 : 0;
+    stats->seconds = t1 - t0;
+  }
+  if (nonfinite || !isfinite(rel)) return fail(SF_ERR_NONFINITE, "CG met a non-finite value");
+  if (rel > rtol && iters >= maxit)
+    return fail(SF_ERR_NOT_CONVERGED, "CG did not reach rtol %g in %d iterations (rel. residual %g)", rtol, maxit, rel);
+  return SF_OK;
+  SF_CATCH
+}
+
+int sf_cg_ir(sf_plan* P, const double* b, double* x, double rtol,
+             int max_outer, int inner_precision, double inner_rtol,
+             int inner_maxit, sf_solve_stats* stats, void* stream) {
+  SF_TRY
+  int rc = check_ready(P);
+  if (rc) return rc;
+  if ((!b || !x) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
+  if (!(rtol >= 0.0) || max_outer < 0 || !(inner_rtol > 0.0) || inner_maxit < 1)
+    return fail(SF_ERR_ARG, "bad tolerances / iteration limits");
+  if (inner_precision != SF_PREC_F32 && inner_precision != SF_PREC_F32_GBF16)
+    return fail(SF_ERR_UNSUPPORTED, "inner precision %d not available for refinement", inner_precision);
+  DeviceGuard g(P->device);
+  hipStream_t s = (hipStream_t)stream;
+  rc = prepare(P, inner_precision);
+  if (rc) return rc;
+  rc = ensure_solver_ws(P, true);
+  if (rc) return rc;
+  SF_HIP(hipStreamSynchronize(s));
+  const double t0 = now_s();
+  double bb = 0.0;
+  rc = dot_host(P, b, b, &bb, s);
+  if (rc) return rc;
+  const double bnorm = sqrt(bb);
+  int inner_total = 0, outer = 0, nonfinite = 0;
+  double rel = 0.0;
+  for (;;) {
+    // FP64 residual r = b - A x
+    rc = do_apply_f64(P, P->w_t.p, x, s);
+    if (rc) return rc;
+    SF_HIP(launch_residual(b, P->w_t.p, P->w_r.p, P->nfree, s));
+    double rr = 0.0;
+    rc = dot_host(P, P->w_r.p, P->w_r.p, &rr, s);
+    if (rc) return rc;
+    const double rn = sqrt(rr);
+    rel = bnorm > 0 ? rn / bnorm : rn;
+    if (!isfinite(rel)) { nonfinite = 1; break; }
+    if (rel <= rtol || outer >= max_outer || rn == 0.0) break;
+    // inner solve on the unit-norm residual in reduced precision
+    SF_HIP(launch_scale_to_f32(P->w_r.p, 1.0 / rn, P->w_r32.p, P->nfree, s));
+    int it = 0, nf = 0;
+    rc = cg_f32_inner(P, inner_precision, inner_rtol, inner_maxit, s, &it, &nf);
+    if (rc) return rc;
+    inner_total += it;
+    ++outer;
+    if (nf) { nonfinite = 1; break; }
+    SF_HIP(launch_axpy_f32_to_f64(x, P->w_d32.p, rn, P->nfree, s));
+  }
+  SF_HIP(hipStreamSynchronize(s));
+  const double t1 = now_s();
+  if (stats) {
+    stats->iterations = inner_total;
+    stats->outer_iterations = outer;
+    stats->rel_residual = rel;
+    stats->converged = (rel <= rtol) ? 1 : 0;
+    stats->seconds = t1 - t0;
+  }
+  if (nonfinite) return fail(SF_ERR_NONFINITE, "refinement met a non-finite value");
+  if (rel > rtol)
+    return fail(SF_ERR_NOT_CONVERGED, "refinement did not reach rtol %g in %d outer steps (rel. residual %g)", rtol, max_outer, rel);
+  return SF_OK;
+  SF_CATCH
+}
+
+}  // extern "C"

@@ -1,0 +1,461 @@
+// sf_apply.hpp -- the matrix-free, sum-factorized Laplacian apply kernels.
+//
+// One workgroup processes EB cells.  Per cell the work is the reference's
+// Arrays.evaluate! pipeline (src/SumFactorizationMaps.jl:78-89): gather,
+// dof->quadrature contractions (src/SumFactorizationKernels.jl:40-65),
+// geometric factor (:174-216), quadrature->dof contractions (:105-130),
+// scatter-add (the assembly of src/LazyMatrices.jl:57-58).
+//
+// The contraction schedule is not the reference's D x D one (each gradient
+// component interpolated separately in every direction).  Directions share
+// partial results: the x-pencil pass produces both B u and D u, the y pass
+// forms three tensors from those two, the z pass three gradient components.
+// In 3D that is 2N^3Q + 3N^2Q^2 + 3NQ^3 FMAs per direction of travel against
+// the reference's 3(N^3Q + N^2Q^2 + NQ^3), and each thread keeps a whole 1D
+// pencil in registers, so every LDS value it reads feeds Q (or N) FMAs.
+//
+// The 1D tables are read through uniform (scalar) loads: every lane uses the
+// same B[i][j] at the same time, so they live in SGPRs / the scalar cache and
+// cost no LDS bandwidth.  LDS holds only the per-cell intermediate tensors.
+//
+// LDS per cell (3D):  A = 2 tensors N x N x QP   (x-pass output, and the
+//                                                 y^T-pass output later)
+//                     B = 3 tensors N x Q x Q    (y-pass output, rewritten in
+//                                                 place by the z column pass)
+// QP = Q rounded up to an odd number, so the x-pencil rows written by
+// consecutive lanes (stride QP doubles) fall into distinct LDS banks.
+#pragma once
+
+#include "sf_common.hpp"
+
+namespace sf {
+
+template <int N, int Q, typename T>
+struct Apply3Shape {
+  static constexpr int NN = N * N, NQ = N * Q, QQ = Q * Q, QQQ = Q * Q * Q;
+  static constexpr int QP = (Q % 2 == 0) ? Q + 1 : Q;
+  static constexpr int SA1 = NN * QP;  // one x-pass tensor [z][y][i]
+  static constexpr int SB1 = N * QQ;   // one y-pass tensor [z][k][i]
+  static constexpr int SE = 2 * SA1 + 3 * SB1;
+  static constexpr int MAXT = cmax(NN, cmax(NQ, QQ));
+  // cells per workgroup: fill ~256 lanes in the widest stage, within 64 KiB
+  static constexpr int EB_T = MAXT > 100 ? 1 : cmax(1, 256 / MAXT);
+  static constexpr int EB_L = cmax(1, 65536 / (SE * (int)sizeof(T)));
+  static constexpr int EB = EB_T < EB_L ? EB_T : EB_L;
+  static constexpr int NT = round_up(EB * MAXT, 64);
+  static constexpr int LDS_BYTES = EB * SE * (int)sizeof(T);
+  static_assert(LDS_BYTES <= 65536, "apply3d: LDS per workgroup above 64 KiB");
+};
+
+// y += A x restricted to the cells of this launch (y must be zeroed by the
+// caller; contributions are added with hardware atomics).
+template <int N, int Q, typename T, typename TG>
+__global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
+    k_apply3d(const T* __restrict__ x, T* __restrict__ y,
+              const int32_t* __restrict__ cells, const TG* __restrict__ G,
+              const T* __restrict__ tB, const T* __restrict__ tD,
+              int64_t ncell, int nblk) {
+  using S = Apply3Shape<N, Q, T>;
+  __shared__ T smem[S::EB * S::SE];
+  const int blk = xcd_remap(blockIdx.x, nblk);
+  const int64_t base = (int64_t)blk * S::EB;
+  const int tid = threadIdx.x;
+
+  // ---- stage 1: gather + x-pencils:  Bx u, Dx u ------------------------
+  for (int t = tid; t < S::EB * S::NN; t += S::NT) {
+    const int e = t / S::NN, r = t - e * S::NN;  // r = z*N + y
+    const int64_t ge = base + e;
+    T u[N];
+    if (ge < ncell) {
+      const int32_t* ids = cells + (ge * S::NN + r) * N;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const int id = ids[j];
+        u[j] = id >= 0 ? x[id] : T(0);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) u[j] = T(0);
+    }
+    T* sA = smem + e * S::SE;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      T b = T(0), d = T(0);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        b = fma(tB[i * N + j], u[j], b);
+        d = fma(tD[i * N + j], u[j], d);
+      }
+      sA[r * S::QP + i] = b;
+      sA[S::SA1 + r * S::QP + i] = d;
+    }
+  }
+  __syncthreads();
+
+  // ---- stage 2: y-pencils: (Dx u, Bx u) -> By Dx u, Dy Bx u, By Bx u ------
+  for (int t = tid; t < S::EB * S::NQ; t += S::NT) {
+    const int e = t / S::NQ, r = t - e * S::NQ;
+    const int z = r / Q, i = r - z * Q;
+    const T* sA = smem + e * S::SE;
+    T* sB = smem + e * S::SE + 2 * S::SA1;
+    T a[N], d[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      a[j] = sA[(z * N + j) * S::QP + i];
+      d[j] = sA[S::SA1 + (z * N + j) * S::QP + i];
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      T bb = T(0), bd = T(0), db = T(0);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        bb = fma(tB[k * N + j], a[j], bb);
+        bd = fma(tD[k * N + j], a[j], bd);
+        db = fma(tB[k * N + j], d[j], db);
+      }
+      const int o = z * S::QQ + k * Q + i;
+      sB[o] = db;               // x-gradient path
+      sB[S::SB1 + o] = bd;      // y-gradient path
+      sB[2 * S::SB1 + o] = bb;  // z-gradient path
+    }
+  }
+  __syncthreads();
+
+  // ---- stage 3: z-columns: gradients, G, transposed z contraction --------
+  for (int t = tid; t < S::EB * S::QQ; t += S::NT) {
+    const int e = t / S::QQ, r = t - e * S::QQ;  // r = k*Q + i
+    const int64_t ge = base + e;
+    T* sB = smem + e * S::SE + 2 * S::SA1;
+    T cx[N], cy[N], cz[N];
+#pragma unroll
+    for (int z = 0; z < N; ++z) {
+      cx[z] = sB[z * S::QQ + r];
+      cy[z] = sB[S::SB1 + z * S::QQ + r];
+      cz[z] = sB[2 * S::SB1 + z * S::QQ + r];
+    }
+    T tx[Q], ty[Q], tz[Q];
+    const bool valid = ge < ncell;
+    const TG* g = G + ge * 6 * S::QQQ + r;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      T gx = T(0), gy = T(0), gz = T(0);
+#pragma unroll
+      for (int z = 0; z < N; ++z) {
+        gx = fma(tB[l * N + z], cx[z], gx);
+        gy = fma(tB[l * N + z], cy[z], gy);
+        gz = fma(tD[l * N + z], cz[z], gz);
+      }
+      T g0 = T(0), g1 = T(0), g2 = T(0), g3 = T(0), g4 = T(0), g5 = T(0);
+      if (valid) {
+        const TG* gl = g + l * S::QQ;
+        g0 = load_g<TG, T>(gl + 0 * S::QQQ);
+        g1 = load_g<TG, T>(gl + 1 * S::QQQ);
+        g2 = load_g<TG, T>(gl + 2 * S::QQQ);
+        g3 = load_g<TG, T>(gl + 3 * S::QQQ);
+        g4 = load_g<TG, T>(gl + 4 * S::QQQ);
+        g5 = load_g<TG, T>(gl + 5 * S::QQQ);
+      }
+      tx[l] = g0 * gx + g1 * gy + g2 * gz;
+      ty[l] = g1 * gx + g3 * gy + g4 * gz;
+      tz[l] = g2 * gx + g4 * gy + g5 * gz;
+    }
+#pragma unroll
+    for (int z = 0; z < N; ++z) {
+      T rx = T(0), ry = T(0), rz = T(0);
+#pragma unroll
+      for (int l = 0; l < Q; ++l) {
+        rx = fma(tB[l * N + z], tx[l], rx);
+        ry = fma(tB[l * N + z], ty[l], ry);
+        rz = fma(tD[l * N + z], tz[l], rz);
+      }
+      sB[z * S::QQ + r] = rx;
+      sB[S::SB1 + z * S::QQ + r] = ry;
+      sB[2 * S::SB1 + z * S::QQ + r] = rz;
+    }
+  }
+  __syncthreads();
+
+  // ---- stage 4: transposed y-pencils --------------------------------------
+  for (int t = tid; t < S::EB * S::NQ; t += S::NT) {
+    const int e = t / S::NQ, r = t - e * S::NQ;
+    const int z = r / Q, i = r - z * Q;
+    T* sA = smem + e * S::SE;
+    const T* sB = smem + e * S::SE + 2 * S::SA1;
+    T rx[Q], ry[Q], rz[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      const int o = z * S::QQ + k * Q + i;
+      rx[k] = sB[o];
+      ry[k] = sB[S::SB1 + o];
+      rz[k] = sB[2 * S::SB1 + o];
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      T sx = T(0), so = T(0);
+#pragma unroll
+      for (int k = 0; k < Q; ++k) {
+        sx = fma(tB[k * N + j], rx[k], sx);
+        so = fma(tD[k * N + j], ry[k], so);
+        so = fma(tB[k * N + j], rz[k], so);
+      }
+      sA[(z * N + j) * S::QP + i] = sx;           // then D_x^T
+      sA[S::SA1 + (z * N + j) * S::QP + i] = so;  // then B_x^T
+    }
+  }
+  __syncthreads();
+
+  // ---- stage 5: transposed x-pencils + scatter-add -------------------------
+  for (int t = tid; t < S::EB * S::NN; t += S::NT) {
+    const int e = t / S::NN, r = t - e * S::NN;
+    const int64_t ge = base + e;
+    if (ge >= ncell) continue;
+    const T* sA = smem + e * S::SE;
+    T sx[Q], so[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      sx[i] = sA[r * S::QP + i];
+      so[i] = sA[S::SA1 + r * S::QP + i];
+    }
+    const int32_t* ids = cells + (ge * S::NN + r) * N;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      T v = T(0);
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        v = fma(tD[i * N + m], sx[i], v);
+        v = fma(tB[i * N + m], so[i], v);
+      }
+      const int id = ids[m];
+      if (id >= 0) atomic_add(y + id, v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2D: per cell  A = 2 tensors N x QP  ([c][y][i]); the column pass works in
+// place (each thread owns one quadrature column i).
+template <int N, int Q, typename T>
+struct Apply2Shape {
+  static constexpr int NQ = N * Q, QQ = Q * Q;
+  static constexpr int QP = (Q % 2 == 0) ? Q + 1 : Q;
+  static constexpr int SA1 = N * QP;
+  static constexpr int SE = 2 * SA1;
+  static constexpr int MAXT = cmax(N, Q);
+  static constexpr int EB = cmax(1, 256 / MAXT);
+  static constexpr int NT = round_up(EB * MAXT, 64);
+};
+
+template <int N, int Q, typename T, typename TG>
+__global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
+    k_apply2d(const T* __restrict__ x, T* __restrict__ y,
+              const int32_t* __restrict__ cells, const TG* __restrict__ G,
+              const T* __restrict__ tB, const T* __restrict__ tD,
+              int64_t ncell, int nblk) {
+  using S = Apply2Shape<N, Q, T>;
+  __shared__ T smem[S::EB * S::SE];
+  const int blk = xcd_remap(blockIdx.x, nblk);
+  const int64_t base = (int64_t)blk * S::EB;
+  const int tid = threadIdx.x;
+
+  // stage 1: gather + x-pencils (one per node row y)
+  for (int t = tid; t < S::EB * N; t += S::NT) {
+    const int e = t / N, r = t - e * N;
+    const int64_t ge = base + e;
+    T u[N];
+    if (ge < ncell) {
+      const int32_t* ids = cells + (ge * N + r) * N;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const int id = ids[j];
+        u[j] = id >= 0 ? x[id] : T(0);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) u[j] = T(0);
+    }
+    T* sA = smem + e * S::SE;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      T b = T(0), d = T(0);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        b = fma(tB[i * N + j], u[j], b);
+        d = fma(tD[i * N + j], u[j], d);
+      }
+      sA[r * S::QP + i] = b;
+      sA[S::SA1 + r * S::QP + i] = d;
+    }
+  }
+  __syncthreads();
+
+  // stage 2: y-columns: gradients, G, transposed y contraction (in place)
+  for (int t = tid; t < S::EB * Q; t += S::NT) {
+    const int e = t / Q, i = t - e * Q;
+    const int64_t ge = base + e;
+    T* sA = smem + e * S::SE;
+    T a[N], d[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      a[j] = sA[j * S::QP + i];
+      d[j] = sA[S::SA1 + j * S::QP + i];
+    }
+    const bool valid = ge < ncell;
+    const TG* g = G + ge * 3 * S::QQ + i;
+    T tx[Q], ty[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      T gx = T(0), gy = T(0);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        gx = fma(tB[k * N + j], d[j], gx);
+        gy = fma(tD[k * N + j], a[j], gy);
+      }
+      T g0 = T(0), g1 = T(0), g2 = T(0);
+      if (valid) {
+        g0 = load_g<TG, T>(g + 0 * S::QQ + k * Q);
+        g1 = load_g<TG, T>(g + 1 * S::QQ + k * Q);
+        g2 = load_g<TG, T>(g + 2 * S::QQ + k * Q);
+      }
+      tx[k] = g0 * gx + g1 * gy;
+      ty[k] = g1 * gx + g2 * gy;
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      T rx = T(0), ry = T(0);
+#pragma unroll
+      for (int k = 0; k < Q; ++k) {
+        rx = fma(tB[k * N + j], tx[k], rx);
+        ry = fma(tD[k * N + j], ty[k], ry);
+      }
+      sA[j * S::QP + i] = rx;            // then D_x^T
+      sA[S::SA1 + j * S::QP + i] = ry;   // then B_x^T
+    }
+  }
+  __syncthreads();
+
+  // stage 3: transposed x-pencils + scatter-add
+  for (int t = tid; t < S::EB * N; t += S::NT) {
+    const int e = t / N, r = t - e * N;
+    const int64_t ge = base + e;
+    if (ge >= ncell) continue;
+    const T* sA = smem + e * S::SE;
+    T sx[Q], so[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      sx[i] = sA[r * S::QP + i];
+      so[i] = sA[S::SA1 + r * S::QP + i];
+    }
+    const int32_t* ids = cells + (ge * N + r) * N;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      T v = T(0);
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        v = fma(tD[i * N + m], sx[i], v);
+        v = fma(tB[i * N + m], so[i], v);
+      }
+      const int id = ids[m];
+      if (id >= 0) atomic_add(y + id, v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers: one (n, q) -> template switch per (T, TG).
+
+template <typename T, typename TG>
+struct ApplyArgs {
+  const T* x;
+  T* y;
+  const int32_t* cells;
+  const TG* G;
+  const T* tB;
+  const T* tD;
+  int64_t ncell;
+};
+
+template <int N, int Q, typename T, typename TG>
+hipError_t launch_apply3d_nq(const ApplyArgs<T, TG>& a, hipStream_t s) {
+  using S = Apply3Shape<N, Q, T>;
+  const int64_t nblk64 = (a.ncell + S::EB - 1) / S::EB;
+  if (nblk64 <= 0) return hipSuccess;
+  if (nblk64 > 0x7fffffff) return hipErrorInvalidValue;
+  const int nblk = (int)nblk64;
+  hipLaunchKernelGGL((k_apply3d<N, Q, T, TG>), dim3(nblk), dim3(S::NT), 0, s,
+                     a.x, a.y, a.cells, a.G, a.tB, a.tD, a.ncell, nblk);
+  return hipGetLastError();
+}
+
+template <int N, int Q, typename T, typename TG>
+hipError_t launch_apply2d_nq(const ApplyArgs<T, TG>& a, hipStream_t s) {
+  using S = Apply2Shape<N, Q, T>;
+  const int64_t nblk64 = (a.ncell + S::EB - 1) / S::EB;
+  if (nblk64 <= 0) return hipSuccess;
+  if (nblk64 > 0x7fffffff) return hipErrorInvalidValue;
+  const int nblk = (int)nblk64;
+  hipLaunchKernelGGL((k_apply2d<N, Q, T, TG>), dim3(nblk), dim3(S::NT), 0, s,
+                     a.x, a.y, a.cells, a.G, a.tB, a.tD, a.ncell, nblk);
+  return hipGetLastError();
+}
+
+// Returns hipErrorNotSupported for an (n, q) pair outside SF_FOR_EACH_NQ.
+template <typename T, typename TG>
+hipError_t launch_apply(int dim, int n, int q, const ApplyArgs<T, TG>& a,
+                        hipStream_t s) {
+  if (dim == 3) {
+    switch (n * 16 + q) {
+#define SF_CASE(N_, Q_) \
+  case N_ * 16 + Q_:    \
+    return launch_apply3d_nq<N_, Q_, T, TG>(a, s);
+      SF_FOR_EACH_NQ(SF_CASE)
+#undef SF_CASE
+      default:
+        return hipErrorNotSupported;
+    }
+  } else if (dim == 2) {
+    switch (n * 16 + q) {
+#define SF_CASE(N_, Q_) \
+  case N_ * 16 + Q_:    \
+    return launch_apply2d_nq<N_, Q_, T, TG>(a, s);
+      SF_FOR_EACH_NQ(SF_CASE)
+#undef SF_CASE
+      default:
+        return hipErrorNotSupported;
+    }
+  }
+  return hipErrorNotSupported;
+}
+
+// Launch shape of the 3D (or 2D) apply for (n, q): threads and cells per
+// workgroup (0, 0 if unsupported).
+inline void apply_shape(int dim, int n, int q, int* nt, int* eb) {
+  *nt = 0;
+  *eb = 0;
+  if (dim == 3) {
+    switch (n * 16 + q) {
+#define SF_CASE(N_, Q_)                         \
+  case N_ * 16 + Q_:                            \
+    *nt = Apply3Shape<N_, Q_, double>::NT;      \
+    *eb = Apply3Shape<N_, Q_, double>::EB;      \
+    return;
+      SF_FOR_EACH_NQ(SF_CASE)
+#undef SF_CASE
+      default:
+        return;
+    }
+  } else if (dim == 2) {
+    switch (n * 16 + q) {
+#define SF_CASE(N_, Q_)                         \
+  case N_ * 16 + Q_:                            \
+    *nt = Apply2Shape<N_, Q_, double>::NT;      \
+    *eb = Apply2Shape<N_, Q_, double>::EB;      \
+    return;
+      SF_FOR_EACH_NQ(SF_CASE)
+#undef SF_CASE
+      default:
+        return;
+    }
+  }
+}
+
+}  // namespace sf

@@ -1,0 +1,16 @@
+// sf_apply_f32.hip -- FP32 instantiations of the apply kernels: the inner
+// operator of the mixed-precision solver (FP32 vectors, tables, G and
+// vector-ALU contractions).
+#include "sf_apply.hpp"
+#include "sf_kernels.h"
+
+namespace sf {
+
+hipError_t apply_f32(int dim, int n, int q, const float* x, float* y,
+                     const int32_t* cells, const float* G, const float* tB,
+                     const float* tD, int64_t ncell, hipStream_t s) {
+  ApplyArgs<float, float> a{x, y, cells, G, tB, tD, ncell};
+  return launch_apply<float, float>(dim, n, q, a, s);
+}
+
+}  // namespace sf

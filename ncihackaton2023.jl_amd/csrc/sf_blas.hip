@@ -1,0 +1,319 @@
+// sf_blas.hip -- BLAS-1 pieces of the device-resident CG / iterative
+// refinement.  Every reduction accumulates in FP64 (also for FP32 vectors)
+// and finishes in the last workgroup to arrive (no second launch, no host
+// round trip); the CG scalars alpha, beta, r.r stay in device memory.
+#include "sf_common.hpp"
+#include "sf_kernels.h"
+
+namespace sf {
+
+namespace {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block sum over kRedThreads threads; result valid on thread 0.
+__device__ double block_sum(double v) {
+  __shared__ double ws[kRedThreads / 64];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();  // ws may still be read by a previous call
+  if (lane == 0) ws[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < kRedThreads / 64; ++w) s += ws[w];
+  }
+  return s;
+}
+
+// Publish this block's partial; true in every thread of the last block.
+__device__ bool arrive(Reducer red, double partial) {
+  __shared__ int am_last;
+  if (threadIdx.x == 0) {
+    red.part[blockIdx.x] = partial;
+    __threadfence();
+    const unsigned prev = atomicAdd(red.counter, 1u);
+    am_last = (prev == gridDim.x - 1);
+    if (am_last) __threadfence();
+  }
+  __syncthreads();
+  return am_last != 0;
+}
+
+// Sum of all partials, called by every thread of the last block; valid on
+// thread 0, which also resets the arrival counter.
+__device__ double gather_partials(Reducer red) {
+  double s = 0.0;
+  for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+    s += __hip_atomic_load(red.part + i, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  s = block_sum(s);
+  if (threadIdx.x == 0) *red.counter = 0u;
+  return s;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256) k_fill(V* __restrict__ a, int64_t n,
+                                              V v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kRedThreads)
+    k_dot(const V* __restrict__ a, const V* __restrict__ b, int64_t n,
+          Reducer red, double* __restrict__ out) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += (double)a[i] * (double)b[i];
+  s = block_sum(s);
+  if (arrive(red, s)) {
+    const double tot = gather_partials(red);
+    if (threadIdx.x == 0) *out = tot;
+  }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kRedThreads)
+    k_cg_start(const V* __restrict__ b, const V* __restrict__ Ax,
+               V* __restrict__ r, V* __restrict__ p, V* __restrict__ q,
+               int64_t n, Reducer red, CGScalars* __restrict__ sc,
+               double stop_rr) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const V ri = b[i] - Ax[i];
+    r[i] = ri;
+    p[i] = ri;
+    q[i] = V(0);
+    s += (double)ri * (double)ri;
+  }
+  s = block_sum(s);
+  if (arrive(red, s)) {
+    const double rr = gather_partials(red);
+    if (threadIdx.x == 0) {
+      sc->rr = rr;
+      sc->rr_new = rr;
+      sc->pq = 0.0;
+      sc->alpha = 0.0;
+      sc->beta = 0.0;
+      sc->stop_rr = stop_rr;
+      sc->iters = 0;
+      sc->nonfinite = isfinite(rr) ? 0 : 1;
+      sc->done = (rr <= stop_rr || !isfinite(rr)) ? 1 : 0;
+    }
+  }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kRedThreads)
+    k_cg_alpha(const V* __restrict__ p, const V* __restrict__ q, int64_t n,
+               Reducer red, CGScalars* __restrict__ sc) {
+  if (sc->done) return;  // uniform over the grid: set only by other kernels
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += (double)p[i] * (double)q[i];
+  s = block_sum(s);
+  if (arrive(red, s)) {
+    const double pq = gather_partials(red);
+    if (threadIdx.x == 0) {
+      sc->pq = pq;
+      const double a = sc->rr / pq;
+      sc->alpha = a;
+      if (!isfinite(a)) {
+        sc->nonfinite = 1;
+        sc->done = 1;
+      }
+    }
+  }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kRedThreads)
+    k_cg_update(V* __restrict__ x, V* __restrict__ r,
+                const V* __restrict__ p, const V* __restrict__ q, int64_t n,
+                Reducer red, CGScalars* __restrict__ sc) {
+  if (sc->done) return;
+  const V a = (V)sc->alpha;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] = fma(a, p[i], x[i]);
+    const V ri = fma(-a, q[i], r[i]);
+    r[i] = ri;
+    s += (double)ri * (double)ri;
+  }
+  s = block_sum(s);
+  if (arrive(red, s)) {
+    const double rr_new = gather_partials(red);
+    if (threadIdx.x == 0) {
+      sc->rr_new = rr_new;
+      sc->beta = rr_new / sc->rr;
+      sc->rr = rr_new;
+      sc->iters += 1;
+      if (!isfinite(rr_new)) {
+        sc->nonfinite = 1;
+        sc->done = 1;
+      } else if (rr_new <= sc->stop_rr) {
+        sc->done = 1;
+      }
+    }
+  }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256)
+    k_cg_dir(V* __restrict__ p, const V* __restrict__ r, V* __restrict__ q,
+             int64_t n, const CGScalars* __restrict__ sc) {
+  if (sc->done) return;
+  const V b = (V)sc->beta;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    p[i] = fma(b, p[i], r[i]);
+    q[i] = V(0);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_residual(const double* __restrict__ b, const double* __restrict__ Ax,
+               double* __restrict__ r, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    r[i] = b[i] - Ax[i];
+}
+
+__global__ void __launch_bounds__(256)
+    k_scale_to_f32(const double* __restrict__ r, double s,
+                   float* __restrict__ r32, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    r32[i] = (float)(r[i] * s);
+}
+
+__global__ void __launch_bounds__(256)
+    k_axpy_f32_to_f64(double* __restrict__ x, const float* __restrict__ d,
+                      double s, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = fma(s, (double)d[i], x[i]);
+}
+
+inline unsigned stream_grid(int64_t n) {
+  // Enough workgroups to fill 256 CUs several times over; grid-stride loops
+  // cover the rest.
+  int64_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+inline unsigned red_grid(int64_t n) {
+  int64_t g = (n + kRedThreads - 1) / kRedThreads;
+  if (g > kRedBlocks) g = kRedBlocks;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+template <typename V>
+hipError_t launch_fill(V* a, int64_t n, V v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill<V>, dim3(stream_grid(n)), dim3(256), 0, s, a, n,
+                     v);
+  return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_dot(const V* a, const V* b, int64_t n, Reducer red,
+                      double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot<V>, dim3(red_grid(n)), dim3(kRedThreads), 0, s, a,
+                     b, n, red, out);
+  return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_cg_start(const V* b, const V* Ax, V* r, V* p, V* q,
+                           int64_t n, Reducer red, CGScalars* sc,
+                           double stop_rr, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_start<V>, dim3(red_grid(n)), dim3(kRedThreads), 0, s,
+                     b, Ax, r, p, q, n, red, sc, stop_rr);
+  return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_cg_alpha(const V* p, const V* q, int64_t n, Reducer red,
+                           CGScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_alpha<V>, dim3(red_grid(n)), dim3(kRedThreads), 0, s,
+                     p, q, n, red, sc);
+  return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_cg_update(V* x, V* r, const V* p, const V* q, int64_t n,
+                            Reducer red, CGScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_update<V>, dim3(red_grid(n)), dim3(kRedThreads), 0,
+                     s, x, r, p, q, n, red, sc);
+  return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_cg_dir(V* p, const V* r, V* q, int64_t n, CGScalars* sc,
+                         hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cg_dir<V>, dim3(stream_grid(n)), dim3(256), 0, s, p, r,
+                     q, n, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_residual(const double* b, const double* Ax, double* r,
+                           int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_residual, dim3(stream_grid(n)), dim3(256), 0, s, b, Ax,
+                     r, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_to_f32(const double* r, double inv_scale, float* r32,
+                               int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scale_to_f32, dim3(stream_grid(n)), dim3(256), 0, s, r,
+                     inv_scale, r32, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy_f32_to_f64(double* x, const float* d32, double scale,
+                                  int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_axpy_f32_to_f64, dim3(stream_grid(n)), dim3(256), 0, s,
+                     x, d32, scale, n);
+  return hipGetLastError();
+}
+
+#define SF_INST(V)                                                            \
+  template hipError_t launch_fill<V>(V*, int64_t, V, hipStream_t);            \
+  template hipError_t launch_dot<V>(const V*, const V*, int64_t, Reducer,     \
+                                    double*, hipStream_t);                    \
+  template hipError_t launch_cg_start<V>(const V*, const V*, V*, V*, V*,      \
+                                         int64_t, Reducer, CGScalars*, double, \
+                                         hipStream_t);                        \
+  template hipError_t launch_cg_alpha<V>(const V*, const V*, int64_t,         \
+                                         Reducer, CGScalars*, hipStream_t);   \
+  template hipError_t launch_cg_update<V>(V*, V*, const V*, const V*,         \
+                                          int64_t, Reducer, CGScalars*,       \
+                                          hipStream_t);                       \
+  template hipError_t launch_cg_dir<V>(V*, const V*, V*, int64_t,             \
+                                       CGScalars*, hipStream_t);
+SF_INST(double)
+SF_INST(float)
+#undef SF_INST
+
+}  // namespace sf

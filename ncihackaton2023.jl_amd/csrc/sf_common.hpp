@@ -1,0 +1,90 @@
+// sf_common.hpp -- shared definitions of the sfhip device code (gfx950).
+//
+// Launch shapes, the supported (n, q) list, the workgroup -> XCD remap and
+// small device helpers used by every kernel family.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sf {
+
+// Supported (n = p + 1, q) pairs: p = 1..9 and q = n-1 .. n+3 (q >= 1).
+// This covers the reference's own choices (q = p for quadrature degree 2p-1
+// or 2(p-1), test/benchmark_v3.jl:36-40, test/Demo.jl:22-23) and q = p + 2
+// used by the perturbed-mesh workloads.
+#define SF_FOR_EACH_NQ(X)                                                      \
+  X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5)                                      \
+  X(3, 2) X(3, 3) X(3, 4) X(3, 5) X(3, 6)                                      \
+  X(4, 3) X(4, 4) X(4, 5) X(4, 6) X(4, 7)                                      \
+  X(5, 4) X(5, 5) X(5, 6) X(5, 7) X(5, 8)                                      \
+  X(6, 5) X(6, 6) X(6, 7) X(6, 8) X(6, 9)                                      \
+  X(7, 6) X(7, 7) X(7, 8) X(7, 9) X(7, 10)                                     \
+  X(8, 7) X(8, 8) X(8, 9) X(8, 10) X(8, 11)                                    \
+  X(9, 8) X(9, 9) X(9, 10) X(9, 11) X(9, 12)                                   \
+  X(10, 9) X(10, 10) X(10, 11) X(10, 12) X(10, 13)
+
+constexpr int kMaxN = 10;
+constexpr int kMaxQ = 13;
+constexpr int kNumXcd = 8;  // MI355X: 8 XCDs, workgroups dealt round-robin
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int round_up(int a, int m) { return (a + m - 1) / m * m; }
+
+// Workgroups are dealt to the 8 XCDs round-robin (id % 8).  Remap so that
+// each XCD works on one contiguous range of cell batches: neighbouring
+// cells share face dofs, so the gather of x and the scatter into y then hit
+// the same XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  const int per = nblk / kNumXcd;
+  const int rem = nblk - per * kNumXcd;
+  const int xcd = bid % kNumXcd;
+  const int j = bid / kNumXcd;
+  return xcd * per + (xcd < rem ? xcd : rem) + j;
+}
+
+// bf16 storage for the geometric factor (round-to-nearest-even on store).
+struct bf16s {
+  uint16_t bits;
+};
+
+__host__ __device__ inline uint16_t f32_to_bf16_bits(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf/nan
+  const uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_bits_to_f32(uint16_t h) {
+  return __uint_as_float(((uint32_t)h) << 16);
+}
+
+// Streamed, read-once load of the geometric factor (non-temporal hint so
+// the G stream does not evict the reused x / y lines from L2).
+template <typename TG, typename T>
+__device__ __forceinline__ T load_g(const TG* p);
+
+template <>
+__device__ __forceinline__ double load_g<double, double>(const double* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <>
+__device__ __forceinline__ float load_g<float, float>(const float* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <>
+__device__ __forceinline__ float load_g<bf16s, float>(const bf16s* p) {
+  const uint16_t h = __builtin_nontemporal_load(&p->bits);
+  return bf16_bits_to_f32(h);
+}
+
+__device__ __forceinline__ void atomic_add(double* p, double v) {
+  unsafeAtomicAdd(p, v);
+}
+__device__ __forceinline__ void atomic_add(float* p, float v) {
+  unsafeAtomicAdd(p, v);
+}
+
+}  // namespace sf

@@ -1,0 +1,107 @@
+// sf_kernels.h -- internal launch entry points shared by the translation
+// units of libsfhip (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sf {
+
+struct bf16s;
+
+// ---- geometry (sf_geom.hip) ----
+hipError_t launch_geometry(int dim, const double* X, const double* pts,
+                           const double* w1d, int q, int64_t ncell, double* G,
+                           int* bad, hipStream_t s);
+hipError_t launch_scale_kappa(const double* G, const double* kappa, int ng,
+                              int64_t nqc, int64_t ncell, double* Gk,
+                              hipStream_t s);
+hipError_t launch_check_positive(const double* v, int64_t n, int* bad,
+                                 hipStream_t s);
+hipError_t launch_to_f32(const double* a, int64_t n, float* b, hipStream_t s);
+hipError_t launch_to_bf16(const double* a, int64_t n, bf16s* b, hipStream_t s);
+
+// ---- applies (sf_apply_*.hip); y += A x over all cells ----
+hipError_t apply_f64(int dim, int n, int q, const double* x, double* y,
+                     const int32_t* cells, const double* G, const double* tB,
+                     const double* tD, int64_t ncell, hipStream_t s);
+hipError_t apply_f32(int dim, int n, int q, const float* x, float* y,
+                     const int32_t* cells, const float* G, const float* tB,
+                     const float* tD, int64_t ncell, hipStream_t s);
+hipError_t apply_f32_gbf16(int dim, int n, int q, const float* x, float* y,
+                           const int32_t* cells, const bf16s* G,
+                           const float* tB, const float* tD, int64_t ncell,
+                           hipStream_t s);
+void apply_shape_f64(int dim, int n, int q, int* nt, int* eb);
+
+// ---- adjoint (sf_adjoint.hip) ----
+// dJdk[cell][qp] = grad(lam) . G grad(u);  ybar += A(kappa) lam
+hipError_t adjoint_f64(int dim, int n, int q, const double* u,
+                       const double* lam, double* dJdk, double* ybar,
+                       const int32_t* cells, const double* G,
+                       const double* kappa, const double* tB,
+                       const double* tD, int64_t ncell, hipStream_t s);
+
+// ---- BLAS-1 for the solvers (sf_blas.hip) ----
+struct CGScalars {
+  double rr;       // current r.r
+  double pq;       // p.Ap
+  double alpha;
+  double beta;
+  double rr_new;
+  double stop_rr;  // stop when rr <= stop_rr
+  int done;        // 1 once rr <= stop_rr (later updates are no-ops)
+  int iters;       // iterations taken while not done
+  int nonfinite;   // a NaN / Inf was met
+  int pad;
+};
+
+struct Reducer {
+  double* part;        // kRedBlocks partial sums
+  unsigned* counter;   // zero at rest, reset by the last block
+};
+
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+template <typename V>
+hipError_t launch_fill(V* a, int64_t n, V v, hipStream_t s);
+
+// out = a . b  (FP64 accumulation); result in *out (device)
+template <typename V>
+hipError_t launch_dot(const V* a, const V* b, int64_t n, Reducer red,
+                      double* out, hipStream_t s);
+
+// CG pieces on device scalars sc:
+//   cg_start : r = b - Ax, p = r, q = 0, rr = r.r, done = rr <= stop_rr
+//   cg_alpha : pq = p.q, alpha = rr / pq
+//   cg_update: x += alpha p, r -= alpha q, rr_new = r.r, beta = rr_new / rr,
+//              rr = rr_new, iters++, done = rr <= stop_rr
+//   cg_dir   : p = r + beta p, q = 0 (ready for the next q += A p)
+// Once done is set every later piece returns at once.
+template <typename V>
+hipError_t launch_cg_start(const V* b, const V* Ax, V* r, V* p, V* q,
+                           int64_t n, Reducer red, CGScalars* sc,
+                           double stop_rr, hipStream_t s);
+template <typename V>
+hipError_t launch_cg_alpha(const V* p, const V* q, int64_t n, Reducer red,
+                           CGScalars* sc, hipStream_t s);
+template <typename V>
+hipError_t launch_cg_update(V* x, V* r, const V* p, const V* q, int64_t n,
+                            Reducer red, CGScalars* sc, hipStream_t s);
+template <typename V>
+hipError_t launch_cg_dir(V* p, const V* r, V* q, int64_t n, CGScalars* sc,
+                         hipStream_t s);
+
+// IR helpers (FP64 outer, FP32 inner)
+// r = b - Ax (FP64)
+hipError_t launch_residual(const double* b, const double* Ax, double* r,
+                           int64_t n, hipStream_t s);
+// r32 = r * inv_scale
+hipError_t launch_scale_to_f32(const double* r, double inv_scale, float* r32,
+                               int64_t n, hipStream_t s);
+// x += scale * d32
+hipError_t launch_axpy_f32_to_f64(double* x, const float* d32, double scale,
+                                  int64_t n, hipStream_t s);
+
+}  // namespace sf

@@ -1,0 +1,58 @@
+"""Device-resident solvers around the operator.
+
+``cg_(x, A, b)`` mirrors ``IterativeSolvers.cg!(x, A, b)`` as the reference's
+driver calls it (compute_solution, profiling/drivers.jl:85-95): unpreconditioned
+CG, x is the initial guess and is overwritten.  ``cg_ir_`` is the
+mixed-precision iterative refinement: FP64 outer residual, norms and dot
+products, reduced-precision (FP32-vector) inner CG.  All vectors are torch
+CUDA float64 tensors; the iteration runs in libsfhip.so on torch's current
+stream.
+"""
+import ctypes
+
+from . import _lib
+from .operator import _torch_stream
+
+
+def _vec_ok(A, v):
+    if not (type(v).__module__.startswith("torch") and v.is_cuda and str(v.dtype) == "torch.float64"
+            and v.is_contiguous() and v.numel() == A.n_free):
+        raise TypeError("solver vectors must be contiguous torch CUDA float64 of n_free entries")
+
+
+def cg_(x, A, b, rtol=1e-10, maxiter=None, raise_on_fail=False):
+    """Plain FP64 CG: stop when ||b - A x|| <= rtol ||b||.  Returns the
+    statistics dict (iterations, rel_residual, converged, seconds)."""
+    _vec_ok(A, x)
+    _vec_ok(A, b)
+    st = _lib.SolveStats()
+    maxiter = A.n_free if maxiter is None else int(maxiter)
+    rc = A._L.sf_cg(A.plan, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                    float(rtol), maxiter, ctypes.byref(st), _torch_stream(x))
+    if rc != _lib.SF_OK and (raise_on_fail or rc != _lib.SF_ERR_NOT_CONVERGED):
+        _lib.check(rc)
+    return st.as_dict()
+
+
+def cg_ir_(x, A, b, rtol=1e-10, inner="f32", inner_rtol=1e-3, inner_maxiter=None,
+           max_outer=50, raise_on_fail=False):
+    """Mixed-precision iterative refinement (FP64 outer, reduced inner)."""
+    _vec_ok(A, x)
+    _vec_ok(A, b)
+    st = _lib.SolveStats()
+    prec = _lib.PRECISIONS[inner] if isinstance(inner, str) else int(inner)
+    inner_maxiter = A.n_free if inner_maxiter is None else int(inner_maxiter)
+    rc = A._L.sf_cg_ir(A.plan, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                       float(rtol), int(max_outer), prec, float(inner_rtol), inner_maxiter,
+                       ctypes.byref(st), _torch_stream(x))
+    if rc != _lib.SF_OK and (raise_on_fail or rc != _lib.SF_ERR_NOT_CONVERGED):
+        _lib.check(rc)
+    return st.as_dict()
+
+
+def compute_solution(A, b, **kw):
+    """x = A \\ b from a zero initial guess (profiling/drivers.jl:85-89)."""
+    import torch
+    x = torch.zeros_like(b)
+    cg_(x, A, b, **kw)
+    return x

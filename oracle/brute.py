@@ -1,0 +1,150 @@
+"""TEST INFRASTRUCTURE ONLY -- brute-force (non sum-factorised) numpy oracle.
+
+Independent of both the C restatement (oracle/sumfac_oracle.c) and the
+product library:
+
+* 1D tables by a Legendre-Vandermonde change of basis (the reference builds
+  its Lagrange basis through a change of basis from a polynomial basis too,
+  SumFactorizationMaps.jl:50 via Gridap's LagrangianRefFE), evaluated at
+  numpy's Gauss-Legendre points (SumFactorizationMaps.jl:51);
+* dense element gradient matrices grad(phi_a)(xi_t) = kron products of the 1D
+  tables (no factorised contractions);
+* geometry from numpy.linalg on J = dx/dxi (not Gridap's transposed layout);
+* element stiffness matrices K_e = sum_t Grad_t^T G_t Grad_t, their
+  assembly into scipy CSR -- the "assembled SpMV" side of the reference's
+  comparison (test/Demo.jl:39-57, profiling/drivers.jl:7-26);
+* the Kronecker-sum closed form of the operator on a uniform Cartesian grid.
+"""
+import numpy as np
+import scipy.sparse as sp
+from numpy.polynomial import legendre as npleg
+
+
+def nodes_1d(p):
+    """Reference 1D node order: [0, 1, 1/p, ..., (p-1)/p]."""
+    if p == 0:
+        return np.array([0.5])
+    return np.array([0.0, 1.0] + [k / p for k in range(1, p)])
+
+
+def node_offsets(p):
+    """Lattice offset (in units of h/p) of each 1D node position."""
+    return np.array([0, p] + list(range(1, p)), dtype=np.int64)
+
+
+def gauss01(q):
+    x, w = npleg.leggauss(q)
+    return 0.5 * (x + 1.0), 0.5 * w
+
+
+def tables(p, q):
+    """(B, Dm, points, weights): B[i, j] = phi_j(xi_i), Dm = phi_j'(xi_i)."""
+    t = nodes_1d(p)
+    n = p + 1
+    # phi_j = sum_k C[k, j] P_k(2x - 1); V[i, k] = P_k(2 t_i - 1)
+    V = npleg.legvander(2.0 * t - 1.0, n - 1)
+    C = np.linalg.solve(V, np.eye(n))
+    pts, w = gauss01(q)
+    Vq = npleg.legvander(2.0 * pts - 1.0, n - 1)
+    # derivative: d/dx P_k(2x-1) = 2 P_k'(2x-1)
+    dV = np.zeros((q, n))
+    for k in range(n):
+        ck = np.zeros(n)
+        ck[k] = 1.0
+        dV[:, k] = 2.0 * npleg.legval(2.0 * pts - 1.0, npleg.legder(ck))
+    return Vq @ C, dV @ C, pts, w
+
+
+def grad_matrices(dim, B, Dm):
+    """Dense reference gradients: list over directions of (q^D, n^D) arrays,
+    row t = quadrature point (x fastest), column a = local dof (x fastest)."""
+    if dim == 2:
+        return [np.kron(B, Dm), np.kron(Dm, B)]
+    return [np.kron(B, np.kron(B, Dm)), np.kron(B, np.kron(Dm, B)),
+            np.kron(Dm, np.kron(B, B))]
+
+
+def geometry_numpy(dim, verts, pts, w):
+    """(G, detJ): G[cell, t, r, k] = w_t |det J| (J^T J)^{-1}[r, k]."""
+    verts = np.asarray(verts, dtype=np.float64)
+    q = len(pts)
+    nv = 2 ** dim
+    if dim == 2:
+        xi = np.stack(np.meshgrid(pts, pts, indexing="xy"), -1).reshape(-1, 2)
+        wq = np.outer(w, w).reshape(-1)
+    else:
+        zz, yy, xx = np.meshgrid(pts, pts, pts, indexing="ij")
+        xi = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], -1)
+        wq = np.einsum("l,k,i->lki", w, w, w).reshape(-1)
+    # dN[t, v, b] = d N_v / d xi_b
+    dN = np.zeros((q ** dim, nv, dim))
+    for v in range(nv):
+        bits = [(v >> d) & 1 for d in range(dim)]
+        s = np.stack([xi[:, d] if bits[d] else 1.0 - xi[:, d] for d in range(dim)], -1)
+        ds = np.array([1.0 if bits[d] else -1.0 for d in range(dim)])
+        for b in range(dim):
+            col = ds[b] * np.ones(len(xi))
+            for d in range(dim):
+                if d != b:
+                    col = col * s[:, d]
+            dN[:, v, b] = col
+    # J[cell, t, a, b] = sum_v X[cell, v, a] dN[t, v, b]
+    J = np.einsum("cva,tvb->ctab", verts, dN)
+    det = np.linalg.det(J)
+    Ji = np.linalg.inv(J)
+    G = np.einsum("t,ct,ctrl,ctkl->ctrk", wq, np.abs(det), Ji, Ji)
+    return G, det
+
+
+def element_matrices(dim, B, Dm, G):
+    """K[cell] = sum_t Grad_t^T G_t Grad_t (dense, no factorisation)."""
+    Gr = np.stack(grad_matrices(dim, B, Dm))  # (D, nq, nl)
+    return np.einsum("rta,ctrk,ktb->cab", Gr, G, Gr, optimize=True)
+
+
+def apply_dense(cells, K, x):
+    """y = sum_e P_e^T K_e P_e x with Dirichlet ids (< 0) dropped."""
+    cells = np.asarray(cells)
+    xl = np.where(cells >= 0, x[np.maximum(cells, 0)], 0.0)
+    yl = np.einsum("cab,cb->ca", K, xl)
+    y = np.zeros_like(x)
+    m = cells >= 0
+    np.add.at(y, cells[m], yl[m])
+    return y
+
+
+def assemble_csr(cells, K, nfree):
+    """Assembled sparse operator on the free dofs (scipy CSR)."""
+    cells = np.asarray(cells)
+    nl = cells.shape[1]
+    rows = np.repeat(cells, nl, axis=1).reshape(-1)
+    cols = np.tile(cells, (1, nl)).reshape(-1)
+    vals = K.reshape(-1)
+    m = (rows >= 0) & (cols >= 0)
+    A = sp.coo_matrix((vals[m], (rows[m], cols[m])), shape=(nfree, nfree))
+    return A.tocsr()
+
+
+def kron_operator(dim, p, q, m):
+    """Closed form on the uniform m^D grid of [0,1]^D with homogeneous
+    Dirichlet boundary:  A = sum_d  M (x) .. K (direction d) .. (x) M, with the
+    1D stiffness K1 and mass M1 assembled from the same Gauss rule and
+    restricted to interior lattice nodes (free id = sum_d (L_d - 1) F^d)."""
+    B, Dm, pts, w = tables(p, q)
+    h = 1.0 / m
+    K1e = (Dm.T * w) @ Dm / h
+    M1e = (B.T * w) @ B * h
+    nL = p * m + 1
+    off = node_offsets(p)
+    K1 = np.zeros((nL, nL))
+    M1 = np.zeros((nL, nL))
+    for e in range(m):
+        L = e * p + off
+        K1[np.ix_(L, L)] += K1e
+        M1[np.ix_(L, L)] += M1e
+    K1 = sp.csr_matrix(K1[1:-1, 1:-1])
+    M1 = sp.csr_matrix(M1[1:-1, 1:-1])
+    if dim == 2:
+        return (sp.kron(M1, K1) + sp.kron(K1, M1)).tocsr()
+    return (sp.kron(M1, sp.kron(M1, K1)) + sp.kron(M1, sp.kron(K1, M1)) +
+            sp.kron(K1, sp.kron(M1, M1))).tocsr()

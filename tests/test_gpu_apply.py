@@ -1,0 +1,233 @@
+"""GPU parity of the HIP apply (FP64) against the CPU oracle, through the
+C ABI.  Bar: 1e-12 relative l2 (the north-star FP64 tolerance)."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(pkg, D, m, p, q, perturb=0.2, seed=1, dirichlet=True, kappa=None):
+    model = pkg.CartesianModel(D, m, perturb=perturb, seed=seed)
+    space = pkg.FESpace(model, p, dirichlet=dirichlet)
+    mp = pkg.SumFactorizationMap.with_points(D, p, q)
+    A = pkg.LazyMatrix(mp, space, kappa=kappa)
+    return model, space, mp, A
+
+
+def gpu_apply(cuda, A, x):
+    xd = cuda.from_numpy(np.ascontiguousarray(x)).cuda()
+    yd = cuda.full_like(xd, float("nan"))  # overwrite semantics: garbage in y
+    A.mul(yd, xd)
+    cuda.cuda.synchronize()
+    return yd.cpu().numpy()
+
+
+NQ = [(n, q) for n in range(2, 11) for q in range(n - 1, n + 4)]
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_every_instantiated_kernel_vs_oracle(pkg, oracle_mod, cuda, D):
+    worst = 0.0
+    for n, q in NQ:
+        p = n - 1
+        m = 3 if D == 3 and n <= 5 else (2 if D == 3 else 4)
+        say("case D=%d p=%d q=%d m=%d" % (D, p, q, m))
+        model, space, mp, A = make(pkg, D, m, p, q)
+        x = pkg.uniform(0, space.n_free, -1, 1)
+        y = gpu_apply(cuda, A, x)
+        y_ref = oracle_mod.apply(D, n, q, space.cell_dofs, model.cell_vertices(), x, mp.tables)
+        e = rel_l2(y, y_ref)
+        worst = max(worst, e)
+        assert e < TOL, (D, p, q, e)
+        A.free()
+    say("worst rel. l2 over %d kernels: %.2e" % (len(NQ), worst))
+
+
+def test_config1_2d_q4_vs_oracle_and_assembled(pkg, oracle_mod, brute, cuda):
+    """2D unit square, 64x64 Cartesian quads, Q4, 6 Gauss points, u ~ U[-1,1]
+    seed 0: against the oracle and the assembled CSR SpMV."""
+    say("config 1: 2D 64^2 Q4 q=6")
+    model, space, mp, A = make(pkg, 2, 64, 4, 6, perturb=0.0)
+    assert space.n_free == 65025
+    x = pkg.uniform(0, space.n_free, -1, 1)
+    y = gpu_apply(cuda, A, x)
+    X = model.cell_vertices()
+    y_or = oracle_mod.apply(2, 5, 6, space.cell_dofs, X, x, mp.tables)
+    B, Dm, pts, w = brute.tables(4, 6)
+    G, _ = brute.geometry_numpy(2, X, pts, w)
+    Acsr = brute.assemble_csr(space.cell_dofs, brute.element_matrices(2, B, Dm, G), space.n_free)
+    y_sp = Acsr @ x
+    say("rel l2 vs oracle %.2e, vs assembled %.2e" % (rel_l2(y, y_or), rel_l2(y, y_sp)))
+    assert rel_l2(y, y_or) < TOL
+    assert rel_l2(y, y_sp) < TOL
+
+
+@pytest.mark.parametrize("D,m,p,q", [(3, 8, 4, 6), (3, 16, 4, 6), (3, 4, 7, 9), (3, 6, 5, 7),
+                                     (3, 3, 9, 11), (3, 8, 2, 4), (3, 8, 3, 5), (3, 5, 6, 8)])
+def test_reduced_configs_vs_oracle(pkg, oracle_mod, cuda, D, m, p, q):
+    say("reduced config D=%d m=%d p=%d q=%d" % (D, m, p, q))
+    model, space, mp, A = make(pkg, D, m, p, q)
+    x = pkg.uniform(0, space.n_free, -1, 1)
+    y = gpu_apply(cuda, A, x)
+    y_ref = oracle_mod.apply(D, p + 1, q, space.cell_dofs, model.cell_vertices(), x, mp.tables)
+    assert rel_l2(y, y_ref) < TOL
+
+
+def test_config2_full_size_vs_oracle(pkg, oracle_mod, cuda):
+    """3D 64^3 perturbed hexes, Q4, 6 points (16.6M dofs), u ~ U[-1,1]."""
+    say("config 2 full size: 3D 64^3 Q4 q=6 perturbed")
+    model, space, mp, A = make(pkg, 3, 64, 4, 6)
+    assert space.n_free == 16581375
+    x = pkg.uniform(0, space.n_free, -1, 1)
+    y = gpu_apply(cuda, A, x)
+    say("GPU done; oracle (CPU) ...")
+    y_ref = oracle_mod.apply(3, 5, 6, space.cell_dofs, model.cell_vertices(), x, mp.tables)
+    e = rel_l2(y, y_ref)
+    say("rel l2 %.2e" % e)
+    assert e < TOL
+    # size-independent properties at full size: symmetry and linearity
+    v = pkg.uniform(9, space.n_free, -1, 1)
+    Av = gpu_apply(cuda, A, v)
+    s1, s2 = v @ y, x @ Av
+    assert abs(s1 - s2) < 1e-12 * abs(x @ y)
+    Axv = gpu_apply(cuda, A, 2.0 * x - 3.0 * v)
+    assert rel_l2(Axv, 2.0 * y - 3.0 * Av) < 1e-13
+    A.free()
+
+
+def test_patch_test_full_operator(pkg, cuda):
+    """All dofs free (no Dirichlet): A 1 = 0 and interior rows of A (a.x + b)
+    vanish on a perturbed mesh (exact quadrature, q = p + 2)."""
+    for D, m, p in [(3, 12, 4), (2, 32, 5)]:
+        say("patch test D=%d m=%d p=%d" % (D, m, p))
+        model, space, mp, A = make(pkg, D, m, p, p + 2, dirichlet=False)
+        one = gpu_apply(cuda, A, np.ones(space.n_free))
+        assert np.max(np.abs(one)) < 1e-12
+        xyz = space.free_dof_coordinates()
+        u = xyz @ np.array([0.3, -0.8, 1.1][:D]) - 0.5
+        r = gpu_apply(cuda, A, u)
+        nL = space.n_lattice
+        idx = np.arange(space.n_free)
+        lat = np.stack([(idx // nL ** d) % nL for d in range(D)], -1)
+        interior = np.all((lat > 0) & (lat < nL - 1), axis=1)
+        assert np.max(np.abs(r[interior])) < 1e-12
+        A.free()
+
+
+def test_device_geometry_vs_oracle(pkg, oracle_mod, cuda):
+    for D, p, q in [(3, 4, 6), (3, 7, 9), (2, 4, 6)]:
+        say("geometry D=%d p=%d q=%d" % (D, p, q))
+        model, space, mp, A = make(pkg, D, 5, p, q)
+        G = A.geometry()
+        G_ref = oracle_mod.geometry(D, q, model.cell_vertices(), mp.tables)
+        assert np.max(np.abs(G - G_ref)) < 1e-14 * np.max(np.abs(G_ref))
+        # the same operator from host-supplied G
+        x = pkg.uniform(0, space.n_free, -1, 1)
+        A2 = pkg.LazyMatrix(mp, space, geometry=G_ref)
+        assert rel_l2(gpu_apply(cuda, A2, x), gpu_apply(cuda, A, x)) < 1e-14
+        A.free()
+        A2.free()
+
+
+def test_host_pointer_apply_and_input_untouched(pkg, cuda):
+    say("host apply")
+    model, space, mp, A = make(pkg, 3, 6, 3, 5)
+    x = pkg.uniform(0, space.n_free, -1, 1)
+    x0 = x.copy()
+    yh = np.full(space.n_free, np.nan)
+    pkg.mul_(yh, A, x)
+    assert np.array_equal(x, x0)
+    yd = gpu_apply(cuda, A, x)
+    assert np.array_equal(yh, yd) or rel_l2(yh, yd) < 1e-15
+    # repeated applies overwrite (no accumulation across calls)
+    yh2 = np.empty_like(yh)
+    pkg.mul_(yh2, A, x)
+    assert rel_l2(yh2, yh) < 1e-15
+
+
+def test_demo_jl_on_gpu(pkg, brute, cuda):
+    """test/Demo.jl:57: norm(A*1 - A_lazy*1) < 1e-6, 3D Q4, degree 6, 10^3."""
+    say("Demo.jl on GPU")
+    m = pkg.SumFactorizationMap(3, (4, 4, 4), (6, 6, 6))
+    model = pkg.CartesianModel(3, 10)
+    space = pkg.FESpace(model, 4)
+    A_lazy = pkg.LazyMatrix(m, space)
+    y = np.ones(space.n_free)
+    x2 = np.zeros(space.n_free)
+    pkg.mul_(x2, A_lazy, y)
+    B, Dm, pts, w = brute.tables(4, 4)
+    G, _ = brute.geometry_numpy(3, model.cell_vertices(), pts, w)
+    A = brute.assemble_csr(space.cell_dofs, brute.element_matrices(3, B, Dm, G), space.n_free)
+    x1 = A @ y
+    assert np.linalg.norm(x1 - x2) < 1e-6
+
+
+def test_error_paths(pkg, cuda):
+    say("error paths")
+    L = pkg.load_library()
+    model = pkg.CartesianModel(3, 2)
+    space = pkg.FESpace(model, 2)
+    mp = pkg.SumFactorizationMap.with_points(3, 2, 4)
+    h = ctypes.c_void_p()
+    assert L.sf_plan_create(ctypes.byref(h), 3, 2, 4, space.n_cells, space.n_free, 0) == 0
+    x = cuda.zeros(space.n_free, dtype=cuda.float64, device="cuda")
+    y = cuda.zeros_like(x)
+    # not ready
+    rc = L.sf_apply(h, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()), None)
+    assert rc == pkg._lib.SF_ERR_NOT_READY
+    # dof id out of range
+    bad = space.cell_dofs.copy()
+    bad[0, 5] = space.n_free + 3
+    assert L.sf_plan_set_cells(h, bad.ctypes.data, 0, None) == pkg._lib.SF_ERR_DOF_RANGE
+    assert b"outside" in L.sf_last_error()
+    # a local_to_tensor that is not a permutation
+    perm = np.zeros(27, dtype=np.int32)
+    assert L.sf_plan_set_cells(h, space.cell_dofs.ctypes.data, 0, perm.ctypes.data) == pkg._lib.SF_ERR_ARG
+    # inverted cell: det J < 0
+    X = model.cell_vertices().copy()
+    X[0] = X[0][[1, 0, 3, 2, 5, 4, 7, 6]]  # mirror in x
+    assert L.sf_plan_set_vertices(h, X.ctypes.data) == pkg._lib.SF_ERR_JACOBIAN
+    # non-positive kappa
+    k = np.ones(space.n_cells * 64)
+    k[3] = -1.0
+    assert L.sf_plan_set_kappa(h, k.ctypes.data) == pkg._lib.SF_ERR_ARG
+    assert L.sf_plan_destroy(h) == 0
+    with pytest.raises(pkg.SFError):
+        pkg.LazyMatrix(mp, space, kappa=-np.ones(space.n_cells * 64))
+    _ = mp
+
+
+def test_gridap_convention_cells_and_local_permutation(pkg, oracle_mod, cuda):
+    """index_base = 1 (Gridap: free ids 1..n, Dirichlet <= 0) and a
+    local-to-tensor permutation (the reference's dof_map) give the same
+    operator as 0-based tensor-ordered ids."""
+    say("gridap convention cells")
+    model, space, mp, A = make(pkg, 3, 4, 3, 5)
+    L = pkg.load_library()
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(64).astype(np.int32)  # local k -> tensor slot perm[k]
+    cd = space.cell_dofs
+    local = np.empty_like(cd)
+    local[:, np.arange(64)] = cd[:, perm]  # local dof k holds tensor slot perm[k]
+    gridap = np.where(local >= 0, local + 1, -7).astype(np.int32)
+    h = ctypes.c_void_p()
+    assert L.sf_plan_create(ctypes.byref(h), 3, 3, 5, space.n_cells, space.n_free, 0) == 0
+    assert L.sf_plan_set_cells(h, gridap.ctypes.data, 1, perm.ctypes.data) == 0
+    X = model.cell_vertices()
+    assert L.sf_plan_set_vertices(h, X.ctypes.data) == 0
+    x = pkg.uniform(0, space.n_free, -1, 1)
+    y = np.zeros(space.n_free)
+    assert L.sf_apply_host(h, y.ctypes.data, x.ctypes.data) == 0
+    L.sf_plan_destroy(h)
+    assert rel_l2(y, gpu_apply(cuda, A, x)) < 1e-15

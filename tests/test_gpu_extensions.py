@@ -1,0 +1,177 @@
+"""GPU tests of the north-star extensions, through the C ABI:
+reduced-precision applies (error against the FP64 oracle), the fused
+adjoint kernel (oracle gradient + central finite differences), and the
+device-resident CG / mixed-precision iterative refinement."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(pkg, D, m, p, q, perturb=0.2, kappa=None):
+    model = pkg.CartesianModel(D, m, perturb=perturb, seed=1)
+    space = pkg.FESpace(model, p)
+    mp = pkg.SumFactorizationMap.with_points(D, p, q)
+    A = pkg.LazyMatrix(mp, space, kappa=kappa)
+    return model, space, mp, A
+
+
+@pytest.mark.parametrize("D,m,p,q", [(3, 8, 4, 6), (3, 4, 7, 9), (2, 16, 4, 6), (3, 3, 9, 11)])
+def test_reduced_precision_apply_error(pkg, oracle_mod, cuda, D, m, p, q):
+    """FP32: relative l2 error ~ FP32 rounding (<= 1e-5); G in bf16: the
+    error of a 2^-9 relative perturbation of the coefficient (<= 2e-2)."""
+    say("reduced precision D=%d m=%d p=%d q=%d" % (D, m, p, q))
+    model, space, mp, A = make(pkg, D, m, p, q)
+    x = pkg.uniform(0, space.n_free, -1, 1)
+    y_ref = oracle_mod.apply(D, p + 1, q, space.cell_dofs, model.cell_vertices(), x, mp.tables)
+    x32 = cuda.from_numpy(x.astype(np.float32)).cuda()
+    y32 = cuda.empty_like(x32)
+    errs = {}
+    for prec, tol in [("f32", 1e-5), ("f32_gbf16", 2e-2)]:
+        A.mul_reduced(y32, x32, prec)
+        cuda.cuda.synchronize()
+        errs[prec] = rel_l2(y32.cpu().numpy().astype(np.float64), y_ref)
+        assert errs[prec] < tol, (prec, errs[prec])
+    assert errs["f32_gbf16"] > errs["f32"]
+    say("errors", errs)
+    A.free()
+
+
+@pytest.mark.parametrize("D,m,p,q", [(3, 4, 5, 7), (3, 3, 3, 5), (2, 8, 4, 6)])
+def test_adjoint_vs_oracle_and_fd(pkg, oracle_mod, cuda, D, m, p, q):
+    say("adjoint D=%d m=%d p=%d q=%d" % (D, m, p, q))
+    model = pkg.CartesianModel(D, m, perturb=0.2, seed=1)
+    space = pkg.FESpace(model, p)
+    mp = pkg.SumFactorizationMap.with_points(D, p, q)
+    nq = q ** D
+    kappa = np.exp(pkg.normal(3, space.n_cells * nq))
+    A = pkg.LazyMatrix(mp, space, kappa=kappa)
+    X = model.cell_vertices()
+    u = pkg.uniform(0, space.n_free, -1, 1)
+    d = pkg.uniform(4, space.n_free, -1, 1)
+    ud = cuda.from_numpy(u).cuda()
+    dd = cuda.from_numpy(d).cuda()
+    J, kb, ub = pkg.adjoint.loss_and_grad(A, kappa, ud, dd)
+    cuda.cuda.synchronize()
+    kb = kb.cpu().numpy()
+    ub = ub.cpu().numpy()
+    # oracle: r = A(kappa) u - d, dJ/dkappa = kappa_grad(u, r), dJ/du = A(kappa) r
+    Au = oracle_mod.apply(D, p + 1, q, space.cell_dofs, X, u, mp.tables, kappa=kappa)
+    r = Au - d
+    J_ref = 0.5 * r @ r
+    assert abs(J - J_ref) < 1e-12 * abs(J_ref)
+    kb_ref = oracle_mod.kappa_grad(D, p + 1, q, space.cell_dofs, X, u, r, mp.tables).reshape(-1)
+    ub_ref = oracle_mod.apply(D, p + 1, q, space.cell_dofs, X, r, mp.tables, kappa=kappa)
+    assert rel_l2(kb, kb_ref) < 1e-12
+    assert rel_l2(ub, ub_ref) < 1e-12
+    # central finite differences, h = 1e-6, FP64, random unit directions.
+    # J is quadratic in kappa and in u, so the only error is cancellation.
+    def Jk(k):
+        A.set_kappa(k)
+        y = cuda.empty_like(ud)
+        A.mul(y, ud)
+        rr = y - dd
+        return 0.5 * float((rr * rr).sum())
+
+    def Ju(v):
+        A.set_kappa(kappa)
+        y = cuda.empty_like(ud)
+        A.mul(y, cuda.from_numpy(v).cuda())
+        rr = y - dd
+        return 0.5 * float((rr * rr).sum())
+
+    h = 1e-6
+    for s in range(4):
+        dk = pkg.normal(100 + s, kappa.size)
+        dk /= np.linalg.norm(dk)
+        fd = (Jk(kappa + h * dk) - Jk(kappa - h * dk)) / (2 * h)
+        an = kb @ dk
+        assert abs(fd - an) < 1e-6 * (abs(an) + 1e-3 * np.linalg.norm(kb))
+        du = pkg.normal(200 + s, u.size)
+        du /= np.linalg.norm(du)
+        fd = (Ju(u + h * du) - Ju(u - h * du)) / (2 * h)
+        an = ub @ du
+        assert abs(fd - an) < 1e-6 * (abs(an) + 1e-3 * np.linalg.norm(ub))
+    A.free()
+
+
+def test_adjoint_outputs_optional(pkg, cuda):
+    say("adjoint optional outputs")
+    model, space, mp, A = make(pkg, 3, 3, 3, 5)
+    u = cuda.from_numpy(pkg.uniform(0, space.n_free, -1, 1)).cuda()
+    lam = cuda.from_numpy(pkg.uniform(1, space.n_free, -1, 1)).cuda()
+    kb1, ub1 = pkg.adjoint.pullback(A, u, lam)
+    kb2, _ = pkg.adjoint.pullback(A, u, lam, want_u=False)
+    _, ub3 = pkg.adjoint.pullback(A, u, lam, want_kappa=False)
+    cuda.cuda.synchronize()
+    assert cuda.equal(kb1, kb2)
+    assert cuda.equal(ub1, ub3)
+    # without kappa, dJ/du = A lam
+    y = cuda.empty_like(lam)
+    A.mul(y, lam)
+    assert rel_l2(ub1.cpu().numpy(), y.cpu().numpy()) < 1e-14
+    A.free()
+
+
+def test_cg_recovers_manufactured_solution(pkg, oracle_mod, cuda):
+    say("cg manufactured")
+    model, space, mp, A = make(pkg, 3, 6, 3, 5)
+    xs = pkg.uniform(7, space.n_free, -1, 1)
+    b = oracle_mod.apply(3, 4, 5, space.cell_dofs, model.cell_vertices(), xs, mp.tables)
+    bd = cuda.from_numpy(b).cuda()
+    x = cuda.zeros_like(bd)
+    st = pkg.cg_(x, A, bd, rtol=1e-12, maxiter=5000)
+    assert st["converged"] == 1 and st["rel_residual"] <= 1e-12
+    xh = x.cpu().numpy()
+    # independent residual with the oracle operator
+    r = b - oracle_mod.apply(3, 4, 5, space.cell_dofs, model.cell_vertices(), xh, mp.tables)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 2e-12
+    assert rel_l2(xh, xs) < 1e-8
+    A.free()
+
+
+def test_cg_maxiter_and_zero_rhs(pkg, cuda):
+    say("cg limits")
+    model, space, mp, A = make(pkg, 3, 4, 2, 4)
+    b = cuda.from_numpy(pkg.normal(2, space.n_free)).cuda()
+    x = cuda.zeros_like(b)
+    st = pkg.cg_(x, A, b, rtol=1e-14, maxiter=3)
+    assert st["iterations"] == 3 and st["converged"] == 0
+    with pytest.raises(pkg.SFError):
+        pkg.cg_(cuda.zeros_like(b), A, b, rtol=1e-14, maxiter=3, raise_on_fail=True)
+    z = cuda.zeros_like(b)
+    x = cuda.zeros_like(b)
+    st = pkg.cg_(x, A, z, rtol=1e-10)
+    assert st["iterations"] == 0 and float(x.abs().max()) == 0.0
+    A.free()
+
+
+@pytest.mark.parametrize("inner", ["f32", "f32_gbf16"])
+def test_mixed_precision_refinement(pkg, oracle_mod, cuda, inner):
+    say("IR inner=%s" % inner)
+    model, space, mp, A = make(pkg, 3, 6, 5, 7)
+    b = cuda.from_numpy(pkg.normal(2, space.n_free)).cuda()
+    x64 = cuda.zeros_like(b)
+    s64 = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=20000)
+    assert s64["converged"] == 1
+    x = cuda.zeros_like(b)
+    st = pkg.cg_ir_(x, A, b, rtol=1e-10, inner=inner, inner_rtol=1e-2, max_outer=60)
+    say("f64", s64, "ir", st)
+    assert st["converged"] == 1 and st["rel_residual"] <= 1e-10
+    assert st["outer_iterations"] >= 2
+    # the FP64 residual, recomputed by the oracle
+    xh = x.cpu().numpy()
+    bh = b.cpu().numpy()
+    r = bh - oracle_mod.apply(3, 6, 7, space.cell_dofs, model.cell_vertices(), xh, mp.tables)
+    assert np.linalg.norm(r) / np.linalg.norm(bh) < 2e-10
+    assert rel_l2(xh, x64.cpu().numpy()) < 1e-7
+    A.free()
