@@ -45,24 +45,36 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
   const int tid = threadIdx.x;
   // per cell: [A_u (2 SA1)][A_l (2 SA1)][B_u (3 SB1)][B_l (3 SB1)]
 
-  // stage 1: gather u, lam; x-pencils
-  for (int t = tid; t < S::EB * S::NN; t += S::NT) {
-    const int e = t / S::NN, r = t - e * S::NN;
-    const int64_t ge = base + e;
+  // stage 1: gather u, lam; x-pencils.  NT >= EB * NN, so the thread of
+  // x-pencil (e, r) also scatters it in stage 5 and keeps its ids.
+  static_assert(S::NT >= S::EB * S::NN, "one x-pencil per thread");
+  int ids[N];
+  const bool own1 = tid < S::EB * S::NN;
+  const int e1 = tid / S::NN, r1 = tid - e1 * S::NN;
+  const int64_t ge1 = base + e1;
+  const bool act1 = own1 && ge1 < ncell;
+  if (own1) {
     double uu[N], ll[N];
-    if (ge < ncell) {
-      const int32_t* ids = cells + (ge * S::NN + r) * N;
+    if (act1) {
+      const int32_t* cid = cells + (ge1 * S::NN + r1) * N;
+#pragma unroll
+      for (int j = 0; j < N; ++j) ids[j] = cid[j];
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        const int id = ids[j];
-        uu[j] = id >= 0 ? u[id] : 0.0;
-        ll[j] = id >= 0 ? lam[id] : 0.0;
+        const int k = ids[j] >= 0 ? ids[j] : 0;
+        uu[j] = u[k];
+        ll[j] = lam[k];
+      }
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        uu[j] = ids[j] >= 0 ? uu[j] : 0.0;
+        ll[j] = ids[j] >= 0 ? ll[j] : 0.0;
       }
     } else {
 #pragma unroll
       for (int j = 0; j < N; ++j) uu[j] = ll[j] = 0.0;
     }
-    double* sA = smem + e * S::SE;
+    double* sA = smem + e1 * S::SE;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       double bu = 0, du = 0, bl = 0, dl = 0;
@@ -73,10 +85,10 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
         bl = fma(tB[i * N + j], ll[j], bl);
         dl = fma(tD[i * N + j], ll[j], dl);
       }
-      sA[r * S::QP + i] = bu;
-      sA[S::SA1 + r * S::QP + i] = du;
-      sA[2 * S::SA1 + r * S::QP + i] = bl;
-      sA[3 * S::SA1 + r * S::QP + i] = dl;
+      sA[r1 * S::QP + i] = bu;
+      sA[S::SA1 + r1 * S::QP + i] = du;
+      sA[2 * S::SA1 + r1 * S::QP + i] = bl;
+      sA[3 * S::SA1 + r1 * S::QP + i] = dl;
     }
   }
   __syncthreads();
@@ -215,18 +227,14 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
   __syncthreads();
 
   // stage 5: transposed x-pencils + scatter-add
-  for (int t = tid; t < S::EB * S::NN; t += S::NT) {
-    const int e = t / S::NN, r = t - e * S::NN;
-    const int64_t ge = base + e;
-    if (ge >= ncell) continue;
-    const double* sA = smem + e * S::SE;
+  if (act1) {
+    const double* sA = smem + e1 * S::SE;
     double sx[Q], so[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      sx[i] = sA[r * S::QP + i];
-      so[i] = sA[S::SA1 + r * S::QP + i];
+      sx[i] = sA[r1 * S::QP + i];
+      so[i] = sA[S::SA1 + r1 * S::QP + i];
     }
-    const int32_t* ids = cells + (ge * S::NN + r) * N;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
       double v = 0;
@@ -235,8 +243,7 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
         v = fma(tD[i * N + m], sx[i], v);
         v = fma(tB[i * N + m], so[i], v);
       }
-      const int id = ids[m];
-      if (id >= 0) atomic_add(ybar + id, v);
+      if (ids[m] >= 0) atomic_add(ybar + ids[m], v);
     }
   }
 }
@@ -268,23 +275,34 @@ __global__ void __launch_bounds__((Adj2Shape<N, Q>::NT))
   const int64_t base = (int64_t)blk * S::EB;
   const int tid = threadIdx.x;
 
-  for (int t = tid; t < S::EB * N; t += S::NT) {
-    const int e = t / N, r = t - e * N;
-    const int64_t ge = base + e;
+  static_assert(S::NT >= S::EB * N, "one x-pencil per thread");
+  int ids[N];
+  const bool own1 = tid < S::EB * N;
+  const int e1 = tid / N, r1 = tid - e1 * N;
+  const int64_t ge1 = base + e1;
+  const bool act1 = own1 && ge1 < ncell;
+  if (own1) {
     double uu[N], ll[N];
-    if (ge < ncell) {
-      const int32_t* ids = cells + (ge * N + r) * N;
+    if (act1) {
+      const int32_t* cid = cells + (ge1 * N + r1) * N;
+#pragma unroll
+      for (int j = 0; j < N; ++j) ids[j] = cid[j];
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        const int id = ids[j];
-        uu[j] = id >= 0 ? u[id] : 0.0;
-        ll[j] = id >= 0 ? lam[id] : 0.0;
+        const int k = ids[j] >= 0 ? ids[j] : 0;
+        uu[j] = u[k];
+        ll[j] = lam[k];
+      }
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        uu[j] = ids[j] >= 0 ? uu[j] : 0.0;
+        ll[j] = ids[j] >= 0 ? ll[j] : 0.0;
       }
     } else {
 #pragma unroll
       for (int j = 0; j < N; ++j) uu[j] = ll[j] = 0.0;
     }
-    double* sA = smem + e * S::SE;
+    double* sA = smem + e1 * S::SE;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       double bu = 0, du = 0, bl = 0, dl = 0;
@@ -295,10 +313,10 @@ __global__ void __launch_bounds__((Adj2Shape<N, Q>::NT))
         bl = fma(tB[i * N + j], ll[j], bl);
         dl = fma(tD[i * N + j], ll[j], dl);
       }
-      sA[r * S::QP + i] = bu;
-      sA[S::SA1 + r * S::QP + i] = du;
-      sA[2 * S::SA1 + r * S::QP + i] = bl;
-      sA[3 * S::SA1 + r * S::QP + i] = dl;
+      sA[r1 * S::QP + i] = bu;
+      sA[S::SA1 + r1 * S::QP + i] = du;
+      sA[2 * S::SA1 + r1 * S::QP + i] = bl;
+      sA[3 * S::SA1 + r1 * S::QP + i] = dl;
     }
   }
   __syncthreads();
@@ -359,18 +377,14 @@ __global__ void __launch_bounds__((Adj2Shape<N, Q>::NT))
   if (!want_y) return;
   __syncthreads();
 
-  for (int t = tid; t < S::EB * N; t += S::NT) {
-    const int e = t / N, r = t - e * N;
-    const int64_t ge = base + e;
-    if (ge >= ncell) continue;
-    const double* sA = smem + e * S::SE;
+  if (act1) {
+    const double* sA = smem + e1 * S::SE;
     double sx[Q], so[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      sx[i] = sA[r * S::QP + i];
-      so[i] = sA[S::SA1 + r * S::QP + i];
+      sx[i] = sA[r1 * S::QP + i];
+      so[i] = sA[S::SA1 + r1 * S::QP + i];
     }
-    const int32_t* ids = cells + (ge * N + r) * N;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
       double v = 0;
@@ -379,8 +393,7 @@ __global__ void __launch_bounds__((Adj2Shape<N, Q>::NT))
         v = fma(tD[i * N + m], sx[i], v);
         v = fma(tB[i * N + m], so[i], v);
       }
-      const int id = ids[m];
-      if (id >= 0) atomic_add(ybar + id, v);
+      if (ids[m] >= 0) atomic_add(ybar + ids[m], v);
     }
   }
 }

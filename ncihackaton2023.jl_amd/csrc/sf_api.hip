@@ -293,6 +293,7 @@ int clear_flag(sf_plan* P) {
 
 // y = A x (FP64) on stream s, y overwritten.
 int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s) {
+  if (P->nfree == 0) return SF_OK;  // nothing to compute (and x may be empty)
   SF_HIP(launch_fill<double>(y, P->nfree, 0.0, s));
   hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
                            P->dB.p, P->dD.p, P->ncell, s);
@@ -329,6 +330,7 @@ int prepare(sf_plan* P, int precision) {
 // y = A x in reduced precision (FP32 vectors), y overwritten.
 int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
                  hipStream_t s) {
+  if (P->nfree == 0) return SF_OK;
   SF_HIP(launch_fill<float>(y, P->nfree, 0.0f, s));
   hipError_t e;
   if (precision == SF_PREC_F32) {
@@ -775,6 +777,7 @@ int sf_apply_add(sf_plan* P, double* y, const double* x, void* stream) {
   int rc = check_ready(P);
   if (rc) return rc;
   if ((!x || !y) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
+  if (P->nfree == 0) return SF_OK;
   DeviceGuard g(P->device);
   hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
                            P->dB.p, P->dD.p, P->ncell, (hipStream_t)stream);
@@ -827,6 +830,10 @@ int sf_adjoint(sf_plan* P, const double* u, const double* lam,
   if ((!u || !lam) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
   DeviceGuard g(P->device);
   hipStream_t s = (hipStream_t)stream;
+  if (P->nfree == 0) {  // every dof Dirichlet: u = lam = 0, so dJ/dkappa = 0
+    if (dJdkappa) SF_HIP(launch_fill<double>(dJdkappa, P->ncell * P->nqc, 0.0, s));
+    return SF_OK;
+  }
   if (dJdu) SF_HIP(launch_fill<double>(dJdu, P->nfree, 0.0, s));
   hipError_t e = adjoint_f64(P->dim, P->n, P->q, u, lam, dJdkappa, dJdu,
                              P->cells.p, P->G.p, P->has_kappa ? P->kappa.p : nullptr,

@@ -61,23 +61,33 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   const int64_t base = (int64_t)blk * S::EB;
   const int tid = threadIdx.x;
 
+  // Every stage has at most one task per thread (NT >= EB * MAXT), so the
+  // thread that gathers x-pencil (e, r) in stage 1 also scatters it in
+  // stage 5 and keeps its N dof ids in registers in between.
+  static_assert(S::NT >= S::EB * S::MAXT, "one task per thread per stage");
+
   // ---- stage 1: gather + x-pencils:  Bx u, Dx u ------------------------
-  for (int t = tid; t < S::EB * S::NN; t += S::NT) {
-    const int e = t / S::NN, r = t - e * S::NN;  // r = z*N + y
-    const int64_t ge = base + e;
+  int ids[N];
+  const bool own1 = tid < S::EB * S::NN;
+  const int e1 = tid / S::NN, r1 = tid - e1 * S::NN;  // r1 = z*N + y
+  const int64_t ge1 = base + e1;
+  const bool act1 = own1 && ge1 < ncell;
+  if (own1) {
     T u[N];
-    if (ge < ncell) {
-      const int32_t* ids = cells + (ge * S::NN + r) * N;
+    if (act1) {
+      const int32_t* cid = cells + (ge1 * S::NN + r1) * N;
 #pragma unroll
-      for (int j = 0; j < N; ++j) {
-        const int id = ids[j];
-        u[j] = id >= 0 ? x[id] : T(0);
-      }
+      for (int j = 0; j < N; ++j) ids[j] = cid[j];
+      // branch-free gather: every lane issues all N loads back to back
+#pragma unroll
+      for (int j = 0; j < N; ++j) u[j] = x[ids[j] >= 0 ? ids[j] : 0];
+#pragma unroll
+      for (int j = 0; j < N; ++j) u[j] = ids[j] >= 0 ? u[j] : T(0);
     } else {
 #pragma unroll
       for (int j = 0; j < N; ++j) u[j] = T(0);
     }
-    T* sA = smem + e * S::SE;
+    T* sA = smem + e1 * S::SE;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       T b = T(0), d = T(0);
@@ -86,14 +96,15 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
         b = fma(tB[i * N + j], u[j], b);
         d = fma(tD[i * N + j], u[j], d);
       }
-      sA[r * S::QP + i] = b;
-      sA[S::SA1 + r * S::QP + i] = d;
+      sA[r1 * S::QP + i] = b;
+      sA[S::SA1 + r1 * S::QP + i] = d;
     }
   }
   __syncthreads();
 
   // ---- stage 2: y-pencils: (Dx u, Bx u) -> By Dx u, Dy Bx u, By Bx u ------
-  for (int t = tid; t < S::EB * S::NQ; t += S::NT) {
+  if (tid < S::EB * S::NQ) {
+    const int t = tid;
     const int e = t / S::NQ, r = t - e * S::NQ;
     const int z = r / Q, i = r - z * Q;
     const T* sA = smem + e * S::SE;
@@ -122,7 +133,8 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   __syncthreads();
 
   // ---- stage 3: z-columns: gradients, G, transposed z contraction --------
-  for (int t = tid; t < S::EB * S::QQ; t += S::NT) {
+  if (tid < S::EB * S::QQ) {
+    const int t = tid;
     const int e = t / S::QQ, r = t - e * S::QQ;  // r = k*Q + i
     const int64_t ge = base + e;
     T* sB = smem + e * S::SE + 2 * S::SA1;
@@ -176,7 +188,8 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   __syncthreads();
 
   // ---- stage 4: transposed y-pencils --------------------------------------
-  for (int t = tid; t < S::EB * S::NQ; t += S::NT) {
+  if (tid < S::EB * S::NQ) {
+    const int t = tid;
     const int e = t / S::NQ, r = t - e * S::NQ;
     const int z = r / Q, i = r - z * Q;
     T* sA = smem + e * S::SE;
@@ -205,18 +218,14 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   __syncthreads();
 
   // ---- stage 5: transposed x-pencils + scatter-add -------------------------
-  for (int t = tid; t < S::EB * S::NN; t += S::NT) {
-    const int e = t / S::NN, r = t - e * S::NN;
-    const int64_t ge = base + e;
-    if (ge >= ncell) continue;
-    const T* sA = smem + e * S::SE;
+  if (act1) {
+    const T* sA = smem + e1 * S::SE;
     T sx[Q], so[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      sx[i] = sA[r * S::QP + i];
-      so[i] = sA[S::SA1 + r * S::QP + i];
+      sx[i] = sA[r1 * S::QP + i];
+      so[i] = sA[S::SA1 + r1 * S::QP + i];
     }
-    const int32_t* ids = cells + (ge * S::NN + r) * N;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
       T v = T(0);
@@ -225,8 +234,7 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
         v = fma(tD[i * N + m], sx[i], v);
         v = fma(tB[i * N + m], so[i], v);
       }
-      const int id = ids[m];
-      if (id >= 0) atomic_add(y + id, v);
+      if (ids[m] >= 0) atomic_add(y + ids[m], v);
     }
   }
 }
@@ -257,23 +265,29 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
   const int64_t base = (int64_t)blk * S::EB;
   const int tid = threadIdx.x;
 
-  // stage 1: gather + x-pencils (one per node row y)
-  for (int t = tid; t < S::EB * N; t += S::NT) {
-    const int e = t / N, r = t - e * N;
-    const int64_t ge = base + e;
+  static_assert(S::NT >= S::EB * S::MAXT, "one task per thread per stage");
+
+  // stage 1: gather + x-pencils (one per node row y); ids stay in registers
+  int ids[N];
+  const bool own1 = tid < S::EB * N;
+  const int e1 = tid / N, r1 = tid - e1 * N;
+  const int64_t ge1 = base + e1;
+  const bool act1 = own1 && ge1 < ncell;
+  if (own1) {
     T u[N];
-    if (ge < ncell) {
-      const int32_t* ids = cells + (ge * N + r) * N;
+    if (act1) {
+      const int32_t* cid = cells + (ge1 * N + r1) * N;
 #pragma unroll
-      for (int j = 0; j < N; ++j) {
-        const int id = ids[j];
-        u[j] = id >= 0 ? x[id] : T(0);
-      }
+      for (int j = 0; j < N; ++j) ids[j] = cid[j];
+#pragma unroll
+      for (int j = 0; j < N; ++j) u[j] = x[ids[j] >= 0 ? ids[j] : 0];
+#pragma unroll
+      for (int j = 0; j < N; ++j) u[j] = ids[j] >= 0 ? u[j] : T(0);
     } else {
 #pragma unroll
       for (int j = 0; j < N; ++j) u[j] = T(0);
     }
-    T* sA = smem + e * S::SE;
+    T* sA = smem + e1 * S::SE;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       T b = T(0), d = T(0);
@@ -282,14 +296,15 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
         b = fma(tB[i * N + j], u[j], b);
         d = fma(tD[i * N + j], u[j], d);
       }
-      sA[r * S::QP + i] = b;
-      sA[S::SA1 + r * S::QP + i] = d;
+      sA[r1 * S::QP + i] = b;
+      sA[S::SA1 + r1 * S::QP + i] = d;
     }
   }
   __syncthreads();
 
   // stage 2: y-columns: gradients, G, transposed y contraction (in place)
-  for (int t = tid; t < S::EB * Q; t += S::NT) {
+  if (tid < S::EB * Q) {
+    const int t = tid;
     const int e = t / Q, i = t - e * Q;
     const int64_t ge = base + e;
     T* sA = smem + e * S::SE;
@@ -334,18 +349,14 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
   __syncthreads();
 
   // stage 3: transposed x-pencils + scatter-add
-  for (int t = tid; t < S::EB * N; t += S::NT) {
-    const int e = t / N, r = t - e * N;
-    const int64_t ge = base + e;
-    if (ge >= ncell) continue;
-    const T* sA = smem + e * S::SE;
+  if (act1) {
+    const T* sA = smem + e1 * S::SE;
     T sx[Q], so[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      sx[i] = sA[r * S::QP + i];
-      so[i] = sA[S::SA1 + r * S::QP + i];
+      sx[i] = sA[r1 * S::QP + i];
+      so[i] = sA[S::SA1 + r1 * S::QP + i];
     }
-    const int32_t* ids = cells + (ge * N + r) * N;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
       T v = T(0);
@@ -354,8 +365,7 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
         v = fma(tD[i * N + m], sx[i], v);
         v = fma(tB[i * N + m], so[i], v);
       }
-      const int id = ids[m];
-      if (id >= 0) atomic_add(y + id, v);
+      if (ids[m] >= 0) atomic_add(y + ids[m], v);
     }
   }
 }

@@ -113,8 +113,10 @@ def test_adjoint_outputs_optional(pkg, cuda):
     kb2, _ = pkg.adjoint.pullback(A, u, lam, want_u=False)
     _, ub3 = pkg.adjoint.pullback(A, u, lam, want_kappa=False)
     cuda.cuda.synchronize()
+    # dJ/dkappa is written per quadrature point: bitwise reproducible;
+    # dJ/du is an atomic scatter-add: equal up to summation order
     assert cuda.equal(kb1, kb2)
-    assert cuda.equal(ub1, ub3)
+    assert rel_l2(ub1.cpu().numpy(), ub3.cpu().numpy()) < 1e-15
     # without kappa, dJ/du = A lam
     y = cuda.empty_like(lam)
     A.mul(y, lam)
