@@ -65,6 +65,8 @@ typedef struct sf_plan_info {
   int threads_per_block;   /* launch shape of the FP64 apply kernel           */
   int cells_per_block;
   int64_t device_bytes;    /* device memory the plan holds                   */
+  int n_blocks;            /* 3D: cell blocks (cells_per_block each)          */
+  int n_colors;            /* 3D: colour phases = kernel launches per apply  */
 } sf_plan_info;
 
 typedef struct sf_solve_stats {

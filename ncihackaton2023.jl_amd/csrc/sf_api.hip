@@ -6,11 +6,13 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <new>
 #include <string>
 #include <vector>
 
+#include "sf_apply.hpp"
 #include "sf_common.hpp"
 #include "sf_kernels.h"
 #include "sfhip.h"
@@ -204,6 +206,13 @@ struct sf_plan {
   bool has_cells = false, has_geom = false, has_kappa = false;
   bool g32_valid = false, gbf_valid = false;
   DevBuf<int> flag;
+  // 3D block structure (block-assembled, colour-phased apply)
+  int eb = 0, nblk = 0, ncolor = 0;
+  std::vector<int> phase_off;
+  DevBuf<uint16_t> pidx;
+  DevBuf<int32_t> bdofs, bfirst, border, untouched;
+  DevBuf<int64_t> boff, bcell;
+  BlockArgs kargs{};
   // staging of the host-pointer apply
   DevBuf<double> xs, ys;
   // solver workspace
@@ -216,12 +225,15 @@ struct sf_plan {
   hipStream_t own_stream = nullptr;
 
   const double* active_G() const { return has_kappa ? Gk.p : G.p; }
+  const BlockArgs* blocks() const { return dim == 3 ? &kargs : nullptr; }
 
   void release_all() {
     dB.release(); dD.release(); dW.release(); dPts.release();
     dB32.release(); dD32.release();
     cells.release(); G.release(); Gk.release(); kappa.release();
     G32.release(); Gbf.release(); flag.release();
+    pidx.release(); bdofs.release(); bfirst.release(); border.release();
+    untouched.release(); boff.release(); bcell.release();
     xs.release(); ys.release();
     w_r.release(); w_p.release(); w_q.release(); w_t.release();
     w_r32.release(); w_s32.release(); w_d32.release(); w_p32.release(); w_q32.release();
@@ -234,6 +246,8 @@ struct sf_plan {
     return dB.bytes() + dD.bytes() + dW.bytes() + dPts.bytes() + dB32.bytes() +
            dD32.bytes() + cells.bytes() + G.bytes() + Gk.bytes() +
            kappa.bytes() + G32.bytes() + Gbf.bytes() + flag.bytes() +
+           pidx.bytes() + bdofs.bytes() + bfirst.bytes() + border.bytes() +
+           untouched.bytes() + boff.bytes() + bcell.bytes() +
            xs.bytes() + ys.bytes() + w_r.bytes() + w_p.bytes() + w_q.bytes() +
            w_t.bytes() + w_r32.bytes() + w_s32.bytes() + w_d32.bytes() + w_p32.bytes() +
            w_q32.bytes() + red_part.bytes() + red_counter.bytes() +
@@ -291,12 +305,15 @@ int clear_flag(sf_plan* P) {
   return SF_OK;
 }
 
-// y = A x (FP64) on stream s, y overwritten.
-int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s) {
+// y = A x (FP64) on stream s; accumulate: y += A x.
+int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
+                 int accumulate = 0) {
   if (P->nfree == 0) return SF_OK;  // nothing to compute (and x may be empty)
-  SF_HIP(launch_fill<double>(y, P->nfree, 0.0, s));
+  if (P->dim == 2 && !accumulate) SF_HIP(launch_fill<double>(y, P->nfree, 0.0, s));
+  if (P->dim == 3 && !accumulate && P->untouched.n > 0)
+    SF_HIP(launch_zero_list<double>(y, P->untouched.p, P->untouched.n, s));
   hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
-                           P->dB.p, P->dD.p, P->ncell, s);
+                           P->dB.p, P->dD.p, P->ncell, P->blocks(), accumulate, s);
   if (e == hipErrorNotSupported)
     return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
   SF_HIP(e);
@@ -331,20 +348,195 @@ int prepare(sf_plan* P, int precision) {
 int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
                  hipStream_t s) {
   if (P->nfree == 0) return SF_OK;
-  SF_HIP(launch_fill<float>(y, P->nfree, 0.0f, s));
+  if (P->dim == 2) SF_HIP(launch_fill<float>(y, P->nfree, 0.0f, s));
+  if (P->dim == 3 && P->untouched.n > 0)
+    SF_HIP(launch_zero_list<float>(y, P->untouched.p, P->untouched.n, s));
   hipError_t e;
   if (precision == SF_PREC_F32) {
     e = apply_f32(P->dim, P->n, P->q, x, y, P->cells.p, P->G32.p, P->dB32.p,
-                  P->dD32.p, P->ncell, s);
+                  P->dD32.p, P->ncell, P->blocks(), 0, s);
   } else if (precision == SF_PREC_F32_GBF16) {
     e = apply_f32_gbf16(P->dim, P->n, P->q, x, y, P->cells.p, P->Gbf.p,
-                        P->dB32.p, P->dD32.p, P->ncell, s);
+                        P->dB32.p, P->dD32.p, P->ncell, P->blocks(), 0, s);
   } else {
     return fail(SF_ERR_UNSUPPORTED, "precision %d has no FP32 apply yet", precision);
   }
   if (e == hipErrorNotSupported)
     return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
   SF_HIP(e);
+  return SF_OK;
+}
+
+// Block structure of the 3D apply from the tensor-ordered, 0-based cell
+// dofs h (negative = Dirichlet):
+//  * blocks of at most eb consecutive cells (eb = the kernel's cells per
+//    workgroup): the cell sequence is cut where two consecutive cells share
+//    no free dof (e.g. at the end of a mesh row) and every run is split into
+//    near-equal blocks, so blocks of neighbouring runs line up;
+//  * per block, its distinct free dofs, sorted;
+//  * greedy colouring of the blocks (two blocks sharing a dof get different
+//    colours) -- every colour is one launch, so writes need no atomics;
+//  * per block, the dofs it is the first (lowest colour) to reach go first:
+//    the kernel stores those and accumulates the rest, so y needs no fill;
+//  * per cell dof, its index in the block's list (uint16, 0xffff =
+//    Dirichlet).
+int build_blocks(sf_plan* P, const std::vector<int32_t>& h) {
+  int nt = 0, eb = 0;
+  apply_shape_f64(3, P->n, P->q, &nt, &eb);
+  if (eb <= 0) return fail(SF_ERR_UNSUPPORTED, "no kernel for n=%d q=%d", P->n, P->q);
+  const int64_t E = P->ncell, nl = P->nloc, nf = P->nfree;
+  // runs of consecutive cells that share a free dof with their predecessor
+  std::vector<int64_t> bcell;
+  {
+    std::vector<int32_t> prev, cur;
+    int64_t run0 = 0;
+    auto close_run = [&](int64_t r0, int64_t r1) {
+      const int64_t len = r1 - r0;
+      const int64_t nbk = (len + eb - 1) / eb;
+      for (int64_t k = 0; k < nbk; ++k) bcell.push_back(r0 + (k * len) / nbk);
+    };
+    for (int64_t c = 0; c < E; ++c) {
+      cur.clear();
+      for (int64_t i = c * nl; i < (c + 1) * nl; ++i)
+        if (h[i] >= 0) cur.push_back(h[i]);
+      std::sort(cur.begin(), cur.end());
+      if (c > 0) {
+        bool shared = false;
+        size_t a = 0, bb = 0;
+        while (a < prev.size() && bb < cur.size()) {
+          if (prev[a] == cur[bb]) { shared = true; break; }
+          if (prev[a] < cur[bb]) ++a; else ++bb;
+        }
+        if (!shared) {
+          close_run(run0, c);
+          run0 = c;
+        }
+      }
+      prev.swap(cur);
+    }
+    if (E > 0) close_run(run0, E);
+    bcell.push_back(E);
+  }
+  const int64_t nb = (int64_t)bcell.size() - 1;
+  if (nb > 0x7fffffffLL) return fail(SF_ERR_ARG, "too many cell blocks");
+  // distinct sorted dofs per block
+  std::vector<int64_t> off(nb + 1, 0);
+  std::vector<int32_t> dofs;
+  dofs.reserve((size_t)(E * nl));
+  std::vector<int32_t> tmp;
+  for (int64_t b = 0; b < nb; ++b) {
+    tmp.clear();
+    const int64_t c0 = bcell[b], c1 = bcell[b + 1];
+    for (int64_t i = c0 * nl; i < c1 * nl; ++i)
+      if (h[i] >= 0) tmp.push_back(h[i]);
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    dofs.insert(dofs.end(), tmp.begin(), tmp.end());
+    off[b + 1] = (int64_t)dofs.size();
+  }
+  // dof -> blocks (CSR)
+  std::vector<int64_t> dptr(nf + 1, 0);
+  for (int32_t d : dofs) ++dptr[d + 1];
+  for (int64_t d = 0; d < nf; ++d) dptr[d + 1] += dptr[d];
+  std::vector<int32_t> dblk(dofs.size());
+  {
+    std::vector<int64_t> fillp(dptr.begin(), dptr.end() - 1);
+    for (int64_t b = 0; b < nb; ++b)
+      for (int64_t i = off[b]; i < off[b + 1]; ++i) dblk[fillp[dofs[i]]++] = (int32_t)b;
+  }
+  // greedy colouring in block order
+  std::vector<int32_t> color(nb, -1);
+  std::vector<int64_t> stamp;
+  int ncolor = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    for (int64_t i = off[b]; i < off[b + 1]; ++i) {
+      const int32_t d = dofs[i];
+      for (int64_t j = dptr[d]; j < dptr[d + 1]; ++j) {
+        const int32_t c = color[dblk[j]];
+        if (c >= 0) {
+          if ((size_t)c >= stamp.size()) stamp.resize(c + 1, -1);
+          stamp[c] = b;
+        }
+      }
+    }
+    int c = 0;
+    while ((size_t)c < stamp.size() && stamp[c] == b) ++c;
+    color[b] = c;
+    if (c + 1 > ncolor) ncolor = c + 1;
+  }
+  // first-touch split and local indices
+  std::vector<int32_t> first(nb, 0);
+  std::vector<uint16_t> li((size_t)(E * nl), kNoDof);
+  std::vector<int32_t> part;
+  for (int64_t b = 0; b < nb; ++b) {
+    const int32_t cb = color[b];
+    part.clear();
+    int nfirst = 0;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int64_t i = off[b]; i < off[b + 1]; ++i) {
+        const int32_t d = dofs[i];
+        int32_t cmin = cb;
+        for (int64_t j = dptr[d]; j < dptr[d + 1]; ++j) cmin = std::min(cmin, color[dblk[j]]);
+        const bool is_first = (cmin == cb);
+        if (is_first == (pass == 0)) {
+          part.push_back(d);
+          if (pass == 0) ++nfirst;
+        }
+      }
+    std::copy(part.begin(), part.end(), dofs.begin() + off[b]);
+    first[b] = nfirst;
+    if (off[b + 1] - off[b] > 0xfffe) return fail(SF_ERR_ARG, "block with too many dofs");
+    // position lookup: sorted (dof, position) pairs of this block
+    std::vector<std::pair<int32_t, int32_t>> pos;
+    pos.reserve(part.size());
+    for (size_t i = 0; i < part.size(); ++i) pos.push_back({part[i], (int32_t)i});
+    std::sort(pos.begin(), pos.end());
+    const int64_t c0 = bcell[b], c1 = bcell[b + 1];
+    for (int64_t i = c0 * nl; i < c1 * nl; ++i) {
+      if (h[i] < 0) continue;
+      auto it = std::lower_bound(pos.begin(), pos.end(), std::make_pair(h[i], (int32_t)-1));
+      li[i] = (uint16_t)it->second;
+    }
+  }
+  // blocks sorted by colour
+  std::vector<int> phase_off(ncolor + 1, 0);
+  for (int64_t b = 0; b < nb; ++b) ++phase_off[color[b] + 1];
+  for (int c = 0; c < ncolor; ++c) phase_off[c + 1] += phase_off[c];
+  std::vector<int32_t> order(nb);
+  {
+    std::vector<int> fp(phase_off.begin(), phase_off.end() - 1);
+    for (int64_t b = 0; b < nb; ++b) order[fp[color[b]]++] = (int32_t)b;
+  }
+  // free dofs no cell touches: their rows are empty, y = 0 there
+  std::vector<int32_t> untouched;
+  for (int64_t d = 0; d < nf; ++d)
+    if (dptr[d + 1] == dptr[d]) untouched.push_back((int32_t)d);
+  // upload
+  SF_HIP(P->pidx.alloc((int64_t)li.size()));
+  SF_HIP(P->bdofs.alloc((int64_t)std::max<size_t>(dofs.size(), 1)));
+  SF_HIP(P->boff.alloc(nb + 1));
+  SF_HIP(P->bcell.alloc(nb + 1));
+  SF_HIP(hipMemcpy(P->bcell.p, bcell.data(), (nb + 1) * 8, hipMemcpyHostToDevice));
+  SF_HIP(P->bfirst.alloc(std::max<int64_t>(nb, 1)));
+  SF_HIP(P->border.alloc(std::max<int64_t>(nb, 1)));
+  if (!li.empty()) SF_HIP(hipMemcpy(P->pidx.p, li.data(), li.size() * 2, hipMemcpyHostToDevice));
+  if (!dofs.empty()) SF_HIP(hipMemcpy(P->bdofs.p, dofs.data(), dofs.size() * 4, hipMemcpyHostToDevice));
+  SF_HIP(hipMemcpy(P->boff.p, off.data(), (nb + 1) * 8, hipMemcpyHostToDevice));
+  if (nb > 0) {
+    SF_HIP(hipMemcpy(P->bfirst.p, first.data(), nb * 4, hipMemcpyHostToDevice));
+    SF_HIP(hipMemcpy(P->border.p, order.data(), nb * 4, hipMemcpyHostToDevice));
+  }
+  P->untouched.release();
+  if (!untouched.empty()) {
+    SF_HIP(P->untouched.alloc((int64_t)untouched.size()));
+    SF_HIP(hipMemcpy(P->untouched.p, untouched.data(), untouched.size() * 4, hipMemcpyHostToDevice));
+  }
+  P->eb = eb;
+  P->nblk = (int)nb;
+  P->ncolor = ncolor;
+  P->phase_off = phase_off;
+  P->kargs = BlockArgs{P->pidx.p,  P->bdofs.p,          P->boff.p, P->bfirst.p,
+                       P->border.p, P->bcell.p, P->phase_off.data(), ncolor};
   return SF_OK;
 }
 
@@ -406,13 +598,11 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   while (!h.done && h.iters < maxit) {
     const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
     for (int k = 0; k < todo; ++k) {
-      hipError_t e = apply_f64(P->dim, P->n, P->q, P->w_p.p, P->w_q.p,
-                               P->cells.p, P->active_G(), P->dB.p, P->dD.p,
-                               P->ncell, s);
-      SF_HIP(e);
+      rc = do_apply_f64(P, P->w_q.p, P->w_p.p, s);
+      if (rc) return rc;
       SF_HIP(launch_cg_alpha<double>(P->w_p.p, P->w_q.p, P->nfree, red, P->scal.p, s));
       SF_HIP(launch_cg_update<double>(x, P->w_r.p, P->w_p.p, P->w_q.p, P->nfree, red, P->scal.p, s));
-      SF_HIP(launch_cg_dir<double>(P->w_p.p, P->w_r.p, P->w_q.p, P->nfree, P->scal.p, s));
+      SF_HIP(launch_cg_dir<double>(P->w_p.p, P->w_r.p, P->nfree, P->scal.p, s));
     }
     rc = read_scalars(P, &h, s);
     if (rc) return rc;
@@ -443,17 +633,11 @@ int cg_f32_inner(sf_plan* P, int precision, double inner_rtol, int maxit,
   while (!h.done && h.iters < maxit) {
     const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
     for (int k = 0; k < todo; ++k) {
-      hipError_t e;
-      if (precision == SF_PREC_F32)
-        e = apply_f32(P->dim, P->n, P->q, p, q, P->cells.p, P->G32.p, P->dB32.p,
-                      P->dD32.p, P->ncell, s);
-      else
-        e = apply_f32_gbf16(P->dim, P->n, P->q, p, q, P->cells.p, P->Gbf.p,
-                            P->dB32.p, P->dD32.p, P->ncell, s);
-      SF_HIP(e);
+      rc = do_apply_f32(P, q, p, precision, s);
+      if (rc) return rc;
       SF_HIP(launch_cg_alpha<float>(p, q, P->nfree, red, P->scal.p, s));
       SF_HIP(launch_cg_update<float>(d, r, p, q, P->nfree, red, P->scal.p, s));
-      SF_HIP(launch_cg_dir<float>(p, r, q, P->nfree, P->scal.p, s));
+      SF_HIP(launch_cg_dir<float>(p, r, P->nfree, P->scal.p, s));
     }
     rc = read_scalars(P, &h, s);
     if (rc) return rc;
@@ -593,6 +777,8 @@ int sf_plan_get_info(const sf_plan* P, sf_plan_info* info) {
   info->has_kappa = P->has_kappa;
   apply_shape_f64(P->dim, P->n, P->q, &info->threads_per_block, &info->cells_per_block);
   info->device_bytes = P->device_bytes();
+  info->n_blocks = P->nblk;
+  info->n_colors = P->ncolor;
   return SF_OK;
   SF_CATCH
 }
@@ -656,6 +842,11 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
   SF_HIP(P->cells.alloc(P->ncell * nl));
   if (!h.empty())
     SF_HIP(hipMemcpy(P->cells.p, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (P->dim == 3) {
+    P->has_cells = false;
+    int rc = build_blocks(P, h);
+    if (rc) return rc;
+  }
   P->has_cells = true;
   return SF_OK;
   SF_CATCH
@@ -779,12 +970,7 @@ int sf_apply_add(sf_plan* P, double* y, const double* x, void* stream) {
   if ((!x || !y) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
   if (P->nfree == 0) return SF_OK;
   DeviceGuard g(P->device);
-  hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
-                           P->dB.p, P->dD.p, P->ncell, (hipStream_t)stream);
-  if (e == hipErrorNotSupported)
-    return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
-  SF_HIP(e);
-  return SF_OK;
+  return do_apply_f64(P, y, x, (hipStream_t)stream, 1);
   SF_CATCH
 }
 

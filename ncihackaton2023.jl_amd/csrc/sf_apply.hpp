@@ -33,82 +33,153 @@ namespace sf {
 template <int N, int Q, typename T>
 struct Apply3Shape {
   static constexpr int NN = N * N, NQ = N * Q, QQ = Q * Q, QQQ = Q * Q * Q;
+  static constexpr int NNN = N * N * N;
   static constexpr int QP = (Q % 2 == 0) ? Q + 1 : Q;
   static constexpr int SA1 = NN * QP;  // one x-pass tensor [z][y][i]
   static constexpr int SB1 = N * QQ;   // one y-pass tensor [z][k][i]
   static constexpr int SE = 2 * SA1 + 3 * SB1;
   static constexpr int MAXT = cmax(NN, cmax(NQ, QQ));
   // cells per workgroup: fill ~256 lanes in the widest stage, within 64 KiB
+  // of FP64 LDS (the same EB for every T, so FP64 and FP32 applies share the
+  // host-side block structure)
   static constexpr int EB_T = MAXT > 100 ? 1 : cmax(1, 256 / MAXT);
-  static constexpr int EB_L = cmax(1, 65536 / (SE * (int)sizeof(T)));
+  static constexpr int EB_L = cmax(1, 65536 / (SE * 8));
   static constexpr int EB = EB_T < EB_L ? EB_T : EB_L;
   static constexpr int NT = round_up(EB * MAXT, 64);
-  static constexpr int LDS_BYTES = EB * SE * (int)sizeof(T);
+  // LDS: region A = EB x (2 SA1), region B = EB x (3 SB1); region B also
+  // holds the block's gathered x values and its assembled y values (at most
+  // EB * N^3 distinct dofs)
+  static constexpr int RA = EB * 2 * SA1;
+  static constexpr int RB = cmax(EB * 3 * SB1, EB * NNN);
+  static constexpr int LDS_BYTES = (RA + RB) * (int)sizeof(T);
+  // distinct dofs of a block handled per thread in the gather / write-back
+  static constexpr int KU = (EB * NNN + NT - 1) / NT;
   static_assert(LDS_BYTES <= 65536, "apply3d: LDS per workgroup above 64 KiB");
+  static_assert(NT >= EB * MAXT, "one task per thread per stage");
 };
 
-// y += A x restricted to the cells of this launch (y must be zeroed by the
-// caller; contributions are added with hardware atomics).
+constexpr uint16_t kNoDof = 0xffff;  // local index of a Dirichlet dof
+
+// Block-assembled apply of one colour (phase) of cell blocks.
+//
+// A workgroup owns block b = order[blockIdx]: cells bcell[b] ..
+// bcell[b+1] - 1, at most EB of them, consecutive in the caller's order.  The
+// host has listed the block's distinct free dofs (bdofs[boff[b] ..
+// boff[b+1]), sorted, the ones this block touches first -- lowest colour --
+// in front: bfirst[b] of them) and, for every cell dof, its index in that
+// list (pidx, kNoDof for Dirichlet dofs).  Blocks of one colour share no dof,
+// so the write-back needs no atomics: the first colour to reach a dof stores
+// it, later colours read-modify-write.  y needs no zero fill and every global
+// access of x and y runs over sorted, mostly contiguous ids.
 template <int N, int Q, typename T, typename TG>
 __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     k_apply3d(const T* __restrict__ x, T* __restrict__ y,
-              const int32_t* __restrict__ cells, const TG* __restrict__ G,
-              const T* __restrict__ tB, const T* __restrict__ tD,
-              int64_t ncell, int nblk) {
+              const uint16_t* __restrict__ pidx,
+              const int32_t* __restrict__ bdofs,
+              const int64_t* __restrict__ boff,
+              const int32_t* __restrict__ bfirst,
+              const int32_t* __restrict__ order, int nblk,
+              const int64_t* __restrict__ bcell,
+              const TG* __restrict__ G, const T* __restrict__ tB,
+              const T* __restrict__ tD, int accumulate) {
   using S = Apply3Shape<N, Q, T>;
-  __shared__ T smem[S::EB * S::SE];
-  const int blk = xcd_remap(blockIdx.x, nblk);
-  const int64_t base = (int64_t)blk * S::EB;
+  __shared__ T smem[S::RA + S::RB];
+  T* const regA = smem;
+  T* const regB = smem + S::RA;
+  const int b = order[xcd_remap(blockIdx.x, nblk)];
+  const int64_t base = bcell[b];
+  const int ncb = (int)(bcell[b + 1] - base);  // cells in this block, <= EB
   const int tid = threadIdx.x;
+  const int64_t d0 = boff[b];
+  const int nu = (int)(boff[b + 1] - d0);
+  const int nfirst = accumulate ? 0 : bfirst[b];
+  const int32_t* __restrict__ dofs = bdofs + d0;
 
-  // Every stage has at most one task per thread (NT >= EB * MAXT), so the
-  // thread that gathers x-pencil (e, r) in stage 1 also scatters it in
-  // stage 5 and keeps its N dof ids in registers in between.
-  static_assert(S::NT >= S::EB * S::MAXT, "one task per thread per stage");
+  // ---- prologue: every global read of the block, issued at once ---------
+  // In the order the stages need them (x values, local indices, G, y), so
+  // waiting for the first ones leaves the later ones in flight: the G
+  // stream and the y read of the write-back overlap the gather and the
+  // first two contraction stages.
+  int32_t dv[S::KU];
+  T xv[S::KU];
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    dv[k] = i < nu ? dofs[i] : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) xv[k] = x[dv[k] >= 0 ? dv[k] : 0];
 
-  // ---- stage 1: gather + x-pencils:  Bx u, Dx u ------------------------
-  int ids[N];
+  uint16_t li[N];
   const bool own1 = tid < S::EB * S::NN;
   const int e1 = tid / S::NN, r1 = tid - e1 * S::NN;  // r1 = z*N + y
   const int64_t ge1 = base + e1;
-  const bool act1 = own1 && ge1 < ncell;
+  const bool act1 = own1 && e1 < ncb;
+  if (act1) {
+    const uint16_t* lp = pidx + (ge1 * S::NN + r1) * N;
+#pragma unroll
+    for (int j = 0; j < N; ++j) li[j] = lp[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) li[j] = kNoDof;
+  }
+
+  const bool own3 = tid < S::EB * S::QQ;
+  const int e3 = tid / S::QQ, r3 = tid - e3 * S::QQ;  // r3 = k*Q + i
+  const int64_t ge3 = base + e3;
+  const bool act3 = own3 && e3 < ncb;
+  T gq[6][Q];
+  {
+    const TG* g = G + (act3 ? ge3 : 0) * 6 * S::QQQ + r3;
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        gq[c][l] = act3 ? load_g<TG, T>(g + c * S::QQQ + l * S::QQ) : T(0);
+  }
+
+  T yv[S::KU];
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    yv[k] = (dv[k] >= 0 && i >= nfirst) ? y[dv[k]] : T(0);
+  }
+
+  // ---- stage 0: the block's distinct x values into LDS --------------------
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    if (i < nu) regB[i] = xv[k];
+  }
+  __syncthreads();
+
+  // ---- stage 1: x-pencils:  Bx u, Dx u ----------------------------------
+  // The thread of x-pencil (e1, r1) also scatters it in stage 5.
   if (own1) {
     T u[N];
-    if (act1) {
-      const int32_t* cid = cells + (ge1 * S::NN + r1) * N;
 #pragma unroll
-      for (int j = 0; j < N; ++j) ids[j] = cid[j];
-      // branch-free gather: every lane issues all N loads back to back
-#pragma unroll
-      for (int j = 0; j < N; ++j) u[j] = x[ids[j] >= 0 ? ids[j] : 0];
-#pragma unroll
-      for (int j = 0; j < N; ++j) u[j] = ids[j] >= 0 ? u[j] : T(0);
-    } else {
-#pragma unroll
-      for (int j = 0; j < N; ++j) u[j] = T(0);
-    }
-    T* sA = smem + e1 * S::SE;
+    for (int j = 0; j < N; ++j) u[j] = li[j] != kNoDof ? regB[li[j]] : T(0);
+    T* sA = regA + e1 * 2 * S::SA1;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      T b = T(0), d = T(0);
+      T bq = T(0), dq = T(0);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        b = fma(tB[i * N + j], u[j], b);
-        d = fma(tD[i * N + j], u[j], d);
+        bq = fma(tB[i * N + j], u[j], bq);
+        dq = fma(tD[i * N + j], u[j], dq);
       }
-      sA[r1 * S::QP + i] = b;
-      sA[S::SA1 + r1 * S::QP + i] = d;
+      sA[r1 * S::QP + i] = bq;
+      sA[S::SA1 + r1 * S::QP + i] = dq;
     }
   }
   __syncthreads();
 
   // ---- stage 2: y-pencils: (Dx u, Bx u) -> By Dx u, Dy Bx u, By Bx u ------
   if (tid < S::EB * S::NQ) {
-    const int t = tid;
-    const int e = t / S::NQ, r = t - e * S::NQ;
+    const int e = tid / S::NQ, r = tid - e * S::NQ;
     const int z = r / Q, i = r - z * Q;
-    const T* sA = smem + e * S::SE;
-    T* sB = smem + e * S::SE + 2 * S::SA1;
+    const T* sA = regA + e * 2 * S::SA1;
+    T* sB = regB + e * 3 * S::SB1;
     T a[N], d[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
@@ -133,21 +204,16 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   __syncthreads();
 
   // ---- stage 3: z-columns: gradients, G, transposed z contraction --------
-  if (tid < S::EB * S::QQ) {
-    const int t = tid;
-    const int e = t / S::QQ, r = t - e * S::QQ;  // r = k*Q + i
-    const int64_t ge = base + e;
-    T* sB = smem + e * S::SE + 2 * S::SA1;
+  if (own3) {
+    T* sB = regB + e3 * 3 * S::SB1;
     T cx[N], cy[N], cz[N];
 #pragma unroll
     for (int z = 0; z < N; ++z) {
-      cx[z] = sB[z * S::QQ + r];
-      cy[z] = sB[S::SB1 + z * S::QQ + r];
-      cz[z] = sB[2 * S::SB1 + z * S::QQ + r];
+      cx[z] = sB[z * S::QQ + r3];
+      cy[z] = sB[S::SB1 + z * S::QQ + r3];
+      cz[z] = sB[2 * S::SB1 + z * S::QQ + r3];
     }
     T tx[Q], ty[Q], tz[Q];
-    const bool valid = ge < ncell;
-    const TG* g = G + ge * 6 * S::QQQ + r;
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
       T gx = T(0), gy = T(0), gz = T(0);
@@ -157,19 +223,9 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
         gy = fma(tB[l * N + z], cy[z], gy);
         gz = fma(tD[l * N + z], cz[z], gz);
       }
-      T g0 = T(0), g1 = T(0), g2 = T(0), g3 = T(0), g4 = T(0), g5 = T(0);
-      if (valid) {
-        const TG* gl = g + l * S::QQ;
-        g0 = load_g<TG, T>(gl + 0 * S::QQQ);
-        g1 = load_g<TG, T>(gl + 1 * S::QQQ);
-        g2 = load_g<TG, T>(gl + 2 * S::QQQ);
-        g3 = load_g<TG, T>(gl + 3 * S::QQQ);
-        g4 = load_g<TG, T>(gl + 4 * S::QQQ);
-        g5 = load_g<TG, T>(gl + 5 * S::QQQ);
-      }
-      tx[l] = g0 * gx + g1 * gy + g2 * gz;
-      ty[l] = g1 * gx + g3 * gy + g4 * gz;
-      tz[l] = g2 * gx + g4 * gy + g5 * gz;
+      tx[l] = gq[0][l] * gx + gq[1][l] * gy + gq[2][l] * gz;
+      ty[l] = gq[1][l] * gx + gq[3][l] * gy + gq[4][l] * gz;
+      tz[l] = gq[2][l] * gx + gq[4][l] * gy + gq[5][l] * gz;
     }
 #pragma unroll
     for (int z = 0; z < N; ++z) {
@@ -180,20 +236,19 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
         ry = fma(tB[l * N + z], ty[l], ry);
         rz = fma(tD[l * N + z], tz[l], rz);
       }
-      sB[z * S::QQ + r] = rx;
-      sB[S::SB1 + z * S::QQ + r] = ry;
-      sB[2 * S::SB1 + z * S::QQ + r] = rz;
+      sB[z * S::QQ + r3] = rx;
+      sB[S::SB1 + z * S::QQ + r3] = ry;
+      sB[2 * S::SB1 + z * S::QQ + r3] = rz;
     }
   }
   __syncthreads();
 
   // ---- stage 4: transposed y-pencils --------------------------------------
   if (tid < S::EB * S::NQ) {
-    const int t = tid;
-    const int e = t / S::NQ, r = t - e * S::NQ;
+    const int e = tid / S::NQ, r = tid - e * S::NQ;
     const int z = r / Q, i = r - z * Q;
-    T* sA = smem + e * S::SE;
-    const T* sB = smem + e * S::SE + 2 * S::SA1;
+    T* sA = regA + e * 2 * S::SA1;
+    const T* sB = regB + e * 3 * S::SB1;
     T rx[Q], ry[Q], rz[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
@@ -217,9 +272,13 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   }
   __syncthreads();
 
-  // ---- stage 5: transposed x-pencils + scatter-add -------------------------
+  // region B is free again: it becomes the block's y accumulator
+  for (int i = tid; i < nu; i += S::NT) regB[i] = T(0);
+  __syncthreads();
+
+  // ---- stage 5: transposed x-pencils, assembled in LDS ----------------------
   if (act1) {
-    const T* sA = smem + e1 * S::SE;
+    const T* sA = regA + e1 * 2 * S::SA1;
     T sx[Q], so[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -234,9 +293,16 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
         v = fma(tD[i * N + m], sx[i], v);
         v = fma(tB[i * N + m], so[i], v);
       }
-      if (ids[m] >= 0) atomic_add(y + ids[m], v);
+      if (li[m] != kNoDof) lds_atomic_add(regB + li[m], v);
     }
   }
+  __syncthreads();
+
+  // ---- write-back: first colour stores, later colours accumulate onto the
+  // value prefetched in the prologue (yv = 0 for first-touch dofs) ----------
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k)
+    if (dv[k] >= 0) y[dv[k]] = yv[k] + regB[tid + k * S::NT];
 }
 
 // ---------------------------------------------------------------------------
@@ -382,18 +448,36 @@ struct ApplyArgs {
   const T* tB;
   const T* tD;
   int64_t ncell;
+  int accumulate;  // 3D: 0 -> y = A x, 1 -> y += A x
+};
+
+// Host-built block structure of the 3D apply (see k_apply3d).
+struct BlockArgs {
+  const uint16_t* pidx;
+  const int32_t* bdofs;
+  const int64_t* boff;
+  const int32_t* bfirst;
+  const int32_t* order;      // blocks sorted by colour
+  const int64_t* bcell;      // first cell of each block (nblocks + 1)
+  const int* phase_off;      // host array, ncolor + 1 entries
+  int ncolor;
 };
 
 template <int N, int Q, typename T, typename TG>
-hipError_t launch_apply3d_nq(const ApplyArgs<T, TG>& a, hipStream_t s) {
+hipError_t launch_apply3d_nq(const ApplyArgs<T, TG>& a, const BlockArgs& k,
+                             hipStream_t s) {
   using S = Apply3Shape<N, Q, T>;
-  const int64_t nblk64 = (a.ncell + S::EB - 1) / S::EB;
-  if (nblk64 <= 0) return hipSuccess;
-  if (nblk64 > 0x7fffffff) return hipErrorInvalidValue;
-  const int nblk = (int)nblk64;
-  hipLaunchKernelGGL((k_apply3d<N, Q, T, TG>), dim3(nblk), dim3(S::NT), 0, s,
-                     a.x, a.y, a.cells, a.G, a.tB, a.tD, a.ncell, nblk);
-  return hipGetLastError();
+  for (int c = 0; c < k.ncolor; ++c) {
+    const int nb = k.phase_off[c + 1] - k.phase_off[c];
+    if (nb <= 0) continue;
+    hipLaunchKernelGGL((k_apply3d<N, Q, T, TG>), dim3(nb), dim3(S::NT), 0, s,
+                       a.x, a.y, k.pidx, k.bdofs, k.boff, k.bfirst,
+                       k.order + k.phase_off[c], nb, k.bcell, a.G, a.tB, a.tD,
+                       a.accumulate);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 template <int N, int Q, typename T, typename TG>
@@ -409,14 +493,17 @@ hipError_t launch_apply2d_nq(const ApplyArgs<T, TG>& a, hipStream_t s) {
 }
 
 // Returns hipErrorNotSupported for an (n, q) pair outside SF_FOR_EACH_NQ.
+// 3D: block-assembled, colour-phased (k needed); 2D: atomic scatter into a
+// y the caller zeroed.
 template <typename T, typename TG>
 hipError_t launch_apply(int dim, int n, int q, const ApplyArgs<T, TG>& a,
-                        hipStream_t s) {
+                        const BlockArgs* k, hipStream_t s) {
   if (dim == 3) {
+    if (!k) return hipErrorInvalidValue;
     switch (n * 16 + q) {
 #define SF_CASE(N_, Q_) \
   case N_ * 16 + Q_:    \
-    return launch_apply3d_nq<N_, Q_, T, TG>(a, s);
+    return launch_apply3d_nq<N_, Q_, T, TG>(a, *k, s);
       SF_FOR_EACH_NQ(SF_CASE)
 #undef SF_CASE
       default:

@@ -10,11 +10,11 @@
 namespace sf {
 
 hipError_t apply_f32_gbf16(int dim, int n, int q, const float* x, float* y,
-                           const int32_t* cells, const bf16s* G,
-                           const float* tB, const float* tD, int64_t ncell,
-                           hipStream_t s) {
-  ApplyArgs<float, bf16s> a{x, y, cells, G, tB, tD, ncell};
-  return launch_apply<float, bf16s>(dim, n, q, a, s);
+                           const int32_t* cells, const bf16s* G, const float* tB,
+                           const float* tD, int64_t ncell, const BlockArgs* k,
+                           int accumulate, hipStream_t s) {
+  ApplyArgs<float, bf16s> a{x, y, cells, G, tB, tD, ncell, accumulate};
+  return launch_apply<float, bf16s>(dim, n, q, a, k, s);
 }
 
 }  // namespace sf

@@ -7,9 +7,10 @@ namespace sf {
 
 hipError_t apply_f64(int dim, int n, int q, const double* x, double* y,
                      const int32_t* cells, const double* G, const double* tB,
-                     const double* tD, int64_t ncell, hipStream_t s) {
-  ApplyArgs<double, double> a{x, y, cells, G, tB, tD, ncell};
-  return launch_apply<double, double>(dim, n, q, a, s);
+                     const double* tD, int64_t ncell, const BlockArgs* k,
+                     int accumulate, hipStream_t s) {
+  ApplyArgs<double, double> a{x, y, cells, G, tB, tD, ncell, accumulate};
+  return launch_apply<double, double>(dim, n, q, a, k, s);
 }
 
 void apply_shape_f64(int dim, int n, int q, int* nt, int* eb) {

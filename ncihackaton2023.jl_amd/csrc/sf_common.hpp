@@ -80,6 +80,15 @@ __device__ __forceinline__ float load_g<bf16s, float>(const bf16s* p) {
   return bf16_bits_to_f32(h);
 }
 
+// LDS (workgroup-scope) atomic add; the result is unused, so the
+// no-return form ds_add_f64 / ds_add_f32 is emitted.
+__device__ __forceinline__ void lds_atomic_add(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_atomic_add(float* p, float v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ void atomic_add(double* p, double v) {
   unsafeAtomicAdd(p, v);
 }

@@ -8,6 +8,7 @@
 namespace sf {
 
 struct bf16s;
+struct BlockArgs;  // host-built block structure of the 3D apply (sf_apply.hpp)
 
 // ---- geometry (sf_geom.hip) ----
 hipError_t launch_geometry(int dim, const double* X, const double* pts,
@@ -21,17 +22,21 @@ hipError_t launch_check_positive(const double* v, int64_t n, int* bad,
 hipError_t launch_to_f32(const double* a, int64_t n, float* b, hipStream_t s);
 hipError_t launch_to_bf16(const double* a, int64_t n, bf16s* b, hipStream_t s);
 
-// ---- applies (sf_apply_*.hip); y += A x over all cells ----
+// ---- applies (sf_apply_*.hip) ----
+// 3D (k != nullptr): y = A x, or y += A x when accumulate; no fill needed.
+// 2D (k == nullptr): y += A x by atomics (the caller zeroes y for y = A x).
 hipError_t apply_f64(int dim, int n, int q, const double* x, double* y,
                      const int32_t* cells, const double* G, const double* tB,
-                     const double* tD, int64_t ncell, hipStream_t s);
+                     const double* tD, int64_t ncell, const BlockArgs* k,
+                     int accumulate, hipStream_t s);
 hipError_t apply_f32(int dim, int n, int q, const float* x, float* y,
                      const int32_t* cells, const float* G, const float* tB,
-                     const float* tD, int64_t ncell, hipStream_t s);
+                     const float* tD, int64_t ncell, const BlockArgs* k,
+                     int accumulate, hipStream_t s);
 hipError_t apply_f32_gbf16(int dim, int n, int q, const float* x, float* y,
                            const int32_t* cells, const bf16s* G,
                            const float* tB, const float* tD, int64_t ncell,
-                           hipStream_t s);
+                           const BlockArgs* k, int accumulate, hipStream_t s);
 void apply_shape_f64(int dim, int n, int q, int* nt, int* eb);
 
 // ---- adjoint (sf_adjoint.hip) ----
@@ -67,6 +72,10 @@ constexpr int kRedThreads = 256;
 template <typename V>
 hipError_t launch_fill(V* a, int64_t n, V v, hipStream_t s);
 
+// y[idx[i]] = 0 for i < n
+template <typename V>
+hipError_t launch_zero_list(V* y, const int32_t* idx, int64_t n, hipStream_t s);
+
 // out = a . b  (FP64 accumulation); result in *out (device)
 template <typename V>
 hipError_t launch_dot(const V* a, const V* b, int64_t n, Reducer red,
@@ -77,7 +86,7 @@ hipError_t launch_dot(const V* a, const V* b, int64_t n, Reducer red,
 //   cg_alpha : pq = p.q, alpha = rr / pq
 //   cg_update: x += alpha p, r -= alpha q, rr_new = r.r, beta = rr_new / rr,
 //              rr = rr_new, iters++, done = rr <= stop_rr
-//   cg_dir   : p = r + beta p, q = 0 (ready for the next q += A p)
+//   cg_dir   : p = r + beta p
 // Once done is set every later piece returns at once.
 template <typename V>
 hipError_t launch_cg_start(const V* b, const V* Ax, V* r, V* p, V* q,
@@ -90,7 +99,7 @@ template <typename V>
 hipError_t launch_cg_update(V* x, V* r, const V* p, const V* q, int64_t n,
                             Reducer red, CGScalars* sc, hipStream_t s);
 template <typename V>
-hipError_t launch_cg_dir(V* p, const V* r, V* q, int64_t n, CGScalars* sc,
+hipError_t launch_cg_dir(V* p, const V* r, int64_t n, CGScalars* sc,
                          hipStream_t s);
 
 // IR helpers (FP64 outer, FP32 inner)
