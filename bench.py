@@ -115,7 +115,7 @@ def mode_apply(args, torch, pkg):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
                      "kernel": "k_apply3d<5,6,double,double>",
                      "kernel_ms": kern * 1e3,
-                     "dispatches_per_apply": A.info()["n_colors"],
+                     "dispatches_per_apply": A.info()["launches_per_apply"],
                      "alg_bytes": Balg, "alg_flops": Falg,
                      "achieved_tflops": Falg / kern / 1e12},
     }

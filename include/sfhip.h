@@ -65,8 +65,8 @@ typedef struct sf_plan_info {
   int threads_per_block;   /* launch shape of the FP64 apply kernel           */
   int cells_per_block;
   int64_t device_bytes;    /* device memory the plan holds                   */
-  int n_blocks;            /* 3D: cell blocks (cells_per_block each)          */
-  int n_colors;            /* 3D: colour phases = kernel launches per apply  */
+  int n_blocks;            /* 3D: cell blocks (<= cells_per_block cells each) */
+  int launches_per_apply;  /* kernel launches of one apply (besides the fill) */
 } sf_plan_info;
 
 typedef struct sf_solve_stats {
@@ -140,6 +140,7 @@ int sf_plan_set_cells(sf_plan* plan, const int32_t* cell_dofs, int index_base,
                       const int32_t* local_to_tensor);
 
 /* Geometry from cell vertices: host array [cell][2^D vertices][D coords],
+ * (3D: after sf_plan_set_cells; setting new cells drops the geometry),
  * vertices tensor-lexicographic (x fastest), trilinear (bilinear) map.
  * Computes G on the device in FP64 once (J, det J, (J^T J)^{-1}).
  * Replaces the lazy cell_jq / cell_djq of src/LazyMatrices.jl:37-41.

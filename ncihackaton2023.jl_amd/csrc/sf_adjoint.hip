@@ -37,7 +37,9 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
                 const double* __restrict__ G,
                 const double* __restrict__ kappa,
                 const double* __restrict__ tB, const double* __restrict__ tD,
-                int64_t ncell, int nblk) {
+                int64_t ncell, int nblk,
+                const int32_t* __restrict__ cell_block,
+                const int64_t* __restrict__ bcell) {
   using S = Adj3Shape<N, Q>;
   __shared__ double smem[S::EB * S::SE];
   const int blk = xcd_remap(blockIdx.x, nblk);
@@ -144,7 +146,16 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
       clz[z] = sBl[2 * S::SB1 + z * S::QQ + r];
     }
     double tx[Q], ty[Q], tz[Q];
-    const double* g = G + ge * 6 * S::QQQ + r;
+    // G in the apply's block-interleaved layout (g3_index)
+    int64_t bs = 0;
+    int ncb = 1;
+    if (valid) {
+      const int cb = cell_block[ge];
+      bs = bcell[cb];
+      ncb = (int)(bcell[cb + 1] - bs);
+    }
+    const double* g = G + g3_index(bs, ncb, valid ? ge : bs, 0, 0, r, Q);
+    const int64_t kst = (int64_t)ncb * S::QQ;
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
       double gux = 0, guy = 0, guz = 0, glx = 0, gly = 0, glz = 0;
@@ -159,13 +170,12 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
       }
       double g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0, g5 = 0, kq = 1.0;
       if (valid) {
-        const double* gl = g + l * S::QQ;
-        g0 = load_g<double, double>(gl + 0 * S::QQQ);
-        g1 = load_g<double, double>(gl + 1 * S::QQQ);
-        g2 = load_g<double, double>(gl + 2 * S::QQQ);
-        g3 = load_g<double, double>(gl + 3 * S::QQQ);
-        g4 = load_g<double, double>(gl + 4 * S::QQQ);
-        g5 = load_g<double, double>(gl + 5 * S::QQQ);
+        g0 = load_g<double, double>(g + (0 * Q + l) * kst);
+        g1 = load_g<double, double>(g + (1 * Q + l) * kst);
+        g2 = load_g<double, double>(g + (2 * Q + l) * kst);
+        g3 = load_g<double, double>(g + (3 * Q + l) * kst);
+        g4 = load_g<double, double>(g + (4 * Q + l) * kst);
+        g5 = load_g<double, double>(g + (5 * Q + l) * kst);
         if (kappa) kq = kappa[ge * S::QQQ + l * S::QQ + r];
       }
       if (dJdk && valid) {
@@ -402,13 +412,16 @@ template <int N, int Q>
 hipError_t launch_adj3(const double* u, const double* lam, double* dJdk,
                        double* ybar, const int32_t* cells, const double* G,
                        const double* kappa, const double* tB,
-                       const double* tD, int64_t ncell, hipStream_t s) {
+                       const double* tD, int64_t ncell,
+                       const int32_t* cell_block, const int64_t* bcell,
+                       hipStream_t s) {
   using S = Adj3Shape<N, Q>;
   const int64_t nb = (ncell + S::EB - 1) / S::EB;
   if (nb <= 0) return hipSuccess;
   if (nb > 0x7fffffff) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_adjoint3d<N, Q>), dim3((int)nb), dim3(S::NT), 0, s, u,
-                     lam, dJdk, ybar, cells, G, kappa, tB, tD, ncell, (int)nb);
+                     lam, dJdk, ybar, cells, G, kappa, tB, tD, ncell, (int)nb,
+                     cell_block, bcell);
   return hipGetLastError();
 }
 
@@ -430,13 +443,15 @@ hipError_t adjoint_f64(int dim, int n, int q, const double* u,
                        const double* lam, double* dJdk, double* ybar,
                        const int32_t* cells, const double* G,
                        const double* kappa, const double* tB,
-                       const double* tD, int64_t ncell, hipStream_t s) {
+                       const double* tD, int64_t ncell,
+                       const int32_t* cell_block, const int64_t* bcell,
+                       hipStream_t s) {
   if (dim == 3) {
     switch (n * 16 + q) {
 #define SF_CASE(N_, Q_)                                                   \
   case N_ * 16 + Q_:                                                      \
     return launch_adj3<N_, Q_>(u, lam, dJdk, ybar, cells, G, kappa, tB, tD, \
-                               ncell, s);
+                               ncell, cell_block, bcell, s);
       SF_FOR_EACH_NQ(SF_CASE)
 #undef SF_CASE
       default:

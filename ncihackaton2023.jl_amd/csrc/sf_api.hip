@@ -206,11 +206,10 @@ struct sf_plan {
   bool has_cells = false, has_geom = false, has_kappa = false;
   bool g32_valid = false, gbf_valid = false;
   DevBuf<int> flag;
-  // 3D block structure (block-assembled, colour-phased apply)
-  int eb = 0, nblk = 0, ncolor = 0;
-  std::vector<int> phase_off;
+  // 3D block structure (block-assembled apply)
+  int eb = 0, nblk = 0;
   DevBuf<uint16_t> pidx;
-  DevBuf<int32_t> bdofs, bfirst, border, untouched;
+  DevBuf<int32_t> bdofs, bexcl, cell_block;
   DevBuf<int64_t> boff, bcell;
   BlockArgs kargs{};
   // staging of the host-pointer apply
@@ -232,8 +231,8 @@ struct sf_plan {
     dB32.release(); dD32.release();
     cells.release(); G.release(); Gk.release(); kappa.release();
     G32.release(); Gbf.release(); flag.release();
-    pidx.release(); bdofs.release(); bfirst.release(); border.release();
-    untouched.release(); boff.release(); bcell.release();
+    pidx.release(); bdofs.release(); bexcl.release(); cell_block.release();
+    boff.release(); bcell.release();
     xs.release(); ys.release();
     w_r.release(); w_p.release(); w_q.release(); w_t.release();
     w_r32.release(); w_s32.release(); w_d32.release(); w_p32.release(); w_q32.release();
@@ -246,8 +245,8 @@ struct sf_plan {
     return dB.bytes() + dD.bytes() + dW.bytes() + dPts.bytes() + dB32.bytes() +
            dD32.bytes() + cells.bytes() + G.bytes() + Gk.bytes() +
            kappa.bytes() + G32.bytes() + Gbf.bytes() + flag.bytes() +
-           pidx.bytes() + bdofs.bytes() + bfirst.bytes() + border.bytes() +
-           untouched.bytes() + boff.bytes() + bcell.bytes() +
+           pidx.bytes() + bdofs.bytes() + bexcl.bytes() + cell_block.bytes() +
+           boff.bytes() + bcell.bytes() +
            xs.bytes() + ys.bytes() + w_r.bytes() + w_p.bytes() + w_q.bytes() +
            w_t.bytes() + w_r32.bytes() + w_s32.bytes() + w_d32.bytes() + w_p32.bytes() +
            w_q32.bytes() + red_part.bytes() + red_counter.bytes() +
@@ -305,13 +304,13 @@ int clear_flag(sf_plan* P) {
   return SF_OK;
 }
 
-// y = A x (FP64) on stream s; accumulate: y += A x.
+// y = A x (FP64) on stream s.  accumulate: y += A x.  zeroed: the caller
+// guarantees y == 0 already (the CG direction update clears it), so the
+// fill is skipped.
 int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
-                 int accumulate = 0) {
+                 int accumulate = 0, bool zeroed = false) {
   if (P->nfree == 0) return SF_OK;  // nothing to compute (and x may be empty)
-  if (P->dim == 2 && !accumulate) SF_HIP(launch_fill<double>(y, P->nfree, 0.0, s));
-  if (P->dim == 3 && !accumulate && P->untouched.n > 0)
-    SF_HIP(launch_zero_list<double>(y, P->untouched.p, P->untouched.n, s));
+  if (!accumulate && !zeroed) SF_HIP(launch_fill<double>(y, P->nfree, 0.0, s));
   hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
                            P->dB.p, P->dD.p, P->ncell, P->blocks(), accumulate, s);
   if (e == hipErrorNotSupported)
@@ -344,13 +343,12 @@ int prepare(sf_plan* P, int precision) {
   return fail(SF_ERR_UNSUPPORTED, "unknown precision %d", precision);
 }
 
-// y = A x in reduced precision (FP32 vectors), y overwritten.
+// y = A x in reduced precision (FP32 vectors), y overwritten (zeroed: y
+// is already zero).
 int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
-                 hipStream_t s) {
+                 hipStream_t s, bool zeroed = false) {
   if (P->nfree == 0) return SF_OK;
-  if (P->dim == 2) SF_HIP(launch_fill<float>(y, P->nfree, 0.0f, s));
-  if (P->dim == 3 && P->untouched.n > 0)
-    SF_HIP(launch_zero_list<float>(y, P->untouched.p, P->untouched.n, s));
+  if (!zeroed) SF_HIP(launch_fill<float>(y, P->nfree, 0.0f, s));
   hipError_t e;
   if (precision == SF_PREC_F32) {
     e = apply_f32(P->dim, P->n, P->q, x, y, P->cells.p, P->G32.p, P->dB32.p,
@@ -372,14 +370,12 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
 //  * blocks of at most eb consecutive cells (eb = the kernel's cells per
 //    workgroup): the cell sequence is cut where two consecutive cells share
 //    no free dof (e.g. at the end of a mesh row) and every run is split into
-//    near-equal blocks, so blocks of neighbouring runs line up;
-//  * per block, its distinct free dofs, sorted;
-//  * greedy colouring of the blocks (two blocks sharing a dof get different
-//    colours) -- every colour is one launch, so writes need no atomics;
-//  * per block, the dofs it is the first (lowest colour) to reach go first:
-//    the kernel stores those and accumulates the rest, so y needs no fill;
+//    near-equal blocks;
+//  * per block, its distinct free dofs, sorted, the ones no other block
+//    touches first (the kernel stores those and adds the shared ones with
+//    atomics);
 //  * per cell dof, its index in the block's list (uint16, 0xffff =
-//    Dirichlet).
+//    Dirichlet); per cell, its block.
 int build_blocks(sf_plan* P, const std::vector<int32_t>& h) {
   int nt = 0, eb = 0;
   apply_shape_f64(3, P->n, P->q, &nt, &eb);
@@ -434,109 +430,56 @@ int build_blocks(sf_plan* P, const std::vector<int32_t>& h) {
     dofs.insert(dofs.end(), tmp.begin(), tmp.end());
     off[b + 1] = (int64_t)dofs.size();
   }
-  // dof -> blocks (CSR)
-  std::vector<int64_t> dptr(nf + 1, 0);
-  for (int32_t d : dofs) ++dptr[d + 1];
-  for (int64_t d = 0; d < nf; ++d) dptr[d + 1] += dptr[d];
-  std::vector<int32_t> dblk(dofs.size());
-  {
-    std::vector<int64_t> fillp(dptr.begin(), dptr.end() - 1);
-    for (int64_t b = 0; b < nb; ++b)
-      for (int64_t i = off[b]; i < off[b + 1]; ++i) dblk[fillp[dofs[i]]++] = (int32_t)b;
-  }
-  // greedy colouring in block order
-  std::vector<int32_t> color(nb, -1);
-  std::vector<int64_t> stamp;
-  int ncolor = 0;
-  for (int64_t b = 0; b < nb; ++b) {
-    for (int64_t i = off[b]; i < off[b + 1]; ++i) {
-      const int32_t d = dofs[i];
-      for (int64_t j = dptr[d]; j < dptr[d + 1]; ++j) {
-        const int32_t c = color[dblk[j]];
-        if (c >= 0) {
-          if ((size_t)c >= stamp.size()) stamp.resize(c + 1, -1);
-          stamp[c] = b;
-        }
-      }
-    }
-    int c = 0;
-    while ((size_t)c < stamp.size() && stamp[c] == b) ++c;
-    color[b] = c;
-    if (c + 1 > ncolor) ncolor = c + 1;
-  }
-  // first-touch split and local indices
-  std::vector<int32_t> first(nb, 0);
+  // how many blocks touch each dof
+  std::vector<int32_t> nblk_of(nf, 0);
+  for (int32_t d : dofs) ++nblk_of[d];
+  // exclusive dofs first, local indices
+  std::vector<int32_t> excl(nb, 0);
   std::vector<uint16_t> li((size_t)(E * nl), kNoDof);
+  std::vector<int32_t> cblk((size_t)E);
   std::vector<int32_t> part;
+  std::vector<std::pair<int32_t, int32_t>> pos;
   for (int64_t b = 0; b < nb; ++b) {
-    const int32_t cb = color[b];
     part.clear();
-    int nfirst = 0;
+    int nex = 0;
     for (int pass = 0; pass < 2; ++pass)
       for (int64_t i = off[b]; i < off[b + 1]; ++i) {
-        const int32_t d = dofs[i];
-        int32_t cmin = cb;
-        for (int64_t j = dptr[d]; j < dptr[d + 1]; ++j) cmin = std::min(cmin, color[dblk[j]]);
-        const bool is_first = (cmin == cb);
-        if (is_first == (pass == 0)) {
-          part.push_back(d);
-          if (pass == 0) ++nfirst;
+        const bool ex = nblk_of[dofs[i]] == 1;
+        if (ex == (pass == 0)) {
+          part.push_back(dofs[i]);
+          if (pass == 0) ++nex;
         }
       }
     std::copy(part.begin(), part.end(), dofs.begin() + off[b]);
-    first[b] = nfirst;
+    excl[b] = nex;
     if (off[b + 1] - off[b] > 0xfffe) return fail(SF_ERR_ARG, "block with too many dofs");
-    // position lookup: sorted (dof, position) pairs of this block
-    std::vector<std::pair<int32_t, int32_t>> pos;
-    pos.reserve(part.size());
+    pos.clear();
     for (size_t i = 0; i < part.size(); ++i) pos.push_back({part[i], (int32_t)i});
     std::sort(pos.begin(), pos.end());
     const int64_t c0 = bcell[b], c1 = bcell[b + 1];
+    for (int64_t c = c0; c < c1; ++c) cblk[c] = (int32_t)b;
     for (int64_t i = c0 * nl; i < c1 * nl; ++i) {
       if (h[i] < 0) continue;
       auto it = std::lower_bound(pos.begin(), pos.end(), std::make_pair(h[i], (int32_t)-1));
       li[i] = (uint16_t)it->second;
     }
   }
-  // blocks sorted by colour
-  std::vector<int> phase_off(ncolor + 1, 0);
-  for (int64_t b = 0; b < nb; ++b) ++phase_off[color[b] + 1];
-  for (int c = 0; c < ncolor; ++c) phase_off[c + 1] += phase_off[c];
-  std::vector<int32_t> order(nb);
-  {
-    std::vector<int> fp(phase_off.begin(), phase_off.end() - 1);
-    for (int64_t b = 0; b < nb; ++b) order[fp[color[b]]++] = (int32_t)b;
-  }
-  // free dofs no cell touches: their rows are empty, y = 0 there
-  std::vector<int32_t> untouched;
-  for (int64_t d = 0; d < nf; ++d)
-    if (dptr[d + 1] == dptr[d]) untouched.push_back((int32_t)d);
   // upload
-  SF_HIP(P->pidx.alloc((int64_t)li.size()));
+  SF_HIP(P->pidx.alloc(std::max<int64_t>((int64_t)li.size(), 1)));
   SF_HIP(P->bdofs.alloc((int64_t)std::max<size_t>(dofs.size(), 1)));
   SF_HIP(P->boff.alloc(nb + 1));
   SF_HIP(P->bcell.alloc(nb + 1));
-  SF_HIP(hipMemcpy(P->bcell.p, bcell.data(), (nb + 1) * 8, hipMemcpyHostToDevice));
-  SF_HIP(P->bfirst.alloc(std::max<int64_t>(nb, 1)));
-  SF_HIP(P->border.alloc(std::max<int64_t>(nb, 1)));
+  SF_HIP(P->bexcl.alloc(std::max<int64_t>(nb, 1)));
+  SF_HIP(P->cell_block.alloc(std::max<int64_t>(E, 1)));
   if (!li.empty()) SF_HIP(hipMemcpy(P->pidx.p, li.data(), li.size() * 2, hipMemcpyHostToDevice));
   if (!dofs.empty()) SF_HIP(hipMemcpy(P->bdofs.p, dofs.data(), dofs.size() * 4, hipMemcpyHostToDevice));
   SF_HIP(hipMemcpy(P->boff.p, off.data(), (nb + 1) * 8, hipMemcpyHostToDevice));
-  if (nb > 0) {
-    SF_HIP(hipMemcpy(P->bfirst.p, first.data(), nb * 4, hipMemcpyHostToDevice));
-    SF_HIP(hipMemcpy(P->border.p, order.data(), nb * 4, hipMemcpyHostToDevice));
-  }
-  P->untouched.release();
-  if (!untouched.empty()) {
-    SF_HIP(P->untouched.alloc((int64_t)untouched.size()));
-    SF_HIP(hipMemcpy(P->untouched.p, untouched.data(), untouched.size() * 4, hipMemcpyHostToDevice));
-  }
+  SF_HIP(hipMemcpy(P->bcell.p, bcell.data(), (nb + 1) * 8, hipMemcpyHostToDevice));
+  if (nb > 0) SF_HIP(hipMemcpy(P->bexcl.p, excl.data(), nb * 4, hipMemcpyHostToDevice));
+  if (E > 0) SF_HIP(hipMemcpy(P->cell_block.p, cblk.data(), E * 4, hipMemcpyHostToDevice));
   P->eb = eb;
   P->nblk = (int)nb;
-  P->ncolor = ncolor;
-  P->phase_off = phase_off;
-  P->kargs = BlockArgs{P->pidx.p,  P->bdofs.p,          P->boff.p, P->bfirst.p,
-                       P->border.p, P->bcell.p, P->phase_off.data(), ncolor};
+  P->kargs = BlockArgs{P->pidx.p, P->bdofs.p, P->boff.p, P->bexcl.p, P->bcell.p, (int)nb};
   return SF_OK;
 }
 
@@ -598,11 +541,11 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   while (!h.done && h.iters < maxit) {
     const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
     for (int k = 0; k < todo; ++k) {
-      rc = do_apply_f64(P, P->w_q.p, P->w_p.p, s);
+      rc = do_apply_f64(P, P->w_q.p, P->w_p.p, s, 0, true);
       if (rc) return rc;
       SF_HIP(launch_cg_alpha<double>(P->w_p.p, P->w_q.p, P->nfree, red, P->scal.p, s));
       SF_HIP(launch_cg_update<double>(x, P->w_r.p, P->w_p.p, P->w_q.p, P->nfree, red, P->scal.p, s));
-      SF_HIP(launch_cg_dir<double>(P->w_p.p, P->w_r.p, P->nfree, P->scal.p, s));
+      SF_HIP(launch_cg_dir<double>(P->w_p.p, P->w_r.p, P->w_q.p, P->nfree, P->scal.p, s));
     }
     rc = read_scalars(P, &h, s);
     if (rc) return rc;
@@ -633,11 +576,11 @@ int cg_f32_inner(sf_plan* P, int precision, double inner_rtol, int maxit,
   while (!h.done && h.iters < maxit) {
     const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
     for (int k = 0; k < todo; ++k) {
-      rc = do_apply_f32(P, q, p, precision, s);
+      rc = do_apply_f32(P, q, p, precision, s, true);
       if (rc) return rc;
       SF_HIP(launch_cg_alpha<float>(p, q, P->nfree, red, P->scal.p, s));
       SF_HIP(launch_cg_update<float>(d, r, p, q, P->nfree, red, P->scal.p, s));
-      SF_HIP(launch_cg_dir<float>(p, r, P->nfree, P->scal.p, s));
+      SF_HIP(launch_cg_dir<float>(p, r, q, P->nfree, P->scal.p, s));
     }
     rc = read_scalars(P, &h, s);
     if (rc) return rc;
@@ -778,7 +721,7 @@ int sf_plan_get_info(const sf_plan* P, sf_plan_info* info) {
   apply_shape_f64(P->dim, P->n, P->q, &info->threads_per_block, &info->cells_per_block);
   info->device_bytes = P->device_bytes();
   info->n_blocks = P->nblk;
-  info->n_colors = P->ncolor;
+  info->launches_per_apply = 1;
   return SF_OK;
   SF_CATCH
 }
@@ -843,7 +786,10 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
   if (!h.empty())
     SF_HIP(hipMemcpy(P->cells.p, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   if (P->dim == 3) {
+    // new blocks: the block-interleaved geometry must be set again
     P->has_cells = false;
+    P->has_geom = false;
+    P->g32_valid = P->gbf_valid = false;
     int rc = build_blocks(P, h);
     if (rc) return rc;
   }
@@ -856,9 +802,21 @@ static int finish_geometry(sf_plan* P) {
   P->g32_valid = P->gbf_valid = false;
   if (P->has_kappa) {
     SF_HIP(P->Gk.alloc(P->ncell * P->ng * P->nqc));
-    SF_HIP(launch_scale_kappa(P->G.p, P->kappa.p, P->ng, P->nqc, P->ncell, P->Gk.p, nullptr));
+    if (P->dim == 3)
+      SF_HIP(launch_scale_kappa3(P->G.p, P->kappa.p, P->q, P->ncell, P->cell_block.p,
+                                 P->bcell.p, P->Gk.p, nullptr));
+    else
+      SF_HIP(launch_scale_kappa(P->G.p, P->kappa.p, P->ng, P->nqc, P->ncell, P->Gk.p, nullptr));
     SF_HIP(hipDeviceSynchronize());
   }
+  return SF_OK;
+}
+
+// 3D geometry lives in the block-interleaved layout of the apply, which
+// depends on the cell blocks: the cells must be set first.
+static int need_cells_for_geometry(const sf_plan* P) {
+  if (P->dim == 3 && !P->has_cells)
+    return fail(SF_ERR_NOT_READY, "set the cell dofs (sf_plan_set_cells) before the geometry");
   return SF_OK;
 }
 
@@ -866,6 +824,7 @@ int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
   SF_TRY
   if (check_plan(P)) return SF_ERR_ARG;
   if (!vertices && P->ncell > 0) return fail(SF_ERR_ARG, "null vertices");
+  if (need_cells_for_geometry(P)) return SF_ERR_NOT_READY;
   DeviceGuard g(P->device);
   const int64_t nv = P->ncell * (P->dim == 3 ? 24 : 8);
   DevBuf<double> X;
@@ -880,7 +839,8 @@ int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
     if (e != hipSuccess) { rc = fail(SF_ERR_HIP, "G alloc: %s", hipGetErrorString(e)); break; }
     rc = clear_flag(P);
     if (rc) break;
-    e = launch_geometry(P->dim, X.p, P->dPts.p, P->dW.p, P->q, P->ncell, P->G.p, P->flag.p, nullptr);
+    e = launch_geometry(P->dim, X.p, P->dPts.p, P->dW.p, P->q, P->ncell, P->cell_block.p,
+                        P->bcell.p, P->G.p, P->flag.p, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) { rc = fail(SF_ERR_HIP, "geometry kernel: %s", hipGetErrorString(e)); break; }
     int bad = 0;
@@ -903,12 +863,26 @@ int sf_plan_set_geometry(sf_plan* P, const double* G) {
   SF_TRY
   if (check_plan(P)) return SF_ERR_ARG;
   if (!G && P->ncell > 0) return fail(SF_ERR_ARG, "null G");
+  if (need_cells_for_geometry(P)) return SF_ERR_NOT_READY;
   DeviceGuard g(P->device);
   const int64_t n = P->ncell * P->ng * P->nqc;
   for (int64_t i = 0; i < n; ++i)
     if (!isfinite(G[i])) return fail(SF_ERR_ARG, "non-finite G entry at %lld", (long long)i);
   SF_HIP(P->G.alloc(n));
-  if (n > 0) SF_HIP(hipMemcpy(P->G.p, G, n * 8, hipMemcpyHostToDevice));
+  if (n > 0) {
+    if (P->dim == 3) {
+      DevBuf<double> tmp;
+      SF_HIP(tmp.alloc(n));
+      hipError_t e = hipMemcpy(tmp.p, G, n * 8, hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+        e = launch_g3_relayout(tmp.p, P->G.p, P->q, P->ncell, P->cell_block.p, P->bcell.p, 1, nullptr);
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+      tmp.release();
+      SF_HIP(e);
+    } else {
+      SF_HIP(hipMemcpy(P->G.p, G, n * 8, hipMemcpyHostToDevice));
+    }
+  }
   P->has_geom = true;
   return finish_geometry(P);
   SF_CATCH
@@ -921,7 +895,19 @@ int sf_plan_get_geometry(const sf_plan* P, double* G) {
   if (!G) return fail(SF_ERR_ARG, "null G");
   DeviceGuard g(P->device);
   const int64_t n = P->ncell * P->ng * P->nqc;
-  if (n > 0) SF_HIP(hipMemcpy(G, P->G.p, n * 8, hipMemcpyDeviceToHost));
+  if (n > 0) {
+    if (P->dim == 3) {
+      DevBuf<double> tmp;
+      SF_HIP(tmp.alloc(n));
+      hipError_t e = launch_g3_relayout(P->G.p, tmp.p, P->q, P->ncell, P->cell_block.p,
+                                        P->bcell.p, 0, nullptr);
+      if (e == hipSuccess) e = hipMemcpy(G, tmp.p, n * 8, hipMemcpyDeviceToHost);
+      tmp.release();
+      SF_HIP(e);
+    } else {
+      SF_HIP(hipMemcpy(G, P->G.p, n * 8, hipMemcpyDeviceToHost));
+    }
+  }
   return SF_OK;
   SF_CATCH
 }
@@ -1023,7 +1009,7 @@ int sf_adjoint(sf_plan* P, const double* u, const double* lam,
   if (dJdu) SF_HIP(launch_fill<double>(dJdu, P->nfree, 0.0, s));
   hipError_t e = adjoint_f64(P->dim, P->n, P->q, u, lam, dJdkappa, dJdu,
                              P->cells.p, P->G.p, P->has_kappa ? P->kappa.p : nullptr,
-                             P->dB.p, P->dD.p, P->ncell, s);
+                             P->dB.p, P->dD.p, P->ncell, P->cell_block.p, P->bcell.p, s);
   if (e == hipErrorNotSupported)
     return fail(SF_ERR_UNSUPPORTED, "no adjoint kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
   SF_HIP(e);

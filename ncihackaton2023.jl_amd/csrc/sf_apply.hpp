@@ -60,46 +60,57 @@ struct Apply3Shape {
 
 constexpr uint16_t kNoDof = 0xffff;  // local index of a Dirichlet dof
 
-// Block-assembled apply of one colour (phase) of cell blocks.
+// Index of G entry c (0..5) at quadrature point qp = l*Q*Q + r of `cell` in
+// the block-interleaved 3D layout: block b (cells bcell[b] .. bcell[b+1]-1,
+// ncb of them) stores its G contiguously as [c*Q + l][cell in block][r],
+// so the stage-3 thread (cell e, column r) reads one value per (c, l) at a
+// fixed stride and every wave-wide load covers contiguous bytes.
+__host__ __device__ inline int64_t g3_index(int64_t bstart, int ncb, int64_t cell,
+                                            int c, int l, int r, int Q) {
+  const int64_t QQ = (int64_t)Q * Q;
+  return bstart * 6 * QQ * Q + ((int64_t)(c * Q + l) * ncb + (cell - bstart)) * QQ + r;
+}
+
+// Block-assembled 3D apply, one launch over all cell blocks.
 //
-// A workgroup owns block b = order[blockIdx]: cells bcell[b] ..
-// bcell[b+1] - 1, at most EB of them, consecutive in the caller's order.  The
-// host has listed the block's distinct free dofs (bdofs[boff[b] ..
-// boff[b+1]), sorted, the ones this block touches first -- lowest colour --
-// in front: bfirst[b] of them) and, for every cell dof, its index in that
-// list (pidx, kNoDof for Dirichlet dofs).  Blocks of one colour share no dof,
-// so the write-back needs no atomics: the first colour to reach a dof stores
-// it, later colours read-modify-write.  y needs no zero fill and every global
-// access of x and y runs over sorted, mostly contiguous ids.
+// Workgroup w owns block b = xcd_remap(w): cells bcell[b] .. bcell[b+1] - 1,
+// at most EB of them, consecutive in the caller's order.  The host listed
+// the block's distinct free dofs (bdofs[boff[b] .. boff[b+1]), sorted; the
+// bexcl[b] dofs no other block touches come first) and, for every cell dof,
+// its index in that list (pidx; kNoDof for Dirichlet dofs).  The block
+// gathers its x values once (sorted ids: coalesced), runs the five
+// contraction stages, assembles its cells' contributions in LDS and writes
+// each distinct dof once: exclusive dofs with plain stores, dofs shared
+// with other blocks with one hardware atomic add (consecutive lanes hit
+// consecutive ids).  y = A x needs y zeroed first (the caller's fill) unless
+// accumulate is set, in which case exclusive dofs are read-modify-written.
 template <int N, int Q, typename T, typename TG>
 __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     k_apply3d(const T* __restrict__ x, T* __restrict__ y,
               const uint16_t* __restrict__ pidx,
               const int32_t* __restrict__ bdofs,
               const int64_t* __restrict__ boff,
-              const int32_t* __restrict__ bfirst,
-              const int32_t* __restrict__ order, int nblk,
-              const int64_t* __restrict__ bcell,
+              const int32_t* __restrict__ bexcl,
+              const int64_t* __restrict__ bcell, int nblk,
               const TG* __restrict__ G, const T* __restrict__ tB,
               const T* __restrict__ tD, int accumulate) {
   using S = Apply3Shape<N, Q, T>;
   __shared__ T smem[S::RA + S::RB];
   T* const regA = smem;
   T* const regB = smem + S::RA;
-  const int b = order[xcd_remap(blockIdx.x, nblk)];
+  const int b = xcd_remap(blockIdx.x, nblk);
   const int64_t base = bcell[b];
   const int ncb = (int)(bcell[b + 1] - base);  // cells in this block, <= EB
   const int tid = threadIdx.x;
   const int64_t d0 = boff[b];
   const int nu = (int)(boff[b + 1] - d0);
-  const int nfirst = accumulate ? 0 : bfirst[b];
+  const int nexcl = bexcl[b];
   const int32_t* __restrict__ dofs = bdofs + d0;
 
   // ---- prologue: every global read of the block, issued at once ---------
-  // In the order the stages need them (x values, local indices, G, y), so
-  // waiting for the first ones leaves the later ones in flight: the G
-  // stream and the y read of the write-back overlap the gather and the
-  // first two contraction stages.
+  // In the order the stages need them (dof ids, x values, local indices,
+  // G), so waiting for the first ones leaves the G stream in flight while
+  // the gather and the first two contraction stages run.
   int32_t dv[S::KU];
   T xv[S::KU];
 #pragma unroll
@@ -113,10 +124,9 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   uint16_t li[N];
   const bool own1 = tid < S::EB * S::NN;
   const int e1 = tid / S::NN, r1 = tid - e1 * S::NN;  // r1 = z*N + y
-  const int64_t ge1 = base + e1;
   const bool act1 = own1 && e1 < ncb;
   if (act1) {
-    const uint16_t* lp = pidx + (ge1 * S::NN + r1) * N;
+    const uint16_t* lp = pidx + ((base + e1) * S::NN + r1) * N;
 #pragma unroll
     for (int j = 0; j < N; ++j) li[j] = lp[j];
   } else {
@@ -126,23 +136,17 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
 
   const bool own3 = tid < S::EB * S::QQ;
   const int e3 = tid / S::QQ, r3 = tid - e3 * S::QQ;  // r3 = k*Q + i
-  const int64_t ge3 = base + e3;
   const bool act3 = own3 && e3 < ncb;
   T gq[6][Q];
   {
-    const TG* g = G + (act3 ? ge3 : 0) * 6 * S::QQQ + r3;
+    // block-interleaved G: thread (e3, r3) is lane tid of every [c*Q + l] row
+    const TG* g = G + base * 6 * S::QQQ + (act3 ? tid : 0);
+    const int kst = ncb * S::QQ;
 #pragma unroll
     for (int l = 0; l < Q; ++l)
 #pragma unroll
       for (int c = 0; c < 6; ++c)
-        gq[c][l] = act3 ? load_g<TG, T>(g + c * S::QQQ + l * S::QQ) : T(0);
-  }
-
-  T yv[S::KU];
-#pragma unroll
-  for (int k = 0; k < S::KU; ++k) {
-    const int i = tid + k * S::NT;
-    yv[k] = (dv[k] >= 0 && i >= nfirst) ? y[dv[k]] : T(0);
+        gq[c][l] = act3 ? load_g<TG, T>(g + (c * Q + l) * kst) : T(0);
   }
 
   // ---- stage 0: the block's distinct x values into LDS --------------------
@@ -273,7 +277,11 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   __syncthreads();
 
   // region B is free again: it becomes the block's y accumulator
-  for (int i = tid; i < nu; i += S::NT) regB[i] = T(0);
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    if (i < nu) regB[i] = T(0);
+  }
   __syncthreads();
 
   // ---- stage 5: transposed x-pencils, assembled in LDS ----------------------
@@ -298,11 +306,21 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   }
   __syncthreads();
 
-  // ---- write-back: first colour stores, later colours accumulate onto the
-  // value prefetched in the prologue (yv = 0 for first-touch dofs) ----------
+  // ---- write-back: one global access per distinct dof of the block --------
 #pragma unroll
-  for (int k = 0; k < S::KU; ++k)
-    if (dv[k] >= 0) y[dv[k]] = yv[k] + regB[tid + k * S::NT];
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    if (dv[k] < 0) continue;
+    const T v = regB[i];
+    if (i < nexcl) {
+      if (accumulate)
+        y[dv[k]] += v;
+      else
+        y[dv[k]] = v;
+    } else {
+      atomic_add(y + dv[k], v);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -448,7 +466,7 @@ struct ApplyArgs {
   const T* tB;
   const T* tD;
   int64_t ncell;
-  int accumulate;  // 3D: 0 -> y = A x, 1 -> y += A x
+  int accumulate;  // 3D: 0 -> y zeroed by the caller, 1 -> y += A x
 };
 
 // Host-built block structure of the 3D apply (see k_apply3d).
@@ -456,28 +474,20 @@ struct BlockArgs {
   const uint16_t* pidx;
   const int32_t* bdofs;
   const int64_t* boff;
-  const int32_t* bfirst;
-  const int32_t* order;      // blocks sorted by colour
+  const int32_t* bexcl;
   const int64_t* bcell;      // first cell of each block (nblocks + 1)
-  const int* phase_off;      // host array, ncolor + 1 entries
-  int ncolor;
+  int nblk;
 };
 
 template <int N, int Q, typename T, typename TG>
 hipError_t launch_apply3d_nq(const ApplyArgs<T, TG>& a, const BlockArgs& k,
                              hipStream_t s) {
   using S = Apply3Shape<N, Q, T>;
-  for (int c = 0; c < k.ncolor; ++c) {
-    const int nb = k.phase_off[c + 1] - k.phase_off[c];
-    if (nb <= 0) continue;
-    hipLaunchKernelGGL((k_apply3d<N, Q, T, TG>), dim3(nb), dim3(S::NT), 0, s,
-                       a.x, a.y, k.pidx, k.bdofs, k.boff, k.bfirst,
-                       k.order + k.phase_off[c], nb, k.bcell, a.G, a.tB, a.tD,
-                       a.accumulate);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+  if (k.nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_apply3d<N, Q, T, TG>), dim3(k.nblk), dim3(S::NT), 0, s,
+                     a.x, a.y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell,
+                     k.nblk, a.G, a.tB, a.tD, a.accumulate);
+  return hipGetLastError();
 }
 
 template <int N, int Q, typename T, typename TG>
@@ -493,8 +503,9 @@ hipError_t launch_apply2d_nq(const ApplyArgs<T, TG>& a, hipStream_t s) {
 }
 
 // Returns hipErrorNotSupported for an (n, q) pair outside SF_FOR_EACH_NQ.
-// 3D: block-assembled, colour-phased (k needed); 2D: atomic scatter into a
-// y the caller zeroed.
+// 3D: block-assembled (k needed); 2D: atomic scatter.  Both add into y:
+// the caller zeroes y for y = A x (3D with accumulate = 0 stores the dofs
+// exclusive to a block instead of adding them).
 template <typename T, typename TG>
 hipError_t launch_apply(int dim, int n, int q, const ApplyArgs<T, TG>& a,
                         const BlockArgs* k, hipStream_t s) {

@@ -171,13 +171,15 @@ __global__ void __launch_bounds__(kRedThreads)
 
 template <typename V>
 __global__ void __launch_bounds__(256)
-    k_cg_dir(V* __restrict__ p, const V* __restrict__ r, int64_t n,
-             const CGScalars* __restrict__ sc) {
+    k_cg_dir(V* __restrict__ p, const V* __restrict__ r, V* __restrict__ q,
+             int64_t n, const CGScalars* __restrict__ sc) {
   if (sc->done) return;
   const V b = (V)sc->beta;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
+       i += (int64_t)gridDim.x * blockDim.x) {
     p[i] = fma(b, p[i], r[i]);
+    q[i] = V(0);
+  }
 }
 
 template <typename V>
@@ -282,11 +284,11 @@ hipError_t launch_cg_update(V* x, V* r, const V* p, const V* q, int64_t n,
 }
 
 template <typename V>
-hipError_t launch_cg_dir(V* p, const V* r, int64_t n, CGScalars* sc,
+hipError_t launch_cg_dir(V* p, const V* r, V* q, int64_t n, CGScalars* sc,
                          hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_cg_dir<V>, dim3(stream_grid(n)), dim3(256), 0, s, p, r,
-                     n, sc);
+                     q, n, sc);
   return hipGetLastError();
 }
 
@@ -328,8 +330,8 @@ hipError_t launch_axpy_f32_to_f64(double* x, const float* d32, double scale,
   template hipError_t launch_cg_update<V>(V*, V*, const V*, const V*,         \
                                           int64_t, Reducer, CGScalars*,       \
                                           hipStream_t);                       \
-  template hipError_t launch_cg_dir<V>(V*, const V*, int64_t, CGScalars*,    \
-                                       hipStream_t);
+  template hipError_t launch_cg_dir<V>(V*, const V*, V*, int64_t,             \
+                                       CGScalars*, hipStream_t);
 SF_INST(double)
 SF_INST(float)
 #undef SF_INST

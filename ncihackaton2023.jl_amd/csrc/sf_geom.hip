@@ -7,14 +7,18 @@
 // The reference forms the same product per call from wq, jq = J^{-T} and
 // djq = |det J| (src/LazyMatrices.jl:37-41, src/SumFactorizationKernels.jl:
 // 191-210); here it is formed once and stored as D(D+1)/2 entries.
+#include "sf_apply.hpp"
 #include "sf_common.hpp"
 #include "sf_kernels.h"
 
 namespace sf {
 
+// G is written in the block-interleaved layout of the 3D apply (g3_index).
 __global__ void k_geom3d(const double* __restrict__ X,
                          const double* __restrict__ pts,
                          const double* __restrict__ w1d, int q, int64_t ncell,
+                         const int32_t* __restrict__ cell_block,
+                         const int64_t* __restrict__ bcell,
                          double* __restrict__ G, int* __restrict__ bad) {
   const int64_t qqq = (int64_t)q * q * q;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -62,9 +66,12 @@ __global__ void k_geom3d(const double* __restrict__ X,
     const int a = pa[c], b = pb[c];
     M[c] = s * (C[0][a] * C[0][b] + C[1][a] * C[1][b] + C[2][a] * C[2][b]);
   }
-  double* Ge = G + e * 6 * qqq + r;
+  const int cb = cell_block[e];
+  const int64_t bs = bcell[cb];
+  const int ncb = (int)(bcell[cb + 1] - bs);
+  const int rr = r % (q * q);
 #pragma unroll
-  for (int c = 0; c < 6; ++c) Ge[c * qqq] = M[c];
+  for (int c = 0; c < 6; ++c) G[g3_index(bs, ncb, e, c, l, rr, q)] = M[c];
 }
 
 __global__ void k_geom2d(const double* __restrict__ X,
@@ -105,7 +112,52 @@ __global__ void k_geom2d(const double* __restrict__ X,
   Ge[2 * qq] = s * (c01 * c01 + c11 * c11);
 }
 
-// Gk[cell][c][qp] = kappa[cell][qp] * G[cell][c][qp]
+// 3D: between the caller's layout [cell][entry][qp] and the block-
+// interleaved one (to_int = 1: abi -> int, 0: int -> abi).
+__global__ void k_g3_relayout(const double* __restrict__ src,
+                              double* __restrict__ dst, int q, int64_t ncell,
+                              const int32_t* __restrict__ cell_block,
+                              const int64_t* __restrict__ bcell, int to_int) {
+  const int64_t qqq = (int64_t)q * q * q;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncell * 6 * qqq) return;
+  const int64_t e = t / (6 * qqq);
+  const int c = (int)((t / qqq) % 6);
+  const int qp = (int)(t % qqq);
+  const int cb = cell_block[e];
+  const int64_t bs = bcell[cb];
+  const int ncb = (int)(bcell[cb + 1] - bs);
+  const int64_t gi = g3_index(bs, ncb, e, c, qp / (q * q), qp % (q * q), q);
+  if (to_int)
+    dst[gi] = src[t];
+  else
+    dst[t] = src[gi];
+}
+
+// 3D: Gk = kappa * G in the block-interleaved layout, kappa [cell][qp].
+__global__ void k_scale_kappa3(const double* __restrict__ G,
+                               const double* __restrict__ kappa, int q,
+                               int64_t ncell,
+                               const int32_t* __restrict__ cell_block,
+                               const int64_t* __restrict__ bcell,
+                               double* __restrict__ Gk) {
+  const int64_t qqq = (int64_t)q * q * q;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncell * qqq) return;
+  const int64_t e = t / qqq;
+  const int qp = (int)(t % qqq);
+  const int cb = cell_block[e];
+  const int64_t bs = bcell[cb];
+  const int ncb = (int)(bcell[cb + 1] - bs);
+  const double k = kappa[t];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const int64_t gi = g3_index(bs, ncb, e, c, qp / (q * q), qp % (q * q), q);
+    Gk[gi] = k * G[gi];
+  }
+}
+
+// 2D: Gk[cell][c][qp] = kappa[cell][qp] * G[cell][c][qp]
 __global__ void k_scale_kappa(const double* __restrict__ G,
                               const double* __restrict__ kappa, int ng,
                               int64_t nqc, int64_t ncell,
@@ -143,20 +195,43 @@ static inline unsigned grid_for(int64_t n, int bs) {
 }
 
 hipError_t launch_geometry(int dim, const double* X, const double* pts,
-                           const double* w1d, int q, int64_t ncell, double* G,
-                           int* bad, hipStream_t s) {
+                           const double* w1d, int q, int64_t ncell,
+                           const int32_t* cell_block, const int64_t* bcell,
+                           double* G, int* bad, hipStream_t s) {
   const int bs = 256;
   if (dim == 3) {
     const int64_t n = ncell * q * q * q;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_geom3d, dim3(grid_for(n, bs)), dim3(bs), 0, s, X, pts,
-                       w1d, q, ncell, G, bad);
+                       w1d, q, ncell, cell_block, bcell, G, bad);
   } else {
     const int64_t n = ncell * q * q;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_geom2d, dim3(grid_for(n, bs)), dim3(bs), 0, s, X, pts,
                        w1d, q, ncell, G, bad);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_g3_relayout(const double* src, double* dst, int q,
+                              int64_t ncell, const int32_t* cell_block,
+                              const int64_t* bcell, int to_int,
+                              hipStream_t s) {
+  const int64_t n = ncell * 6 * q * q * q;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_g3_relayout, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     src, dst, q, ncell, cell_block, bcell, to_int);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_kappa3(const double* G, const double* kappa, int q,
+                               int64_t ncell, const int32_t* cell_block,
+                               const int64_t* bcell, double* Gk,
+                               hipStream_t s) {
+  const int64_t n = ncell * q * q * q;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scale_kappa3, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     G, kappa, q, ncell, cell_block, bcell, Gk);
   return hipGetLastError();
 }
 

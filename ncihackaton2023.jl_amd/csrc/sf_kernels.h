@@ -11,9 +11,18 @@ struct bf16s;
 struct BlockArgs;  // host-built block structure of the 3D apply (sf_apply.hpp)
 
 // ---- geometry (sf_geom.hip) ----
+// 3D writes G in the block-interleaved layout (cell_block, bcell); 2D in
+// the caller's layout [cell][entry][qp].
 hipError_t launch_geometry(int dim, const double* X, const double* pts,
-                           const double* w1d, int q, int64_t ncell, double* G,
-                           int* bad, hipStream_t s);
+                           const double* w1d, int q, int64_t ncell,
+                           const int32_t* cell_block, const int64_t* bcell,
+                           double* G, int* bad, hipStream_t s);
+hipError_t launch_g3_relayout(const double* src, double* dst, int q,
+                              int64_t ncell, const int32_t* cell_block,
+                              const int64_t* bcell, int to_int, hipStream_t s);
+hipError_t launch_scale_kappa3(const double* G, const double* kappa, int q,
+                               int64_t ncell, const int32_t* cell_block,
+                               const int64_t* bcell, double* Gk, hipStream_t s);
 hipError_t launch_scale_kappa(const double* G, const double* kappa, int ng,
                               int64_t nqc, int64_t ncell, double* Gk,
                               hipStream_t s);
@@ -41,11 +50,14 @@ void apply_shape_f64(int dim, int n, int q, int* nt, int* eb);
 
 // ---- adjoint (sf_adjoint.hip) ----
 // dJdk[cell][qp] = grad(lam) . G grad(u);  ybar += A(kappa) lam
+// 3D reads G through the block-interleaved layout (cell_block, bcell).
 hipError_t adjoint_f64(int dim, int n, int q, const double* u,
                        const double* lam, double* dJdk, double* ybar,
                        const int32_t* cells, const double* G,
                        const double* kappa, const double* tB,
-                       const double* tD, int64_t ncell, hipStream_t s);
+                       const double* tD, int64_t ncell,
+                       const int32_t* cell_block, const int64_t* bcell,
+                       hipStream_t s);
 
 // ---- BLAS-1 for the solvers (sf_blas.hip) ----
 struct CGScalars {
@@ -86,7 +98,7 @@ hipError_t launch_dot(const V* a, const V* b, int64_t n, Reducer red,
 //   cg_alpha : pq = p.q, alpha = rr / pq
 //   cg_update: x += alpha p, r -= alpha q, rr_new = r.r, beta = rr_new / rr,
 //              rr = rr_new, iters++, done = rr <= stop_rr
-//   cg_dir   : p = r + beta p
+//   cg_dir   : p = r + beta p, q = 0 (ready for the next q += A p)
 // Once done is set every later piece returns at once.
 template <typename V>
 hipError_t launch_cg_start(const V* b, const V* Ax, V* r, V* p, V* q,
@@ -99,7 +111,7 @@ template <typename V>
 hipError_t launch_cg_update(V* x, V* r, const V* p, const V* q, int64_t n,
                             Reducer red, CGScalars* sc, hipStream_t s);
 template <typename V>
-hipError_t launch_cg_dir(V* p, const V* r, int64_t n, CGScalars* sc,
+hipError_t launch_cg_dir(V* p, const V* r, V* q, int64_t n, CGScalars* sc,
                          hipStream_t s);
 
 // IR helpers (FP64 outer, FP32 inner)

@@ -194,8 +194,11 @@ def test_error_paths(pkg, cuda):
     # a local_to_tensor that is not a permutation
     perm = np.zeros(27, dtype=np.int32)
     assert L.sf_plan_set_cells(h, space.cell_dofs.ctypes.data, 0, perm.ctypes.data) == pkg._lib.SF_ERR_ARG
-    # inverted cell: det J < 0
+    # 3D geometry needs the cells first (its layout follows the cell blocks)
     X = model.cell_vertices().copy()
+    assert L.sf_plan_set_vertices(h, X.ctypes.data) == pkg._lib.SF_ERR_NOT_READY
+    assert L.sf_plan_set_cells(h, space.cell_dofs.ctypes.data, 0, None) == 0
+    # inverted cell: det J < 0
     X[0] = X[0][[1, 0, 3, 2, 5, 4, 7, 6]]  # mirror in x
     assert L.sf_plan_set_vertices(h, X.ctypes.data) == pkg._lib.SF_ERR_JACOBIAN
     # non-positive kappa
