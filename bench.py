@@ -190,7 +190,7 @@ def mode_cg(args, torch, pkg):
     st = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=args.maxit)
     res["f64"] = st
     log("f64", st)
-    for inner, irt in [("f32", 1e-3), ("f32", 1e-2), ("f32_gbf16", 1e-2)]:
+    for inner, irt in [("f32", 1e-4), ("f32", 1e-2), ("f32_gbf16", 1e-2)]:
         x = torch.zeros_like(b)
         st = pkg.cg_ir_(x, A, b, rtol=1e-10, inner=inner, inner_rtol=irt,
                         inner_maxiter=args.maxit)
@@ -249,7 +249,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", default="apply", choices=["apply", "sweep", "cg", "adjoint"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--maxit", type=int, default=20000)
+    ap.add_argument("--maxit", type=int, default=300000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); the apply "

@@ -79,13 +79,15 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
     double* sA = smem + e1 * S::SE;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       double bu = 0, du = 0, bl = 0, dl = 0;
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        bu = fma(tB[i * N + j], uu[j], bu);
-        du = fma(tD[i * N + j], uu[j], du);
-        bl = fma(tB[i * N + j], ll[j], bl);
-        dl = fma(tD[i * N + j], ll[j], dl);
+        bu = fma(b_[i * N + j], uu[j], bu);
+        du = fma(d_[i * N + j], uu[j], du);
+        bl = fma(b_[i * N + j], ll[j], bl);
+        dl = fma(d_[i * N + j], ll[j], dl);
       }
       sA[r1 * S::QP + i] = bu;
       sA[S::SA1 + r1 * S::QP + i] = du;
@@ -112,12 +114,14 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
     }
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       double bb = 0, bd = 0, db = 0;
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        bb = fma(tB[k * N + j], a[j], bb);
-        bd = fma(tD[k * N + j], a[j], bd);
-        db = fma(tB[k * N + j], d[j], db);
+        bb = fma(b_[k * N + j], a[j], bb);
+        bd = fma(d_[k * N + j], a[j], bd);
+        db = fma(b_[k * N + j], d[j], db);
       }
       const int o = z * S::QQ + k * Q + i;
       sB[o] = db;
@@ -158,15 +162,17 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
     const int64_t kst = (int64_t)ncb * S::QQ;
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       double gux = 0, guy = 0, guz = 0, glx = 0, gly = 0, glz = 0;
 #pragma unroll
       for (int z = 0; z < N; ++z) {
-        gux = fma(tB[l * N + z], cux[z], gux);
-        guy = fma(tB[l * N + z], cuy[z], guy);
-        guz = fma(tD[l * N + z], cuz[z], guz);
-        glx = fma(tB[l * N + z], clx[z], glx);
-        gly = fma(tB[l * N + z], cly[z], gly);
-        glz = fma(tD[l * N + z], clz[z], glz);
+        gux = fma(b_[l * N + z], cux[z], gux);
+        guy = fma(b_[l * N + z], cuy[z], guy);
+        guz = fma(d_[l * N + z], cuz[z], guz);
+        glx = fma(b_[l * N + z], clx[z], glx);
+        gly = fma(b_[l * N + z], cly[z], gly);
+        glz = fma(d_[l * N + z], clz[z], glz);
       }
       double g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0, g5 = 0, kq = 1.0;
       if (valid) {
@@ -191,12 +197,14 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
     if (want_y) {
 #pragma unroll
       for (int z = 0; z < N; ++z) {
+        const auto b_ = table_row(tB);
+        const auto d_ = table_row(tD);
         double rx = 0, ry = 0, rz = 0;
 #pragma unroll
         for (int l = 0; l < Q; ++l) {
-          rx = fma(tB[l * N + z], tx[l], rx);
-          ry = fma(tB[l * N + z], ty[l], ry);
-          rz = fma(tD[l * N + z], tz[l], rz);
+          rx = fma(b_[l * N + z], tx[l], rx);
+          ry = fma(b_[l * N + z], ty[l], ry);
+          rz = fma(d_[l * N + z], tz[l], rz);
         }
         sBl[z * S::QQ + r] = rx;
         sBl[S::SB1 + z * S::QQ + r] = ry;
@@ -223,12 +231,14 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       double sx = 0, so = 0;
 #pragma unroll
       for (int k = 0; k < Q; ++k) {
-        sx = fma(tB[k * N + j], rx[k], sx);
-        so = fma(tD[k * N + j], ry[k], so);
-        so = fma(tB[k * N + j], rz[k], so);
+        sx = fma(b_[k * N + j], rx[k], sx);
+        so = fma(d_[k * N + j], ry[k], so);
+        so = fma(b_[k * N + j], rz[k], so);
       }
       sA[(z * N + j) * S::QP + i] = sx;
       sA[S::SA1 + (z * N + j) * S::QP + i] = so;
@@ -247,11 +257,13 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
     }
 #pragma unroll
     for (int m = 0; m < N; ++m) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       double v = 0;
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
-        v = fma(tD[i * N + m], sx[i], v);
-        v = fma(tB[i * N + m], so[i], v);
+        v = fma(d_[i * N + m], sx[i], v);
+        v = fma(b_[i * N + m], so[i], v);
       }
       if (ids[m] >= 0) atomic_add(ybar + ids[m], v);
     }
@@ -315,13 +327,15 @@ __global__ void __launch_bounds__((Adj2Shape<N, Q>::NT))
     double* sA = smem + e1 * S::SE;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       double bu = 0, du = 0, bl = 0, dl = 0;
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        bu = fma(tB[i * N + j], uu[j], bu);
-        du = fma(tD[i * N + j], uu[j], du);
-        bl = fma(tB[i * N + j], ll[j], bl);
-        dl = fma(tD[i * N + j], ll[j], dl);
+        bu = fma(b_[i * N + j], uu[j], bu);
+        du = fma(d_[i * N + j], uu[j], du);
+        bl = fma(b_[i * N + j], ll[j], bl);
+        dl = fma(d_[i * N + j], ll[j], dl);
       }
       sA[r1 * S::QP + i] = bu;
       sA[S::SA1 + r1 * S::QP + i] = du;
@@ -349,13 +363,15 @@ __global__ void __launch_bounds__((Adj2Shape<N, Q>::NT))
     double tx[Q], ty[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       double gux = 0, guy = 0, glx = 0, gly = 0;
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        gux = fma(tB[k * N + j], du[j], gux);
-        guy = fma(tD[k * N + j], au[j], guy);
-        glx = fma(tB[k * N + j], dl[j], glx);
-        gly = fma(tD[k * N + j], al[j], gly);
+        gux = fma(b_[k * N + j], du[j], gux);
+        guy = fma(d_[k * N + j], au[j], guy);
+        glx = fma(b_[k * N + j], dl[j], glx);
+        gly = fma(d_[k * N + j], al[j], gly);
       }
       double g0 = 0, g1 = 0, g2 = 0, kq = 1.0;
       if (valid) {
@@ -373,11 +389,13 @@ __global__ void __launch_bounds__((Adj2Shape<N, Q>::NT))
     if (want_y) {
 #pragma unroll
       for (int j = 0; j < N; ++j) {
+        const auto b_ = table_row(tB);
+        const auto d_ = table_row(tD);
         double rx = 0, ry = 0;
 #pragma unroll
         for (int k = 0; k < Q; ++k) {
-          rx = fma(tB[k * N + j], tx[k], rx);
-          ry = fma(tD[k * N + j], ty[k], ry);
+          rx = fma(b_[k * N + j], tx[k], rx);
+          ry = fma(d_[k * N + j], ty[k], ry);
         }
         sA[j * S::QP + i] = rx;
         sA[S::SA1 + j * S::QP + i] = ry;
@@ -397,11 +415,13 @@ __global__ void __launch_bounds__((Adj2Shape<N, Q>::NT))
     }
 #pragma unroll
     for (int m = 0; m < N; ++m) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       double v = 0;
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
-        v = fma(tD[i * N + m], sx[i], v);
-        v = fma(tB[i * N + m], so[i], v);
+        v = fma(d_[i * N + m], sx[i], v);
+        v = fma(b_[i * N + m], so[i], v);
       }
       if (ids[m] >= 0) atomic_add(ybar + ids[m], v);
     }

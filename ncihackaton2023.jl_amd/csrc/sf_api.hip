@@ -325,7 +325,11 @@ int prepare(sf_plan* P, int precision) {
   if (precision == SF_PREC_F32 || precision == SF_PREC_MFMA_BF16) {
     if (!P->g32_valid) {
       SF_HIP(P->G32.alloc(ng));
-      SF_HIP(launch_to_f32(P->active_G(), ng, P->G32.p, nullptr));
+      if (P->dim == 3)
+        SF_HIP(launch_g3_to_f32_pairs(P->active_G(), P->G32.p, P->q, P->ncell,
+                                      P->cell_block.p, P->bcell.p, nullptr));
+      else
+        SF_HIP(launch_to_f32(P->active_G(), ng, P->G32.p, nullptr));
       SF_HIP(hipDeviceSynchronize());
       P->g32_valid = true;
     }
@@ -334,7 +338,11 @@ int prepare(sf_plan* P, int precision) {
   if (precision == SF_PREC_F32_GBF16) {
     if (!P->gbf_valid) {
       SF_HIP(P->Gbf.alloc(ng));
-      SF_HIP(launch_to_bf16(P->active_G(), ng, P->Gbf.p, nullptr));
+      if (P->dim == 3)
+        SF_HIP(launch_g3_to_bf16_pairs(P->active_G(), P->Gbf.p, P->q, P->ncell,
+                                       P->cell_block.p, P->bcell.p, nullptr));
+      else
+        SF_HIP(launch_to_bf16(P->active_G(), ng, P->Gbf.p, nullptr));
       SF_HIP(hipDeviceSynchronize());
       P->gbf_valid = true;
     }

@@ -60,6 +60,18 @@ struct Apply3Shape {
 
 constexpr uint16_t kNoDof = 0xffff;  // local index of a Dirichlet dof
 
+// Reduced-precision (FP32 / bf16) 3D G pairs rows 2m, 2m+1 of the block-
+// interleaved layout: entry (k = c*Q + l) of lane t of block b sits at
+// bstart*6Q^3 + (k/2)*2*ncb*Q^2 + 2*t + (k%2).
+__host__ __device__ inline int64_t g3_pair_index(int64_t bstart, int ncb,
+                                                 int64_t cell, int c, int l,
+                                                 int r, int Q) {
+  const int64_t QQ = (int64_t)Q * Q;
+  const int k = c * Q + l;
+  return bstart * 6 * QQ * Q + (int64_t)(k / 2) * 2 * ncb * QQ +
+         2 * ((cell - bstart) * QQ + r) + (k % 2);
+}
+
 // Index of G entry c (0..5) at quadrature point qp = l*Q*Q + r of `cell` in
 // the block-interleaved 3D layout: block b (cells bcell[b] .. bcell[b+1]-1,
 // ncb of them) stores its G contiguously as [c*Q + l][cell in block][r],
@@ -138,7 +150,7 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   const int e3 = tid / S::QQ, r3 = tid - e3 * S::QQ;  // r3 = k*Q + i
   const bool act3 = own3 && e3 < ncb;
   T gq[6][Q];
-  {
+  if constexpr (sizeof(TG) == 8) {
     // block-interleaved G: thread (e3, r3) is lane tid of every [c*Q + l] row
     const TG* g = G + base * 6 * S::QQQ + (act3 ? tid : 0);
     const int kst = ncb * S::QQ;
@@ -147,6 +159,19 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
 #pragma unroll
       for (int c = 0; c < 6; ++c)
         gq[c][l] = act3 ? load_g<TG, T>(g + (c * Q + l) * kst) : T(0);
+  } else {
+    // reduced-precision G, pair-interleaved: rows 2m and 2m+1 sit side by
+    // side for every lane, so one 4-byte (bf16) or 8-byte (FP32) load per
+    // lane brings two entries
+    const TG* g = G + base * 6 * S::QQQ + (act3 ? 2 * tid : 0);
+    const int kst = 2 * ncb * S::QQ;
+#pragma unroll
+    for (int m = 0; m < 3 * Q; ++m) {
+      T lo = T(0), hi = T(0);
+      if (act3) load_g_pair<TG, T>(g + m * kst, &lo, &hi);
+      gq[(2 * m) / Q][(2 * m) % Q] = lo;
+      gq[(2 * m + 1) / Q][(2 * m + 1) % Q] = hi;
+    }
   }
 
   // ---- stage 0: the block's distinct x values into LDS --------------------
@@ -166,11 +191,13 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     T* sA = regA + e1 * 2 * S::SA1;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       T bq = T(0), dq = T(0);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        bq = fma(tB[i * N + j], u[j], bq);
-        dq = fma(tD[i * N + j], u[j], dq);
+        bq = fma(b_[i * N + j], u[j], bq);
+        dq = fma(d_[i * N + j], u[j], dq);
       }
       sA[r1 * S::QP + i] = bq;
       sA[S::SA1 + r1 * S::QP + i] = dq;
@@ -192,12 +219,14 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     }
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       T bb = T(0), bd = T(0), db = T(0);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        bb = fma(tB[k * N + j], a[j], bb);
-        bd = fma(tD[k * N + j], a[j], bd);
-        db = fma(tB[k * N + j], d[j], db);
+        bb = fma(b_[k * N + j], a[j], bb);
+        bd = fma(d_[k * N + j], a[j], bd);
+        db = fma(b_[k * N + j], d[j], db);
       }
       const int o = z * S::QQ + k * Q + i;
       sB[o] = db;               // x-gradient path
@@ -220,12 +249,14 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     T tx[Q], ty[Q], tz[Q];
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       T gx = T(0), gy = T(0), gz = T(0);
 #pragma unroll
       for (int z = 0; z < N; ++z) {
-        gx = fma(tB[l * N + z], cx[z], gx);
-        gy = fma(tB[l * N + z], cy[z], gy);
-        gz = fma(tD[l * N + z], cz[z], gz);
+        gx = fma(b_[l * N + z], cx[z], gx);
+        gy = fma(b_[l * N + z], cy[z], gy);
+        gz = fma(d_[l * N + z], cz[z], gz);
       }
       tx[l] = gq[0][l] * gx + gq[1][l] * gy + gq[2][l] * gz;
       ty[l] = gq[1][l] * gx + gq[3][l] * gy + gq[4][l] * gz;
@@ -233,12 +264,14 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     }
 #pragma unroll
     for (int z = 0; z < N; ++z) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       T rx = T(0), ry = T(0), rz = T(0);
 #pragma unroll
       for (int l = 0; l < Q; ++l) {
-        rx = fma(tB[l * N + z], tx[l], rx);
-        ry = fma(tB[l * N + z], ty[l], ry);
-        rz = fma(tD[l * N + z], tz[l], rz);
+        rx = fma(b_[l * N + z], tx[l], rx);
+        ry = fma(b_[l * N + z], ty[l], ry);
+        rz = fma(d_[l * N + z], tz[l], rz);
       }
       sB[z * S::QQ + r3] = rx;
       sB[S::SB1 + z * S::QQ + r3] = ry;
@@ -263,12 +296,14 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       T sx = T(0), so = T(0);
 #pragma unroll
       for (int k = 0; k < Q; ++k) {
-        sx = fma(tB[k * N + j], rx[k], sx);
-        so = fma(tD[k * N + j], ry[k], so);
-        so = fma(tB[k * N + j], rz[k], so);
+        sx = fma(b_[k * N + j], rx[k], sx);
+        so = fma(d_[k * N + j], ry[k], so);
+        so = fma(b_[k * N + j], rz[k], so);
       }
       sA[(z * N + j) * S::QP + i] = sx;           // then D_x^T
       sA[S::SA1 + (z * N + j) * S::QP + i] = so;  // then B_x^T
@@ -295,11 +330,13 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     }
 #pragma unroll
     for (int m = 0; m < N; ++m) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       T v = T(0);
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
-        v = fma(tD[i * N + m], sx[i], v);
-        v = fma(tB[i * N + m], so[i], v);
+        v = fma(d_[i * N + m], sx[i], v);
+        v = fma(b_[i * N + m], so[i], v);
       }
       if (li[m] != kNoDof) lds_atomic_add(regB + li[m], v);
     }
@@ -374,11 +411,13 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
     T* sA = smem + e1 * S::SE;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       T b = T(0), d = T(0);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        b = fma(tB[i * N + j], u[j], b);
-        d = fma(tD[i * N + j], u[j], d);
+        b = fma(b_[i * N + j], u[j], b);
+        d = fma(d_[i * N + j], u[j], d);
       }
       sA[r1 * S::QP + i] = b;
       sA[S::SA1 + r1 * S::QP + i] = d;
@@ -403,11 +442,13 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
     T tx[Q], ty[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       T gx = T(0), gy = T(0);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        gx = fma(tB[k * N + j], d[j], gx);
-        gy = fma(tD[k * N + j], a[j], gy);
+        gx = fma(b_[k * N + j], d[j], gx);
+        gy = fma(d_[k * N + j], a[j], gy);
       }
       T g0 = T(0), g1 = T(0), g2 = T(0);
       if (valid) {
@@ -420,11 +461,13 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       T rx = T(0), ry = T(0);
 #pragma unroll
       for (int k = 0; k < Q; ++k) {
-        rx = fma(tB[k * N + j], tx[k], rx);
-        ry = fma(tD[k * N + j], ty[k], ry);
+        rx = fma(b_[k * N + j], tx[k], rx);
+        ry = fma(d_[k * N + j], ty[k], ry);
       }
       sA[j * S::QP + i] = rx;            // then D_x^T
       sA[S::SA1 + j * S::QP + i] = ry;   // then B_x^T
@@ -443,11 +486,13 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
     }
 #pragma unroll
     for (int m = 0; m < N; ++m) {
+      const auto b_ = table_row(tB);
+      const auto d_ = table_row(tD);
       T v = T(0);
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
-        v = fma(tD[i * N + m], sx[i], v);
-        v = fma(tB[i * N + m], so[i], v);
+        v = fma(d_[i * N + m], sx[i], v);
+        v = fma(b_[i * N + m], so[i], v);
       }
       if (ids[m] >= 0) atomic_add(y + ids[m], v);
     }

@@ -43,6 +43,22 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   return xcd * per + (xcd < rem ? xcd : rem) + j;
 }
 
+// The 1D tables are read with uniform scalar loads.  A fully unrolled
+// contraction uses Q*N entries of each table; left alone the compiler hoists
+// all of them into SGPRs and spills the excess into VGPR lanes.  table_row()
+// hands each unrolled output row an opaque copy of the table pointer (kept
+// in the constant address space, so loads stay scalar): the loads of a row
+// cannot move above it, and only about one row of each table is live.
+template <typename P>
+using cptr = const __attribute__((address_space(4))) P*;
+
+template <typename P>
+__device__ __forceinline__ cptr<P> table_row(const P* p) {
+  cptr<P> c = (cptr<P>)p;
+  asm volatile("" : "+s"(c));
+  return c;
+}
+
 // bf16 storage for the geometric factor (round-to-nearest-even on store).
 struct bf16s {
   uint16_t bits;
@@ -82,6 +98,26 @@ __device__ __forceinline__ float load_g<bf16s, float>(const bf16s* p) {
 
 // LDS (workgroup-scope) atomic add; the result is unused, so the
 // no-return form ds_add_f64 / ds_add_f32 is emitted.
+// Two adjacent reduced-precision G entries in one load.
+template <typename TG, typename T>
+__device__ __forceinline__ void load_g_pair(const TG* p, T* lo, T* hi);
+
+template <>
+__device__ __forceinline__ void load_g_pair<float, float>(const float* p,
+                                                         float* lo, float* hi) {
+  typedef float fv2 __attribute__((ext_vector_type(2)));
+  const fv2 v = __builtin_nontemporal_load((const fv2*)p);
+  *lo = v.x;
+  *hi = v.y;
+}
+template <>
+__device__ __forceinline__ void load_g_pair<bf16s, float>(const bf16s* p,
+                                                         float* lo, float* hi) {
+  const uint32_t v = __builtin_nontemporal_load((const uint32_t*)p);
+  *lo = __uint_as_float(v << 16);
+  *hi = __uint_as_float(v & 0xffff0000u);
+}
+
 __device__ __forceinline__ void lds_atomic_add(double* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }

@@ -134,6 +134,37 @@ __global__ void k_g3_relayout(const double* __restrict__ src,
     dst[t] = src[gi];
 }
 
+// 3D: FP64 block-interleaved G -> reduced-precision pair-interleaved G.
+template <typename TG>
+__device__ inline TG to_reduced(double v);
+template <>
+__device__ inline float to_reduced<float>(double v) { return (float)v; }
+template <>
+__device__ inline bf16s to_reduced<bf16s>(double v) {
+  bf16s h;
+  h.bits = f32_to_bf16_bits((float)v);
+  return h;
+}
+
+template <typename TG>
+__global__ void k_g3_to_pairs(const double* __restrict__ G, TG* __restrict__ out,
+                              int q, int64_t ncell,
+                              const int32_t* __restrict__ cell_block,
+                              const int64_t* __restrict__ bcell) {
+  const int64_t qqq = (int64_t)q * q * q;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncell * 6 * qqq) return;
+  const int64_t e = t / (6 * qqq);
+  const int c = (int)((t / qqq) % 6);
+  const int qp = (int)(t % qqq);
+  const int cb = cell_block[e];
+  const int64_t bs = bcell[cb];
+  const int ncb = (int)(bcell[cb + 1] - bs);
+  const int l = qp / (q * q), r = qp % (q * q);
+  out[g3_pair_index(bs, ncb, e, c, l, r, q)] =
+      to_reduced<TG>(G[g3_index(bs, ncb, e, c, l, r, q)]);
+}
+
 // 3D: Gk = kappa * G in the block-interleaved layout, kappa [cell][qp].
 __global__ void k_scale_kappa3(const double* __restrict__ G,
                                const double* __restrict__ kappa, int q,
@@ -250,6 +281,26 @@ hipError_t launch_check_positive(const double* v, int64_t n, int* bad,
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_check_positive, dim3(grid_for(n, 256)), dim3(256), 0, s,
                      v, n, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_g3_to_f32_pairs(const double* G, float* out, int q,
+                                  int64_t ncell, const int32_t* cell_block,
+                                  const int64_t* bcell, hipStream_t s) {
+  const int64_t n = ncell * 6 * q * q * q;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_g3_to_pairs<float>, dim3(grid_for(n, 256)), dim3(256), 0,
+                     s, G, out, q, ncell, cell_block, bcell);
+  return hipGetLastError();
+}
+
+hipError_t launch_g3_to_bf16_pairs(const double* G, bf16s* out, int q,
+                                   int64_t ncell, const int32_t* cell_block,
+                                   const int64_t* bcell, hipStream_t s) {
+  const int64_t n = ncell * 6 * q * q * q;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_g3_to_pairs<bf16s>, dim3(grid_for(n, 256)), dim3(256), 0,
+                     s, G, out, q, ncell, cell_block, bcell);
   return hipGetLastError();
 }
 

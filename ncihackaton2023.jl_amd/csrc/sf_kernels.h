@@ -29,6 +29,13 @@ hipError_t launch_scale_kappa(const double* G, const double* kappa, int ng,
 hipError_t launch_check_positive(const double* v, int64_t n, int* bad,
                                  hipStream_t s);
 hipError_t launch_to_f32(const double* a, int64_t n, float* b, hipStream_t s);
+// 3D reduced-precision G in the pair-interleaved layout (g3_pair_index)
+hipError_t launch_g3_to_f32_pairs(const double* G, float* out, int q,
+                                  int64_t ncell, const int32_t* cell_block,
+                                  const int64_t* bcell, hipStream_t s);
+hipError_t launch_g3_to_bf16_pairs(const double* G, bf16s* out, int q,
+                                   int64_t ncell, const int32_t* cell_block,
+                                   const int64_t* bcell, hipStream_t s);
 hipError_t launch_to_bf16(const double* a, int64_t n, bf16s* b, hipStream_t s);
 
 // ---- applies (sf_apply_*.hip) ----
