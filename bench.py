@@ -159,7 +159,7 @@ def mode_sweep(args, torch, pkg):
         row = {"order": p, "nquad": q, "m": m, "n_free": space.n_free,
                "f64_gdofs": space.n_free / wall / 1e9, "f64_kernel_ms": kern * 1e3,
                "f64_roofline_frac": troof / kern, "f64_GBs": B64 / kern / 1e9}
-        for prec in ("f32", "f32_gbf16"):
+        for prec in ("f32", "f32_gbf16", "mfma_bf16x3", "mfma_bf16"):
             x32 = x.float()
             y32 = torch.empty_like(x32)
             A.prepare(prec)
@@ -190,10 +190,17 @@ def mode_cg(args, torch, pkg):
     st = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=args.maxit)
     res["f64"] = st
     log("f64", st)
-    for inner, irt in [("f32", 1e-4), ("f32", 1e-2), ("f32_gbf16", 1e-2)]:
+    for spec in args.inner.split(","):
+        inner, irt = spec.split(":")
+        irt = float(irt)
         x = torch.zeros_like(b)
-        st = pkg.cg_ir_(x, A, b, rtol=1e-10, inner=inner, inner_rtol=irt,
-                        inner_maxiter=args.maxit)
+        try:
+            st = pkg.cg_ir_(x, A, b, rtol=1e-10, inner=inner, inner_rtol=irt,
+                            inner_maxiter=args.maxit, max_outer=args.max_outer)
+        except pkg.SFError as e:  # e.g. refinement stalls with bf16 operands
+            res["%s_inner%g" % (inner, irt)] = {"error": str(e)}
+            log(inner, irt, "failed:", e)
+            continue
         st["err_vs_f64"] = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
         st["speedup_vs_f64"] = res["f64"]["seconds"] / st["seconds"]
         res["%s_inner%g" % (inner, irt)] = st
@@ -250,6 +257,9 @@ def main():
     ap.add_argument("--mode", default="apply", choices=["apply", "sweep", "cg", "adjoint"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--maxit", type=int, default=300000)
+    ap.add_argument("--max-outer", type=int, default=50)
+    ap.add_argument("--inner", default="f32:1e-2,f32_gbf16:1e-2,mfma_bf16x3:1e-2,mfma_bf16:1e-2",
+                    help="cg mode: comma list of inner_precision:inner_rtol")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); the apply "

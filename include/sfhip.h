@@ -52,7 +52,10 @@ enum sf_precision {
   SF_PREC_F64 = 0,       /* FP64 everywhere                                       */
   SF_PREC_F32 = 1,       /* FP32 vectors, tables, G and contractions (vector ALU) */
   SF_PREC_F32_GBF16 = 2, /* as F32, but G stored in bfloat16 (half the G bytes)   */
-  SF_PREC_MFMA_BF16 = 3  /* contractions as bf16 MFMA GEMMs, FP32 accumulation    */
+  SF_PREC_MFMA_BF16 = 3, /* 3D: contractions as bf16 MFMA GEMMs (operands rounded
+                            to bf16), FP32 accumulation, G in bf16               */
+  SF_PREC_MFMA_BF16X3 = 4 /* 3D: as MFMA_BF16 with every operand split into three
+                            bf16 terms (six MFMAs per tile): FP32-level accuracy */
 };
 
 typedef struct sf_plan sf_plan; /* opaque */
@@ -185,8 +188,9 @@ int sf_apply_add(sf_plan* plan, double* y, const double* x, void* stream);
 int sf_apply_host(sf_plan* plan, double* y, const double* x);
 
 /* y = A x with FP32 vectors (device pointers).  precision is SF_PREC_F32,
- * SF_PREC_F32_GBF16 or SF_PREC_MFMA_BF16.  No reference counterpart: the
- * reduced-precision inner operator of the mixed-precision solver. */
+ * SF_PREC_F32_GBF16, SF_PREC_MFMA_BF16 or SF_PREC_MFMA_BF16X3 (the MFMA
+ * forms are 3D only: SF_ERR_UNSUPPORTED in 2D).  No reference counterpart:
+ * the reduced-precision inner operator of the mixed-precision solver. */
 int sf_apply_f32(sf_plan* plan, float* y, const float* x, int precision,
                  void* stream);
 

@@ -31,9 +31,10 @@ SF_PREC_F64 = 0
 SF_PREC_F32 = 1
 SF_PREC_F32_GBF16 = 2
 SF_PREC_MFMA_BF16 = 3
+SF_PREC_MFMA_BF16X3 = 4
 
 PRECISIONS = {"f64": SF_PREC_F64, "f32": SF_PREC_F32, "f32_gbf16": SF_PREC_F32_GBF16,
-              "mfma_bf16": SF_PREC_MFMA_BF16}
+              "mfma_bf16": SF_PREC_MFMA_BF16, "mfma_bf16x3": SF_PREC_MFMA_BF16X3}
 
 
 class SFError(RuntimeError):

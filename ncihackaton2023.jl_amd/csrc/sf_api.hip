@@ -203,8 +203,9 @@ struct sf_plan {
   DevBuf<double> G, Gk, kappa;
   DevBuf<float> G32;
   DevBuf<bf16s> Gbf;
+  DevBuf<uint32_t> Gm;  // bf16 pairs in the layout of the MFMA apply (3D)
   bool has_cells = false, has_geom = false, has_kappa = false;
-  bool g32_valid = false, gbf_valid = false;
+  bool g32_valid = false, gbf_valid = false, gm_valid = false;
   DevBuf<int> flag;
   // 3D block structure (block-assembled apply)
   int eb = 0, nblk = 0;
@@ -230,7 +231,7 @@ struct sf_plan {
     dB.release(); dD.release(); dW.release(); dPts.release();
     dB32.release(); dD32.release();
     cells.release(); G.release(); Gk.release(); kappa.release();
-    G32.release(); Gbf.release(); flag.release();
+    G32.release(); Gbf.release(); Gm.release(); flag.release();
     pidx.release(); bdofs.release(); bexcl.release(); cell_block.release();
     boff.release(); bcell.release();
     xs.release(); ys.release();
@@ -244,7 +245,7 @@ struct sf_plan {
   int64_t device_bytes() const {
     return dB.bytes() + dD.bytes() + dW.bytes() + dPts.bytes() + dB32.bytes() +
            dD32.bytes() + cells.bytes() + G.bytes() + Gk.bytes() +
-           kappa.bytes() + G32.bytes() + Gbf.bytes() + flag.bytes() +
+           kappa.bytes() + G32.bytes() + Gbf.bytes() + Gm.bytes() + flag.bytes() +
            pidx.bytes() + bdofs.bytes() + bexcl.bytes() + cell_block.bytes() +
            boff.bytes() + bcell.bytes() +
            xs.bytes() + ys.bytes() + w_r.bytes() + w_p.bytes() + w_q.bytes() +
@@ -319,10 +320,27 @@ int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
   return SF_OK;
 }
 
+bool is_reduced(int precision) {
+  return precision == SF_PREC_F32 || precision == SF_PREC_F32_GBF16 ||
+         precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X3;
+}
+
 int prepare(sf_plan* P, int precision) {
   const int64_t ng = P->ncell * P->ng * P->nqc;
   if (precision == SF_PREC_F64) return SF_OK;
-  if (precision == SF_PREC_F32 || precision == SF_PREC_MFMA_BF16) {
+  if (precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X3) {
+    if (P->dim != 3)
+      return fail(SF_ERR_UNSUPPORTED, "the MFMA apply is 3D only (dim=%d)", P->dim);
+    if (!P->gm_valid) {
+      SF_HIP(P->Gm.alloc(P->ncell * P->nqc * 3));
+      SF_HIP(launch_g3_to_mfma(P->active_G(), P->Gm.p, P->q, P->ncell,
+                               P->cell_block.p, P->bcell.p, nullptr));
+      SF_HIP(hipDeviceSynchronize());
+      P->gm_valid = true;
+    }
+    return SF_OK;
+  }
+  if (precision == SF_PREC_F32) {
     if (!P->g32_valid) {
       SF_HIP(P->G32.alloc(ng));
       if (P->dim == 3)
@@ -364,8 +382,11 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
   } else if (precision == SF_PREC_F32_GBF16) {
     e = apply_f32_gbf16(P->dim, P->n, P->q, x, y, P->cells.p, P->Gbf.p,
                         P->dB32.p, P->dD32.p, P->ncell, P->blocks(), 0, s);
+  } else if (precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X3) {
+    e = apply_mfma(P->n, P->q, precision == SF_PREC_MFMA_BF16 ? 1 : 3, x, y,
+                   P->Gm.p, P->dB32.p, P->dD32.p, P->blocks(), s);
   } else {
-    return fail(SF_ERR_UNSUPPORTED, "precision %d has no FP32 apply yet", precision);
+    return fail(SF_ERR_UNSUPPORTED, "precision %d has no FP32 apply", precision);
   }
   if (e == hipErrorNotSupported)
     return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
@@ -797,7 +818,7 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
     // new blocks: the block-interleaved geometry must be set again
     P->has_cells = false;
     P->has_geom = false;
-    P->g32_valid = P->gbf_valid = false;
+    P->g32_valid = P->gbf_valid = P->gm_valid = false;
     int rc = build_blocks(P, h);
     if (rc) return rc;
   }
@@ -807,7 +828,7 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
 }
 
 static int finish_geometry(sf_plan* P) {
-  P->g32_valid = P->gbf_valid = false;
+  P->g32_valid = P->gbf_valid = P->gm_valid = false;
   if (P->has_kappa) {
     SF_HIP(P->Gk.alloc(P->ncell * P->ng * P->nqc));
     if (P->dim == 3)
@@ -933,7 +954,7 @@ int sf_plan_set_kappa(sf_plan* P, const double* kappa) {
   if (n > 0) SF_HIP(hipMemcpy(P->kappa.p, kappa, n * 8, hipMemcpyHostToDevice));
   P->has_kappa = true;
   if (P->has_geom) return finish_geometry(P);
-  P->g32_valid = P->gbf_valid = false;
+  P->g32_valid = P->gbf_valid = P->gm_valid = false;
   return SF_OK;
   SF_CATCH
 }
@@ -993,7 +1014,7 @@ int sf_apply_f32(sf_plan* P, float* y, const float* x, int precision,
   int rc = check_ready(P);
   if (rc) return rc;
   if ((!x || !y) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
-  if (precision != SF_PREC_F32 && precision != SF_PREC_F32_GBF16 && precision != SF_PREC_MFMA_BF16)
+  if (!is_reduced(precision))
     return fail(SF_ERR_ARG, "precision %d is not a reduced-precision operator", precision);
   DeviceGuard g(P->device);
   rc = prepare(P, precision);
@@ -1075,7 +1096,7 @@ int sf_cg_ir(sf_plan* P, const double* b, double* x, double rtol,
   if ((!b || !x) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
   if (!(rtol >= 0.0) || max_outer < 0 || !(inner_rtol > 0.0) || inner_maxit < 1)
     return fail(SF_ERR_ARG, "bad tolerances / iteration limits");
-  if (inner_precision != SF_PREC_F32 && inner_precision != SF_PREC_F32_GBF16)
+  if (!is_reduced(inner_precision))
     return fail(SF_ERR_UNSUPPORTED, "inner precision %d not available for refinement", inner_precision);
   DeviceGuard g(P->device);
   hipStream_t s = (hipStream_t)stream;

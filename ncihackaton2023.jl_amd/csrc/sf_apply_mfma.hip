@@ -1,0 +1,478 @@
+// sf_apply_mfma.hip -- 3D apply with the tensor contractions on the matrix
+// cores (v_mfma_f32_16x16x16_bf16: bf16 operands, FP32 accumulation).
+//
+// Same operator, block structure and assembly as k_apply3d (sf_apply.hpp);
+// FP32 vectors, the geometric factor in bfloat16.  Every 1D contraction of
+// a cell is a small GEMM: table (rows = output index, K = contracted index)
+// times the cell tensor viewed as K x (the other two indices).  A 16x16x16
+// MFMA takes one 16-column tile of that view; tiles of all cells of the
+// workgroup are dealt to its four waves.
+//
+//   forward   X : B, D   (Q x N)  . x      [a][(b,c)]        ->  2 tensors
+//             Y : B, D, B         . (Bx, Dx) [b][(i1,c)]      ->  3 tensors
+//             Z : B, B, D         . [c][(i1,i2)] -> gradients in registers
+//   pointwise    h = G grad  (the MFMA output tile is the next B operand)
+//   backward  Z^T: B^T, B^T, D^T  -> in place over the Y output
+//             Y^T: B^T, D^T + B^T -> 2 tensors [i1][(b,c)]
+//             X^T: D^T + B^T      -> y, assembled in LDS as in k_apply3d
+//
+// Operand precision.  SPLIT = 1 rounds every operand to bf16 (8 significant
+// bits).  SPLIT = 3 writes each FP32 operand as the sum of three bf16
+// terms (hi + mid + lo, each the rounded remainder of the previous) and
+// keeps the six partial products of order >= 2^-24, so the contraction is
+// as accurate as FP32 arithmetic at six MFMAs per tile.
+//
+// MFMA 16x16x16 bf16 lane layout (wave64, lane l, g = l / 16, j = 0..3):
+//   A[m][k]: m = l % 16, k = 4 g + j      B[k][n]: n = l % 16, k = 4 g + j
+//   C[m][n]: n = l % 16, m = 4 g + j
+// so a C tile is, lane for lane, the B operand of a contraction over m.
+
+#include "sf_apply.hpp"
+#include "sf_kernels.h"
+
+namespace sf {
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+
+template <int SPLIT>
+struct Op {
+  s4 t[SPLIT];
+};
+
+// FP32 -> SPLIT bf16 terms (round to nearest; each remainder is exact).
+template <int SPLIT>
+__device__ __forceinline__ Op<SPLIT> split4(f4 v) {
+  Op<SPLIT> o;
+#pragma unroll
+  for (int s = 0; s < SPLIT; ++s) {
+    const bf4 h = __builtin_convertvector(v, bf4);
+    o.t[s] = __builtin_bit_cast(s4, h);
+    if (s + 1 < SPLIT) v = v - __builtin_convertvector(h, f4);
+  }
+  return o;
+}
+
+__device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// c += a . b with split operands, small terms first.
+template <int SPLIT>
+__device__ __forceinline__ f4 mma(const Op<SPLIT>& a, const Op<SPLIT>& b, f4 c) {
+  if constexpr (SPLIT >= 3) {
+    c = mfma(a.t[2], b.t[0], c);
+    c = mfma(a.t[1], b.t[1], c);
+    c = mfma(a.t[0], b.t[2], c);
+  }
+  if constexpr (SPLIT >= 2) {
+    c = mfma(a.t[1], b.t[0], c);
+    c = mfma(a.t[0], b.t[1], c);
+  }
+  return mfma(a.t[0], b.t[0], c);
+}
+
+template <int N, int Q, int SPLIT>
+struct MfmaShape {
+  static constexpr int EB = Apply3Shape<N, Q, double>::EB;  // block structure
+  static constexpr int NT = 256, W = NT / 64;
+  static constexpr int NN = N * N, NQ = N * Q, QQ = Q * Q;
+  static constexpr int NNN = N * N * N, QQQ = Q * Q * Q;
+  static constexpr int TX = (NN + 15) / 16, TY = (NQ + 15) / 16, TZ = (QQ + 15) / 16;
+  static constexpr int UX = (EB * TX + W - 1) / W;  // tiles per wave, X and X^T
+  static constexpr int UY = (EB * TY + W - 1) / W;
+  static constexpr int UZ = (EB * TZ + W - 1) / W;
+  static constexpr int S1 = N * Q * N;  // one X-pass (or Y^T-pass) tensor
+  static constexpr int S2 = Q * Q * N;  // one Y-pass tensor
+  static constexpr int RX = EB * NNN;   // the block's x values, later its y
+  static constexpr int LDS_FLOATS = RX + EB * (2 * S1 + 3 * S2);
+  static constexpr int KU = (EB * NNN + NT - 1) / NT;
+  static constexpr bool GPRE = UZ <= 4;  // G of every Z tile loaded up front
+  static_assert(N <= 16 && Q <= 16, "one MFMA tile per contraction");
+  static_assert(LDS_FLOATS * 4 <= 65536, "apply3d_mfma: LDS above 64 KiB");
+};
+
+// G of one Z tile for lane (g, cl): rows i3 = 4g + j, column col; the pair
+// p of entries (2p, 2p+1) of point (i3, col) sits at
+// cell * 6Q^3 + ((i3 * 3 + p) * Q^2 + col) * 2  (bf16 units).
+template <int N, int Q>
+__device__ __forceinline__ void load_gz(const uint32_t* __restrict__ Gm,
+                                        int64_t cell, int col, int g, bool ok,
+                                        uint32_t gv[12]) {
+  constexpr int QQ = Q * Q;
+  const uint32_t* p = Gm + cell * 3 * QQ * Q;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int i3 = 4 * g + j;
+    const bool v = ok && i3 < Q && col < QQ;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      gv[j * 3 + k] = v ? __builtin_nontemporal_load(p + (i3 * 3 + k) * QQ + col) : 0u;
+  }
+}
+
+template <int N, int Q, int SPLIT>
+__global__ void __launch_bounds__(256)
+    k_apply3d_mfma(const float* __restrict__ x, float* __restrict__ y,
+                   const uint16_t* __restrict__ pidx,
+                   const int32_t* __restrict__ bdofs,
+                   const int64_t* __restrict__ boff,
+                   const int32_t* __restrict__ bexcl,
+                   const int64_t* __restrict__ bcell, int nblk,
+                   const uint32_t* __restrict__ Gm,
+                   const float* __restrict__ tB, const float* __restrict__ tD) {
+  using M = MfmaShape<N, Q, SPLIT>;
+  constexpr int NN = M::NN, NQ = M::NQ, QQ = M::QQ, NNN = M::NNN;
+  constexpr int S1 = M::S1, S2 = M::S2, W = M::W;
+  __shared__ float smem[M::LDS_FLOATS];
+  float* const bx = smem;
+  float* const T1 = smem + M::RX;
+  float* const T2 = T1 + M::EB * 2 * S1;
+
+  const int b = xcd_remap(blockIdx.x, nblk);
+  const int64_t base = bcell[b];
+  const int ncb = (int)(bcell[b + 1] - base);
+  const int tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63, g = lane >> 4, cl = lane & 15;
+  const int64_t d0 = boff[b];
+  const int nu = (int)(boff[b + 1] - d0);
+  const int nexcl = bexcl[b];
+  const int32_t* __restrict__ dofs = bdofs + d0;
+
+  // ---- prologue: dof ids, x, local indices, G ------------------------------
+  int32_t dv[M::KU];
+  float xv[M::KU];
+#pragma unroll
+  for (int k = 0; k < M::KU; ++k) {
+    const int i = tid + k * M::NT;
+    dv[k] = i < nu ? dofs[i] : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < M::KU; ++k) xv[k] = x[dv[k] >= 0 ? dv[k] : 0];
+
+  // X-tile u = w + W i: cell u / TX, tile u % TX; lane value j is node
+  // (a = 4g + j, col = b + N c) -- the same slots in X (B operand) and X^T
+  // (C tile), so one set of local indices serves both.
+  uint16_t pv[M::UX][4];
+#pragma unroll
+  for (int i = 0; i < M::UX; ++i) {
+    const int u = w + W * i;
+    const int e = u / M::TX, col = (u - e * M::TX) * 16 + cl;
+    const bool ok = u < M::EB * M::TX && e < ncb && col < NN;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int a = 4 * g + j;
+      pv[i][j] = (ok && a < N) ? pidx[(base + e) * NNN + a + N * col] : kNoDof;
+    }
+  }
+
+  uint32_t gv[M::GPRE ? M::UZ : 1][12];
+  if constexpr (M::GPRE) {
+#pragma unroll
+    for (int i = 0; i < M::UZ; ++i) {
+      const int u = w + W * i;
+      const int e = u / M::TZ, col = (u - e * M::TZ) * 16 + cl;
+      load_gz<N, Q>(Gm, base + e, col, g, u < M::EB * M::TZ && e < ncb, gv[i]);
+    }
+  }
+
+  // tables as A operands: forward A[m][k] = T[m][k] (m < Q, k < N),
+  // backward A[m][k] = T[k][m] (m < N, k < Q); zero outside
+  Op<SPLIT> aB, aD, aBt, aDt;
+  {
+    f4 vb, vd, vbt, vdt;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = 4 * g + j;
+      const bool f = cl < Q && k < N, r = cl < N && k < Q;
+      vb[j] = f ? tB[cl * N + k] : 0.f;
+      vd[j] = f ? tD[cl * N + k] : 0.f;
+      vbt[j] = r ? tB[k * N + cl] : 0.f;
+      vdt[j] = r ? tD[k * N + cl] : 0.f;
+    }
+    aB = split4<SPLIT>(vb);
+    aD = split4<SPLIT>(vd);
+    aBt = split4<SPLIT>(vbt);
+    aDt = split4<SPLIT>(vdt);
+  }
+
+#pragma unroll
+  for (int k = 0; k < M::KU; ++k) {
+    const int i = tid + k * M::NT;
+    if (i < nu) bx[i] = xv[k];
+  }
+  __syncthreads();
+
+  const f4 zero = {0.f, 0.f, 0.f, 0.f};
+
+  // ---- X: [a][(b,c)] -> B x, D x stored [b][(i1 + Q c)] ---------------------
+#pragma unroll
+  for (int i = 0; i < M::UX; ++i) {
+    const int u = w + W * i;
+    if (u >= M::EB * M::TX) break;
+    const int e = u / M::TX;
+    if (e >= ncb) break;
+    const int col = (u - e * M::TX) * 16 + cl;
+    f4 v;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = pv[i][j] != kNoDof ? bx[pv[i][j]] : 0.f;
+    const Op<SPLIT> o = split4<SPLIT>(v);
+    const f4 cb = mma(aB, o, zero), cd = mma(aD, o, zero);
+    if (col < NN) {
+      const int bb = col % N, c = col / N;
+      float* t = T1 + e * 2 * S1 + bb * NQ + Q * c;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i1 = 4 * g + j;
+        if (i1 < Q) {
+          t[i1] = cb[j];
+          t[S1 + i1] = cd[j];
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // the x values are consumed: region bx becomes the block's y accumulator
+#pragma unroll
+  for (int k = 0; k < M::KU; ++k) {
+    const int i = tid + k * M::NT;
+    if (i < nu) bx[i] = 0.f;
+  }
+
+  // ---- Y: K = b -> (B.Bx, D.Bx, B.Dx) stored [c][(i1 + Q i2)] ---------------
+#pragma unroll
+  for (int i = 0; i < M::UY; ++i) {
+    const int u = w + W * i;
+    if (u >= M::EB * M::TY) break;
+    const int e = u / M::TY;
+    if (e >= ncb) break;
+    const int col = (u - e * M::TY) * 16 + cl;  // i1 + Q c
+    const float* t = T1 + e * 2 * S1;
+    f4 vx, vd;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int bb = 4 * g + j;
+      const bool ok = bb < N && col < NQ;
+      vx[j] = ok ? t[bb * NQ + col] : 0.f;
+      vd[j] = ok ? t[S1 + bb * NQ + col] : 0.f;
+    }
+    const Op<SPLIT> ox = split4<SPLIT>(vx), od = split4<SPLIT>(vd);
+    const f4 cbb = mma(aB, ox, zero), cbd = mma(aD, ox, zero), cdb = mma(aB, od, zero);
+    if (col < NQ) {
+      const int i1 = col % Q, c = col / Q;
+      float* o = T2 + e * 3 * S2 + c * QQ + i1;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i2 = 4 * g + j;
+        if (i2 < Q) {
+          o[Q * i2] = cdb[j];           // x-gradient path
+          o[S2 + Q * i2] = cbd[j];      // y-gradient path
+          o[2 * S2 + Q * i2] = cbb[j];  // z-gradient path
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- Z, G, Z^T: K = c; every lane rewrites exactly the words it read ------
+#pragma unroll
+  for (int i = 0; i < M::UZ; ++i) {
+    const int u = w + W * i;
+    if (u >= M::EB * M::TZ) break;
+    const int e = u / M::TZ;
+    if (e >= ncb) break;
+    const int col = (u - e * M::TZ) * 16 + cl;  // i1 + Q i2
+    float* t = T2 + e * 3 * S2 + col;
+    uint32_t gl[12];
+    if constexpr (M::GPRE) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) gl[k] = gv[i][k];
+    } else {
+      load_gz<N, Q>(Gm, base + e, col, g, true, gl);
+    }
+    f4 v0, v1, v2;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = 4 * g + j;
+      const bool ok = c < N && col < QQ;
+      v0[j] = ok ? t[c * QQ] : 0.f;
+      v1[j] = ok ? t[S2 + c * QQ] : 0.f;
+      v2[j] = ok ? t[2 * S2 + c * QQ] : 0.f;
+    }
+    const f4 gx = mma(aB, split4<SPLIT>(v0), zero);
+    const f4 gy = mma(aB, split4<SPLIT>(v1), zero);
+    const f4 gz = mma(aD, split4<SPLIT>(v2), zero);
+    f4 h0, h1, h2;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      // entries xx, xy | xz, yy | yz, zz (zero outside the Q^3 points)
+      const float g0 = __uint_as_float(gl[3 * j] << 16);
+      const float g1 = __uint_as_float(gl[3 * j] & 0xffff0000u);
+      const float g2 = __uint_as_float(gl[3 * j + 1] << 16);
+      const float g3 = __uint_as_float(gl[3 * j + 1] & 0xffff0000u);
+      const float g4 = __uint_as_float(gl[3 * j + 2] << 16);
+      const float g5 = __uint_as_float(gl[3 * j + 2] & 0xffff0000u);
+      h0[j] = g0 * gx[j] + g1 * gy[j] + g2 * gz[j];
+      h1[j] = g1 * gx[j] + g3 * gy[j] + g4 * gz[j];
+      h2[j] = g2 * gx[j] + g4 * gy[j] + g5 * gz[j];
+    }
+    const f4 ax = mma(aBt, split4<SPLIT>(h0), zero);
+    const f4 ay = mma(aBt, split4<SPLIT>(h1), zero);
+    const f4 az = mma(aDt, split4<SPLIT>(h2), zero);
+    if (col < QQ) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = 4 * g + j;
+        if (c < N) {
+          t[c * QQ] = ax[j];
+          t[S2 + c * QQ] = ay[j];
+          t[2 * S2 + c * QQ] = az[j];
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- Y^T: K = i2 -> B^T (x path), D^T (y) + B^T (z), stored [i1][(b,c)] ---
+#pragma unroll
+  for (int i = 0; i < M::UY; ++i) {
+    const int u = w + W * i;
+    if (u >= M::EB * M::TY) break;
+    const int e = u / M::TY;
+    if (e >= ncb) break;
+    const int col = (u - e * M::TY) * 16 + cl;  // i1 + Q c
+    const int i1 = col % Q, c = col / Q;
+    const float* t = T2 + e * 3 * S2 + c * QQ + i1;
+    f4 v0, v1, v2;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i2 = 4 * g + j;
+      const bool ok = i2 < Q && col < NQ;
+      v0[j] = ok ? t[Q * i2] : 0.f;
+      v1[j] = ok ? t[S2 + Q * i2] : 0.f;
+      v2[j] = ok ? t[2 * S2 + Q * i2] : 0.f;
+    }
+    const f4 b0 = mma(aBt, split4<SPLIT>(v0), zero);
+    f4 b12 = mma(aDt, split4<SPLIT>(v1), zero);
+    b12 = mma(aBt, split4<SPLIT>(v2), b12);
+    if (col < NQ) {
+      float* o = T1 + e * 2 * S1 + i1 * NN + N * c;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int bb = 4 * g + j;
+        if (bb < N) {
+          o[bb] = b0[j];        // then D_x^T
+          o[S1 + bb] = b12[j];  // then B_x^T
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- X^T: K = i1 -> y of node (a, b, c), assembled in LDS ----------------
+#pragma unroll
+  for (int i = 0; i < M::UX; ++i) {
+    const int u = w + W * i;
+    if (u >= M::EB * M::TX) break;
+    const int e = u / M::TX;
+    if (e >= ncb) break;
+    const int col = (u - e * M::TX) * 16 + cl;  // b + N c
+    const float* t = T1 + e * 2 * S1 + col;
+    f4 v0, v1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i1 = 4 * g + j;
+      const bool ok = i1 < Q && col < NN;
+      v0[j] = ok ? t[i1 * NN] : 0.f;
+      v1[j] = ok ? t[S1 + i1 * NN] : 0.f;
+    }
+    f4 yv = mma(aDt, split4<SPLIT>(v0), zero);
+    yv = mma(aBt, split4<SPLIT>(v1), yv);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (pv[i][j] != kNoDof) lds_atomic_add(bx + pv[i][j], yv[j]);
+  }
+  __syncthreads();
+
+  // ---- write-back: exclusive dofs stored, shared dofs added ----------------
+#pragma unroll
+  for (int k = 0; k < M::KU; ++k) {
+    const int i = tid + k * M::NT;
+    if (dv[k] < 0) continue;
+    const float v = bx[i];
+    if (i < nexcl)
+      y[dv[k]] = v;
+    else
+      atomic_add(y + dv[k], v);
+  }
+}
+
+template <int N, int Q, int SPLIT>
+hipError_t launch_mfma_nq(const float* x, float* y, const uint32_t* Gm,
+                          const float* tB, const float* tD, const BlockArgs& k,
+                          hipStream_t s) {
+  if (k.nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_apply3d_mfma<N, Q, SPLIT>), dim3(k.nblk), dim3(256), 0, s,
+                     x, y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell, k.nblk,
+                     Gm, tB, tD);
+  return hipGetLastError();
+}
+
+// G (FP64, block-interleaved) -> bf16 pairs in the MFMA layout of load_gz.
+__global__ void k_g3_to_mfma(const double* __restrict__ G, uint32_t* __restrict__ out,
+                             int q, int64_t ncell, const int32_t* __restrict__ cell_block,
+                             const int64_t* __restrict__ bcell) {
+  const int64_t QQ = (int64_t)q * q, QQQ = QQ * q;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncell * QQQ) return;
+  const int64_t cell = t / QQQ;
+  const int qp = (int)(t - cell * QQQ);
+  const int i3 = qp / (int)QQ, col = qp - i3 * (int)QQ;
+  const int blk = cell_block[cell];
+  const int64_t b0 = bcell[blk];
+  const int ncb = (int)(bcell[blk + 1] - b0);
+  uint32_t* o = out + cell * 3 * QQQ;
+  for (int p = 0; p < 3; ++p) {
+    const float lo = (float)G[g3_index(b0, ncb, cell, 2 * p, i3, col, q)];
+    const float hi = (float)G[g3_index(b0, ncb, cell, 2 * p + 1, i3, col, q)];
+    o[(i3 * 3 + p) * QQ + col] =
+        (uint32_t)f32_to_bf16_bits(lo) | ((uint32_t)f32_to_bf16_bits(hi) << 16);
+  }
+}
+
+}  // namespace
+
+hipError_t apply_mfma(int n, int q, int split, const float* x, float* y,
+                      const uint32_t* Gm, const float* tB, const float* tD,
+                      const BlockArgs* k, hipStream_t s) {
+  if (!k) return hipErrorInvalidValue;
+  if (split != 1 && split != 3) return hipErrorNotSupported;
+  switch (n * 16 + q) {
+#define SF_CASE(N_, Q_)                                                  \
+  case N_ * 16 + Q_:                                                     \
+    return split == 1 ? launch_mfma_nq<N_, Q_, 1>(x, y, Gm, tB, tD, *k, s) \
+                      : launch_mfma_nq<N_, Q_, 3>(x, y, Gm, tB, tD, *k, s);
+    SF_FOR_EACH_NQ(SF_CASE)
+#undef SF_CASE
+    default:
+      return hipErrorNotSupported;
+  }
+}
+
+hipError_t launch_g3_to_mfma(const double* G, uint32_t* out, int q, int64_t ncell,
+                             const int32_t* cell_block, const int64_t* bcell,
+                             hipStream_t s) {
+  const int64_t n = ncell * (int64_t)q * q * q;
+  if (n <= 0) return hipSuccess;
+  const int64_t nb = (n + 255) / 256;
+  if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_g3_to_mfma, dim3((unsigned)nb), dim3(256), 0, s, G, out, q,
+                     ncell, cell_block, bcell);
+  return hipGetLastError();
+}
+
+}  // namespace sf

@@ -54,6 +54,15 @@ hipError_t apply_f32_gbf16(int dim, int n, int q, const float* x, float* y,
                            const float* tB, const float* tD, int64_t ncell,
                            const BlockArgs* k, int accumulate, hipStream_t s);
 void apply_shape_f64(int dim, int n, int q, int* nt, int* eb);
+// 3D, matrix cores: y = A x over the block structure k (y zeroed by the
+// caller); split = 1 (bf16 operands) or 3 (three-term bf16 split).
+hipError_t apply_mfma(int n, int q, int split, const float* x, float* y,
+                      const uint32_t* Gm, const float* tB, const float* tD,
+                      const BlockArgs* k, hipStream_t s);
+// G (FP64, block-interleaved) -> the bf16-pair layout of apply_mfma
+hipError_t launch_g3_to_mfma(const double* G, uint32_t* out, int q,
+                             int64_t ncell, const int32_t* cell_block,
+                             const int64_t* bcell, hipStream_t s);
 
 // ---- adjoint (sf_adjoint.hip) ----
 // dJdk[cell][qp] = grad(lam) . G grad(u);  ybar += A(kappa) lam

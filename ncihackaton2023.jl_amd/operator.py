@@ -126,7 +126,7 @@ class LazyMatrix:
 
     def mul_reduced(self, y, x, precision="f32"):
         """y = A x with FP32 device vectors in a reduced-precision operator
-        ('f32', 'f32_gbf16' or 'mfma_bf16')."""
+        ('f32', 'f32_gbf16', 'mfma_bf16' or 'mfma_bf16x3'; the MFMA forms are 3D only)."""
         self._check_dev(x, y, "float32")
         prec = _lib.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         _lib.check(self._L.sf_apply_f32(self.plan, ctypes.c_void_p(y.data_ptr()),

@@ -36,14 +36,55 @@ def test_reduced_precision_apply_error(pkg, oracle_mod, cuda, D, m, p, q):
     x32 = cuda.from_numpy(x.astype(np.float32)).cuda()
     y32 = cuda.empty_like(x32)
     errs = {}
-    for prec, tol in [("f32", 1e-5), ("f32_gbf16", 2e-2)]:
+    cases = [("f32", 1e-5), ("f32_gbf16", 2e-2)]
+    if D == 3:
+        cases += [("mfma_bf16x3", 2e-2), ("mfma_bf16", 5e-2)]
+    for prec, tol in cases:
         A.mul_reduced(y32, x32, prec)
         cuda.cuda.synchronize()
         errs[prec] = rel_l2(y32.cpu().numpy().astype(np.float64), y_ref)
         assert errs[prec] < tol, (prec, errs[prec])
     assert errs["f32_gbf16"] > errs["f32"]
+    if D == 3:
+        # bf16x3 operands carry FP32 accuracy: same error as bf16 G alone
+        assert abs(errs["mfma_bf16x3"] - errs["f32_gbf16"]) < 1e-3 * errs["f32_gbf16"] + 1e-6
+        assert errs["mfma_bf16"] > errs["mfma_bf16x3"]
+    else:
+        with pytest.raises(pkg.SFError) as ei:
+            A.mul_reduced(y32, x32, "mfma_bf16")
+        assert ei.value.code == 2  # SF_ERR_UNSUPPORTED: the MFMA apply is 3D only
     say("errors", errs)
     A.free()
+
+
+NQ = [(n, q) for n in range(2, 11) for q in range(n - 1, n + 4)]
+
+
+def test_mfma_every_instantiated_kernel(pkg, cuda):
+    """Every (n, q) of the MFMA apply against the FP32 vector-ALU apply that
+    reads the same bf16 G: with three-term operands the two differ by FP32
+    rounding only, so a wrong lane layout, table transpose or tile bound
+    shows at once.  With one-term operands the gap is bf16 rounding."""
+    worst3 = worst1 = 0.0
+    for n, q in NQ:
+        p = n - 1
+        m = 3 if n <= 5 else 2
+        say("mfma case p=%d q=%d m=%d" % (p, q, m))
+        model, space, mp, A = make(pkg, 3, m, p, q)
+        x32 = cuda.from_numpy(pkg.uniform(0, space.n_free, -1, 1).astype(np.float32)).cuda()
+        ys = {}
+        for prec in ("f32_gbf16", "mfma_bf16x3", "mfma_bf16"):
+            y = cuda.empty_like(x32)
+            A.mul_reduced(y, x32, prec)
+            cuda.cuda.synchronize()
+            ys[prec] = y.cpu().numpy().astype(np.float64)
+        e3 = rel_l2(ys["mfma_bf16x3"], ys["f32_gbf16"])
+        e1 = rel_l2(ys["mfma_bf16"], ys["f32_gbf16"])
+        worst3, worst1 = max(worst3, e3), max(worst1, e1)
+        assert e3 < 1e-5, (p, q, e3)
+        assert e1 < 5e-2, (p, q, e1)
+        A.free()
+    say("worst rel. l2 vs FP32 apply: bf16x3 %.2e, bf16 %.2e" % (worst3, worst1))
 
 
 @pytest.mark.parametrize("D,m,p,q", [(3, 4, 5, 7), (3, 3, 3, 5), (2, 8, 4, 6)])
@@ -157,7 +198,7 @@ def test_cg_maxiter_and_zero_rhs(pkg, cuda):
     A.free()
 
 
-@pytest.mark.parametrize("inner", ["f32", "f32_gbf16"])
+@pytest.mark.parametrize("inner", ["f32", "f32_gbf16", "mfma_bf16x3"])
 def test_mixed_precision_refinement(pkg, oracle_mod, cuda, inner):
     say("IR inner=%s" % inner)
     model, space, mp, A = make(pkg, 3, 6, 5, 7)
