@@ -190,7 +190,16 @@ def mode_cg(args, torch, pkg):
     st = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=args.maxit)
     res["f64"] = st
     log("f64", st)
-    for spec in args.inner.split(","):
+    for spec in [t for t in args.mixed.split(",") if t]:
+        inner, delta = spec.split(":")
+        x = torch.zeros_like(b)
+        st = pkg.cg_mixed_(x, A, b, rtol=1e-10, inner=inner, delta=float(delta),
+                           maxiter=args.maxit)
+        st["err_vs_f64"] = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
+        st["speedup_vs_f64"] = res["f64"]["seconds"] / st["seconds"]
+        res["mixed_%s_delta%s" % (inner, delta)] = st
+        log("mixed", inner, delta, st)
+    for spec in [t for t in args.inner.split(",") if t]:
         inner, irt = spec.split(":")
         irt = float(irt)
         x = torch.zeros_like(b)
@@ -259,7 +268,11 @@ def main():
     ap.add_argument("--maxit", type=int, default=300000)
     ap.add_argument("--max-outer", type=int, default=50)
     ap.add_argument("--inner", default="f32:1e-2,f32_gbf16:1e-2,mfma_bf16x3:1e-2,mfma_bf16:1e-2",
-                    help="cg mode: comma list of inner_precision:inner_rtol")
+                    help="cg mode: restarted refinement runs, comma list of "
+                         "inner_precision:inner_rtol")
+    ap.add_argument("--mixed", default="f32:0.1,f32_gbf16:0.1,mfma_bf16x3:0.1,mfma_bf16:0.1",
+                    help="cg mode: reliable-update mixed CG runs, comma list of "
+                         "inner_precision:delta")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); the apply "

@@ -619,6 +619,127 @@ int cg_f32_inner(sf_plan* P, int precision, double inner_rtol, int maxit,
   return SF_OK;
 }
 
+int write_scalars(sf_plan* P, const CGScalars* h, hipStream_t s) {
+  SF_HIP(hipMemcpyAsync(P->scal.p, h, sizeof(CGScalars), hipMemcpyHostToDevice, s));
+  SF_HIP(hipStreamSynchronize(s));
+  return SF_OK;
+}
+
+int dot_host_f32(sf_plan* P, const float* a, const float* b, double* out,
+                 hipStream_t s) {
+  Reducer red{P->red_part.p, P->red_counter.p};
+  SF_HIP(launch_dot<float>(a, b, P->nfree, red, P->dscal.p, s));
+  SF_HIP(hipMemcpyAsync(out, P->dscal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  SF_HIP(hipStreamSynchronize(s));
+  return SF_OK;
+}
+
+// FP64 residual w_r = b - A x and its norm.
+int true_residual(sf_plan* P, const double* b, const double* x, double* rn,
+                  hipStream_t s) {
+  int rc = do_apply_f64(P, P->w_t.p, x, s);
+  if (rc) return rc;
+  SF_HIP(launch_residual(b, P->w_t.p, P->w_r.p, P->nfree, s));
+  double rr = 0.0;
+  rc = dot_host(P, P->w_r.p, P->w_r.p, &rr, s);
+  if (rc) return rc;
+  *rn = sqrt(rr);
+  return SF_OK;
+}
+
+// Mixed-precision CG with reliable updates (residual replacement, van der
+// Vorst & Ye 2000; the mixed-precision form of Clark et al. 2010).  One CG
+// recurrence runs in FP32 on the reduced-precision operator, in units of
+// the last true residual norm rn (so ||r|| starts at 1).  When the
+// recurred residual has dropped below max(delta, rtol ||b|| / rn), x (FP64)
+// absorbs rn * xl, the residual is replaced by the FP64 b - A x, and the
+// recurrence continues with the same search direction:
+//   p <- r_true + beta' p,  beta' = ||r_true||^2 / ||r_{k-1}||^2,
+// rescaled to the new units.  Krylov information survives the update,
+// unlike a restarted refinement.
+int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
+                double rtol, int maxit, double delta, hipStream_t s,
+                int* iters_out, int* updates_out, int* nonfinite_out,
+                double* rel_out) {
+  Reducer red{P->red_part.p, P->red_counter.p};
+  float* r = P->w_s32.p;
+  float* xl = P->w_d32.p;
+  float* p = P->w_p32.p;
+  float* q = P->w_q32.p;
+  double bb = 0.0;
+  int rc = dot_host(P, b, b, &bb, s);
+  if (rc) return rc;
+  const double bnorm = sqrt(bb), target = rtol * bnorm;
+  double rn = 0.0;
+  rc = true_residual(P, b, x, &rn, s);
+  if (rc) return rc;
+  *iters_out = 0;
+  *updates_out = 0;
+  *nonfinite_out = !isfinite(rn);
+  *rel_out = bnorm > 0 ? rn / bnorm : rn;
+  if (!isfinite(rn) || rn <= target || rn == 0.0) return SF_OK;
+  auto threshold = [&](double rnow) {
+    const double t = target / rnow;
+    return t > delta ? t : delta;
+  };
+  SF_HIP(launch_scale_to_f32(P->w_r.p, 1.0 / rn, P->w_r32.p, P->nfree, s));
+  SF_HIP(launch_fill<float>(xl, P->nfree, 0.0f, s));
+  const double t0 = threshold(rn);
+  SF_HIP(launch_cg_start<float>(P->w_r32.p, xl, r, p, q, P->nfree, red, P->scal.p,
+                                t0 * t0, s));
+  CGScalars h{};
+  rc = read_scalars(P, &h, s);
+  if (rc) return rc;
+  const int chunk = 8;
+  for (;;) {
+    while (!h.done && h.iters < maxit) {
+      const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
+      for (int k = 0; k < todo; ++k) {
+        rc = do_apply_f32(P, q, p, precision, s, true);
+        if (rc) return rc;
+        SF_HIP(launch_cg_alpha<float>(p, q, P->nfree, red, P->scal.p, s));
+        SF_HIP(launch_cg_update<float>(xl, r, p, q, P->nfree, red, P->scal.p, s));
+        SF_HIP(launch_cg_dir<float>(p, r, q, P->nfree, P->scal.p, s));
+      }
+      rc = read_scalars(P, &h, s);
+      if (rc) return rc;
+    }
+    *iters_out = h.iters;
+    if (h.nonfinite) {
+      *nonfinite_out = 1;
+      return SF_OK;
+    }
+    // reliable update: x += rn xl, r <- b - A x (FP64)
+    SF_HIP(launch_axpy_f32_to_f64(x, xl, rn, P->nfree, s));
+    double rn_new = 0.0;
+    rc = true_residual(P, b, x, &rn_new, s);
+    if (rc) return rc;
+    *updates_out += 1;
+    *rel_out = bnorm > 0 ? rn_new / bnorm : rn_new;
+    if (!isfinite(rn_new)) {
+      *nonfinite_out = 1;
+      return SF_OK;
+    }
+    if (rn_new <= target || rn_new == 0.0 || h.iters >= maxit) return SF_OK;
+    // continue the recurrence in units of rn_new; the queued pieces after
+    // `done` were no-ops, so p is the last direction and q is stale
+    const double c = (h.rr_new > 0.0) ? (rn_new / rn) * h.beta / h.rr_new : 0.0;
+    SF_HIP(launch_scale_to_f32(P->w_r.p, 1.0 / rn_new, r, P->nfree, s));
+    SF_HIP(launch_dir_coef(p, r, q, (float)c, P->nfree, s));
+    SF_HIP(launch_fill<float>(xl, P->nfree, 0.0f, s));
+    double rr = 0.0;
+    rc = dot_host_f32(P, r, r, &rr, s);
+    if (rc) return rc;
+    rn = rn_new;
+    const double t = threshold(rn);
+    h.rr = rr;
+    h.stop_rr = t * t;
+    h.done = 0;
+    rc = write_scalars(P, &h, s);
+    if (rc) return rc;
+  }
+}
+
 double now_s() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -1083,6 +1204,46 @@ int sf_cg(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   if (nonfinite || !isfinite(rel)) return fail(SF_ERR_NONFINITE, "CG met a non-finite value");
   if (rel > rtol && iters >= maxit)
     return fail(SF_ERR_NOT_CONVERGED, "CG did not reach rtol %g in %d iterations (rel. residual %g)", rtol, maxit, rel);
+  return SF_OK;
+  SF_CATCH
+}
+
+int sf_cg_mixed(sf_plan* P, const double* b, double* x, double rtol, int maxit,
+                int inner_precision, double delta, sf_solve_stats* stats,
+                void* stream) {
+  SF_TRY
+  int rc = check_ready(P);
+  if (rc) return rc;
+  if ((!b || !x) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
+  if (!(rtol >= 0.0) || maxit < 0 || !(delta > 0.0 && delta < 1.0))
+    return fail(SF_ERR_ARG, "bad rtol / maxit / delta");
+  if (!is_reduced(inner_precision))
+    return fail(SF_ERR_UNSUPPORTED, "inner precision %d not available", inner_precision);
+  DeviceGuard g(P->device);
+  hipStream_t s = (hipStream_t)stream;
+  rc = prepare(P, inner_precision);
+  if (rc) return rc;
+  rc = ensure_solver_ws(P, true);
+  if (rc) return rc;
+  SF_HIP(hipStreamSynchronize(s));
+  const double t0 = now_s();
+  int iters = 0, updates = 0, nonfinite = 0;
+  double rel = 0.0;
+  rc = cg_reliable(P, inner_precision, b, x, rtol, maxit, delta, s, &iters,
+                   &updates, &nonfinite, &rel);
+  if (rc) return rc;
+  SF_HIP(hipStreamSynchronize(s));
+  const double t1 = now_s();
+  if (stats) {
+    stats->iterations = iters;
+    stats->outer_iterations = updates;
+    stats->rel_residual = rel;
+    stats->converged = (rel <= rtol) ? 1 : 0;
+    stats->seconds = t1 - t0;
+  }
+  if (nonfinite) return fail(SF_ERR_NONFINITE, "mixed CG met a non-finite value");
+  if (rel > rtol)
+    return fail(SF_ERR_NOT_CONVERGED, "mixed CG did not reach rtol %g in %d iterations (rel. residual %g)", rtol, maxit, rel);
   return SF_OK;
   SF_CATCH
 }

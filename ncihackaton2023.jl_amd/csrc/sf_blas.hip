@@ -182,6 +182,18 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// p = r + c p, q = 0: the direction update after a residual replacement,
+// with the coefficient computed on the host.
+__global__ void __launch_bounds__(256)
+    k_dir_coef(float* __restrict__ p, const float* __restrict__ r,
+               float* __restrict__ q, float c, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    p[i] = fma(c, p[i], r[i]);
+    q[i] = 0.f;
+  }
+}
+
 template <typename V>
 __global__ void __launch_bounds__(256)
     k_zero_list(V* __restrict__ y, const int32_t* __restrict__ idx,
@@ -289,6 +301,14 @@ hipError_t launch_cg_dir(V* p, const V* r, V* q, int64_t n, CGScalars* sc,
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_cg_dir<V>, dim3(stream_grid(n)), dim3(256), 0, s, p, r,
                      q, n, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_dir_coef(float* p, const float* r, float* q, float c,
+                           int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dir_coef, dim3(stream_grid(n)), dim3(256), 0, s, p, r,
+                     q, c, n);
   return hipGetLastError();
 }
 

@@ -137,6 +137,9 @@ hipError_t launch_residual(const double* b, const double* Ax, double* r,
 // r32 = r * inv_scale
 hipError_t launch_scale_to_f32(const double* r, double inv_scale, float* r32,
                                int64_t n, hipStream_t s);
+// p = r + c p, q = 0 (FP32)
+hipError_t launch_dir_coef(float* p, const float* r, float* q, float c,
+                           int64_t n, hipStream_t s);
 // x += scale * d32
 hipError_t launch_axpy_f32_to_f64(double* x, const float* d32, double scale,
                                   int64_t n, hipStream_t s);

@@ -4,7 +4,9 @@
 driver calls it (compute_solution, profiling/drivers.jl:85-95): unpreconditioned
 CG, x is the initial guess and is overwritten.  ``cg_ir_`` is the
 mixed-precision iterative refinement: FP64 outer residual, norms and dot
-products, reduced-precision (FP32-vector) inner CG.  All vectors are torch
+products, reduced-precision (FP32-vector) inner CG, restarted per outer
+step.  ``cg_mixed_`` keeps a single reduced-precision CG recurrence and
+replaces its residual by the FP64 one at reliable updates.  All vectors are torch
 CUDA float64 tensors; the iteration runs in libsfhip.so on torch's current
 stream.
 """
@@ -45,6 +47,24 @@ def cg_ir_(x, A, b, rtol=1e-10, inner="f32", inner_rtol=1e-3, inner_maxiter=None
     rc = A._L.sf_cg_ir(A.plan, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
                        float(rtol), int(max_outer), prec, float(inner_rtol), inner_maxiter,
                        ctypes.byref(st), _torch_stream(x))
+    if rc != _lib.SF_OK and (raise_on_fail or rc != _lib.SF_ERR_NOT_CONVERGED):
+        _lib.check(rc)
+    return st.as_dict()
+
+
+def cg_mixed_(x, A, b, rtol=1e-10, inner="f32", delta=0.1, maxiter=None,
+              raise_on_fail=False):
+    """Mixed-precision CG with reliable updates: one FP32 CG recurrence on
+    the reduced-precision operator, residual replaced by the FP64 b - A x
+    whenever it has fallen by `delta` (search direction kept)."""
+    _vec_ok(A, x)
+    _vec_ok(A, b)
+    st = _lib.SolveStats()
+    prec = _lib.PRECISIONS[inner] if isinstance(inner, str) else int(inner)
+    maxiter = 10 * A.n_free if maxiter is None else int(maxiter)
+    rc = A._L.sf_cg_mixed(A.plan, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                          float(rtol), maxiter, prec, float(delta), ctypes.byref(st),
+                          _torch_stream(x))
     if rc != _lib.SF_OK and (raise_on_fail or rc != _lib.SF_ERR_NOT_CONVERGED):
         _lib.check(rc)
     return st.as_dict()

@@ -218,3 +218,32 @@ def test_mixed_precision_refinement(pkg, oracle_mod, cuda, inner):
     assert np.linalg.norm(r) / np.linalg.norm(bh) < 2e-10
     assert rel_l2(xh, x64.cpu().numpy()) < 1e-7
     A.free()
+
+
+@pytest.mark.parametrize("inner", ["f32", "f32_gbf16", "mfma_bf16x3"])
+def test_mixed_cg_reliable_updates(pkg, oracle_mod, cuda, inner):
+    """One FP32 CG recurrence with FP64 residual replacement: reaches the
+    FP64 tolerance, matches the FP64 CG solution, and keeps the Krylov
+    space (iteration count close to FP64 CG's, far below a restarted
+    refinement's)."""
+    say("mixed CG inner=%s" % inner)
+    model, space, mp, A = make(pkg, 3, 6, 5, 7)
+    b = cuda.from_numpy(pkg.normal(2, space.n_free)).cuda()
+    x64 = cuda.zeros_like(b)
+    s64 = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=20000)
+    assert s64["converged"] == 1
+    x = cuda.zeros_like(b)
+    st = pkg.cg_mixed_(x, A, b, rtol=1e-10, inner=inner, delta=0.1)
+    say("f64", s64, "mixed", st)
+    assert st["converged"] == 1 and st["rel_residual"] <= 1e-10
+    assert st["outer_iterations"] >= 5  # 1e-10 in steps of at most 0.1
+    assert st["iterations"] < 1.5 * s64["iterations"]
+    xh = x.cpu().numpy()
+    bh = b.cpu().numpy()
+    r = bh - oracle_mod.apply(3, 6, 7, space.cell_dofs, model.cell_vertices(), xh, mp.tables)
+    assert np.linalg.norm(r) / np.linalg.norm(bh) < 2e-10
+    assert rel_l2(xh, x64.cpu().numpy()) < 1e-7
+    # a converged start returns at once
+    st2 = pkg.cg_mixed_(x, A, b, rtol=1e-9, inner=inner)
+    assert st2["iterations"] == 0 and st2["converged"] == 1
+    A.free()
