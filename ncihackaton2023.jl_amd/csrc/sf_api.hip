@@ -1181,19 +1181,24 @@ int sf_cg(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   SF_HIP(hipStreamSynchronize(s));
   const double t0 = now_s();
   int iters = 0, nonfinite = 0;
-  rc = cg_f64(P, b, x, rtol, maxit, s, &iters, &nonfinite);
-  if (rc) return rc;
-  // true residual
-  rc = do_apply_f64(P, P->w_t.p, x, s);
-  if (rc) return rc;
-  SF_HIP(launch_residual(b, P->w_t.p, P->w_r.p, P->nfree, s));
-  double rr = 0.0, bb = 0.0;
-  rc = dot_host(P, P->w_r.p, P->w_r.p, &rr, s);
-  if (rc) return rc;
+  double bb = 0.0, rel = 0.0;
   rc = dot_host(P, b, b, &bb, s);
   if (rc) return rc;
+  // The recurred residual drifts from the true one; when the recurrence
+  // stops short of rtol in the true residual, restart from x (the restart
+  // recomputes r = b - A x).
+  for (;;) {
+    int it = 0;
+    rc = cg_f64(P, b, x, rtol, maxit - iters, s, &it, &nonfinite);
+    if (rc) return rc;
+    iters += it;
+    double rn = 0.0;
+    rc = true_residual(P, b, x, &rn, s);
+    if (rc) return rc;
+    rel = bb > 0 ? rn / sqrt(bb) : rn;
+    if (nonfinite || !(rel > rtol) || iters >= maxit || it == 0) break;
+  }
   const double t1 = now_s();
-  const double rel = bb > 0 ? sqrt(rr / bb) : sqrt(rr);
   if (stats) {
     stats->iterations = iters;
     stats->outer_iterations = 0;

@@ -24,6 +24,13 @@
 //                                                 place by the z column pass)
 // QP = Q rounded up to an odd number, so the x-pencil rows written by
 // consecutive lanes (stride QP doubles) fall into distinct LDS banks.
+// From N = 6 on, A and B share one region: the y pass and the y^T pass
+// read their input pencils into registers, cross a barrier, then overwrite
+// them, so a workgroup holds max(B, A + x values) instead of A + B (more
+// cells per workgroup and more workgroups per CU -- the occupancy is what
+// hides the gather and G latencies at high order).  Below N = 6 the LDS is
+// not what limits the occupancy (the registers are) and the two extra
+// barriers would only cost.
 #pragma once
 
 #include "sf_common.hpp"
@@ -39,19 +46,26 @@ struct Apply3Shape {
   static constexpr int SB1 = N * QQ;   // one y-pass tensor [z][k][i]
   static constexpr int SE = 2 * SA1 + 3 * SB1;
   static constexpr int MAXT = cmax(NN, cmax(NQ, QQ));
+  // shared A / B region (N >= 6): LDS per cell is the y-pass output, or
+  // the x-pass output next to the cell's x values
+  static constexpr bool LC = N >= 6;
+  static constexpr int SC = LC ? cmax(3 * SB1, 2 * SA1 + NNN) : SE;
   // cells per workgroup: fill ~256 lanes in the widest stage, within 64 KiB
   // of FP64 LDS (the same EB for every T, so FP64 and FP32 applies share the
   // host-side block structure)
   static constexpr int EB_T = MAXT > 100 ? 1 : cmax(1, 256 / MAXT);
-  static constexpr int EB_L = cmax(1, 65536 / (SE * 8));
+  static constexpr int EB_L = cmax(1, 65536 / (SC * 8));
   static constexpr int EB = EB_T < EB_L ? EB_T : EB_L;
   static constexpr int NT = round_up(EB * MAXT, 64);
-  // LDS: region A = EB x (2 SA1), region B = EB x (3 SB1); region B also
-  // holds the block's gathered x values and its assembled y values (at most
-  // EB * N^3 distinct dofs)
+  // LC: one region of RC values -- the x-pass output (A, EB x 2 SA1) at its
+  // start with the block's gathered x values (at most EB * N^3 distinct
+  // dofs) behind it; later the y-pass output (B, EB x 3 SB1) over both;
+  // later the y^T-pass output over A and the assembled y values at the end.
+  // Otherwise A, then B; B also holds the x values and later the y values.
   static constexpr int RA = EB * 2 * SA1;
-  static constexpr int RB = cmax(EB * 3 * SB1, EB * NNN);
-  static constexpr int LDS_BYTES = (RA + RB) * (int)sizeof(T);
+  static constexpr int RX = EB * NNN;
+  static constexpr int RC = LC ? cmax(EB * 3 * SB1, RA + RX) : RA + cmax(EB * 3 * SB1, RX);
+  static constexpr int LDS_BYTES = RC * (int)sizeof(T);
   // distinct dofs of a block handled per thread in the gather / write-back
   static constexpr int KU = (EB * NNN + NT - 1) / NT;
   static_assert(LDS_BYTES <= 65536, "apply3d: LDS per workgroup above 64 KiB");
@@ -107,9 +121,11 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
               const TG* __restrict__ G, const T* __restrict__ tB,
               const T* __restrict__ tD, int accumulate) {
   using S = Apply3Shape<N, Q, T>;
-  __shared__ T smem[S::RA + S::RB];
-  T* const regA = smem;
-  T* const regB = smem + S::RA;
+  __shared__ T smem[S::RC];
+  T* const regA = smem;                               // x / y^T pass out
+  T* const regB = S::LC ? smem : smem + S::RA;        // y pass out, z in place
+  T* const regX = S::LC ? smem + S::RA : regB;        // the block's x values
+  T* const regY = S::LC ? smem + (S::RC - S::RX) : regB;  // its assembled y
   const int b = xcd_remap(blockIdx.x, nblk);
   const int64_t base = bcell[b];
   const int ncb = (int)(bcell[b + 1] - base);  // cells in this block, <= EB
@@ -178,7 +194,7 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
 #pragma unroll
   for (int k = 0; k < S::KU; ++k) {
     const int i = tid + k * S::NT;
-    if (i < nu) regB[i] = xv[k];
+    if (i < nu) regX[i] = xv[k];
   }
   __syncthreads();
 
@@ -187,7 +203,7 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   if (own1) {
     T u[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) u[j] = li[j] != kNoDof ? regB[li[j]] : T(0);
+    for (int j = 0; j < N; ++j) u[j] = li[j] != kNoDof ? regX[li[j]] : T(0);
     T* sA = regA + e1 * 2 * S::SA1;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -206,8 +222,10 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   __syncthreads();
 
   // ---- stage 2: y-pencils: (Dx u, Bx u) -> By Dx u, Dy Bx u, By Bx u ------
-  if (tid < S::EB * S::NQ) {
-    const int e = tid / S::NQ, r = tid - e * S::NQ;
+  // (LC: pencils into registers, barrier, outputs over the A region)
+  if (S::LC || tid < S::EB * S::NQ) {  // LC: every thread meets the barrier
+    const bool on2 = tid < S::EB * S::NQ;
+    const int e = on2 ? tid / S::NQ : 0, r = on2 ? tid - e * S::NQ : 0;
     const int z = r / Q, i = r - z * Q;
     const T* sA = regA + e * 2 * S::SA1;
     T* sB = regB + e * 3 * S::SB1;
@@ -217,6 +235,8 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
       a[j] = sA[(z * N + j) * S::QP + i];
       d[j] = sA[S::SA1 + (z * N + j) * S::QP + i];
     }
+    if constexpr (S::LC) __syncthreads();
+    if (on2)
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
       const auto b_ = table_row(tB);
@@ -280,9 +300,10 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   }
   __syncthreads();
 
-  // ---- stage 4: transposed y-pencils --------------------------------------
-  if (tid < S::EB * S::NQ) {
-    const int e = tid / S::NQ, r = tid - e * S::NQ;
+  // ---- stage 4: transposed y-pencils (LC: into registers, barrier, over B)
+  if (S::LC || tid < S::EB * S::NQ) {
+    const bool on4 = tid < S::EB * S::NQ;
+    const int e = on4 ? tid / S::NQ : 0, r = on4 ? tid - e * S::NQ : 0;
     const int z = r / Q, i = r - z * Q;
     T* sA = regA + e * 2 * S::SA1;
     const T* sB = regB + e * 3 * S::SB1;
@@ -294,6 +315,8 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
       ry[k] = sB[S::SB1 + o];
       rz[k] = sB[2 * S::SB1 + o];
     }
+    if constexpr (S::LC) __syncthreads();
+    if (on4)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       const auto b_ = table_row(tB);
@@ -311,11 +334,12 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   }
   __syncthreads();
 
-  // region B is free again: it becomes the block's y accumulator
+  // region B (LC: the end of the region, behind the y^T-pass output)
+  // accumulates y
 #pragma unroll
   for (int k = 0; k < S::KU; ++k) {
     const int i = tid + k * S::NT;
-    if (i < nu) regB[i] = T(0);
+    if (i < nu) regY[i] = T(0);
   }
   __syncthreads();
 
@@ -338,7 +362,7 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
         v = fma(d_[i * N + m], sx[i], v);
         v = fma(b_[i * N + m], so[i], v);
       }
-      if (li[m] != kNoDof) lds_atomic_add(regB + li[m], v);
+      if (li[m] != kNoDof) lds_atomic_add(regY + li[m], v);
     }
   }
   __syncthreads();
@@ -348,7 +372,7 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   for (int k = 0; k < S::KU; ++k) {
     const int i = tid + k * S::NT;
     if (dv[k] < 0) continue;
-    const T v = regB[i];
+    const T v = regY[i];
     if (i < nexcl) {
       if (accumulate)
         y[dv[k]] += v;
