@@ -137,8 +137,78 @@ def mode_apply(args, torch, pkg):
                                "sample": "full 64^3 Q4 q=6 apply (the same workload), "
                                          "mean of %d after 1 warm-up, OpenMP C oracle" % reps}
         out["parity_rel_l2"] = float(np.linalg.norm(yd - y_ref) / np.linalg.norm(y_ref))
+        A.free()
+        out["assembled_comparator"] = assembled_comparator(torch, pkg, threads)
+        return out
     A.free()
     return out
+
+
+def assembled_comparator(torch, pkg, threads):
+    """Matrix-free vs assembled at a reduced 3D size (full-size CSR would
+    not fit): the oracle's brute-force element matrices assembled into CSR,
+    SpMV on the host (scipy, one thread) and on the GPU (torch CSR), the
+    OpenMP oracle on 1 and on `threads` cores, and the GPU matrix-free
+    apply, all on the same seeded vector."""
+    import numpy as np
+    import oracle
+    from oracle import brute
+    D, m, p, q = 3, 16, 4, 6
+    n = p + 1
+    log("assembled comparator: 3D %d^3 Q%d q=%d ..." % (m, p, q))
+    model, space, mp, A = build_problem(pkg, D, m, p, q)
+    X = model.cell_vertices()
+    B, Dm, pts, w = brute.tables(p, q)
+    G, _ = brute.geometry_numpy(D, X, pts, w)
+    Acsr = brute.assemble_csr(space.cell_dofs, brute.element_matrices(D, B, Dm, G), space.n_free)
+    xh = pkg.uniform(0, space.n_free, -1.0, 1.0)
+
+    def best(f, reps):
+        f()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    y_sp = Acsr @ xh
+    t_cpu_csr = best(lambda: Acsr @ xh, 5)
+    t_or1 = best(lambda: oracle.apply(D, n, q, space.cell_dofs, X, xh, mp.tables, nthreads=1), 2)
+    t_orN = best(lambda: oracle.apply(D, n, q, space.cell_dofs, X, xh, mp.tables,
+                                      nthreads=threads), 3)
+    x = torch.from_numpy(xh).cuda()
+    y = torch.empty_like(x)
+    wall, kern = time_apply(torch, A, x, y, 50, 5)
+    err_mf = float(np.linalg.norm(y.cpu().numpy() - y_sp) / np.linalg.norm(y_sp))
+    Ag = torch.sparse_csr_tensor(torch.from_numpy(Acsr.indptr.astype(np.int64)),
+                                 torch.from_numpy(Acsr.indices.astype(np.int64)),
+                                 torch.from_numpy(Acsr.data), size=Acsr.shape).cuda()
+    yg = Ag @ x
+    torch.cuda.synchronize()
+    err_csr = float(np.linalg.norm(yg.cpu().numpy() - y_sp) / np.linalg.norm(y_sp))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(5):
+        yg = Ag @ x
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(50):
+        yg = Ag @ x
+    e1.record()
+    torch.cuda.synchronize()
+    t_gpu_csr = e0.elapsed_time(e1) / 50 / 1e3
+    nf = space.n_free
+    res = {"workload": "3D %d^3 Q%d q=%d perturbed, FP64 (full-size CSR would not fit)" % (m, p, q),
+           "n_free": nf, "csr_nnz": int(Acsr.nnz),
+           "gpu_matrix_free_gdofs": nf / wall / 1e9,
+           "gpu_csr_gdofs": nf / t_gpu_csr / 1e9,
+           "cpu_csr_1core_gdofs": nf / t_cpu_csr / 1e9,
+           "cpu_oracle_1core_gdofs": nf / t_or1 / 1e9,
+           "cpu_oracle_gdofs": nf / t_orN / 1e9, "cpu_oracle_cores": threads,
+           "matrix_free_vs_csr_rel_l2": err_mf, "gpu_csr_vs_cpu_csr_rel_l2": err_csr}
+    log("assembled comparator", res)
+    A.free()
+    return res
 
 
 def mode_sweep(args, torch, pkg):
@@ -258,12 +328,27 @@ def mode_adjoint(args, torch, pkg):
             "n_free": space.n_free, "n_gpus": 1}
 
 
+def mode_copycal(args, torch, pkg):
+    """A plain device copy of 1 GiB, three times: the reference dispatch that
+    calibrates the FETCH_SIZE / WRITE_SIZE counters (bytes moved are known)."""
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"metric": "device copy of 1 GiB (counter calibration)", "bytes_read": n * 8,
+            "bytes_written": n * 8, "n_gpus": 1}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--mode", default="apply", choices=["apply", "sweep", "cg", "adjoint"])
+    ap.add_argument("--mode", default="apply",
+                    choices=["apply", "sweep", "cg", "adjoint", "copycal"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--maxit", type=int, default=300000)
     ap.add_argument("--max-outer", type=int, default=50)
@@ -283,7 +368,8 @@ def main():
     import __graft_entry__
     pkg = __graft_entry__.import_package()
     torch.cuda.set_device(0)
-    fn = {"apply": mode_apply, "sweep": mode_sweep, "cg": mode_cg, "adjoint": mode_adjoint}
+    fn = {"apply": mode_apply, "sweep": mode_sweep, "cg": mode_cg, "adjoint": mode_adjoint,
+          "copycal": mode_copycal}
     out = fn[args.mode](args, torch, pkg)
     print(json.dumps(out), flush=True)
 

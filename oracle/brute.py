@@ -99,7 +99,18 @@ def geometry_numpy(dim, verts, pts, w):
 def element_matrices(dim, B, Dm, G):
     """K[cell] = sum_t Grad_t^T G_t Grad_t (dense, no factorisation)."""
     Gr = np.stack(grad_matrices(dim, B, Dm))  # (D, nq, nl)
-    return np.einsum("rta,ctrk,ktb->cab", Gr, G, Gr, optimize=True)
+    G = np.asarray(G)
+    C, nl = G.shape[0], Gr.shape[2]
+    K = np.zeros((C, nl, nl))
+    # K[c] += Grad_r^T diag(G[c, :, r, k]) Grad_k, as batched matrix products
+    # in chunks of cells (a plain einsum may pick a slow contraction order)
+    for c0 in range(0, C, 512):
+        c1 = min(C, c0 + 512)
+        for r in range(dim):
+            for k in range(dim):
+                tmp = G[c0:c1, :, r, k][:, :, None] * Gr[k][None, :, :]
+                K[c0:c1] += np.matmul(Gr[r].T[None, :, :], tmp)
+    return K
 
 
 def apply_dense(cells, K, x):
