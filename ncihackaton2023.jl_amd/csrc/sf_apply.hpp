@@ -55,6 +55,10 @@ struct Apply3Shape {
   // host-side block structure)
   static constexpr int EB_T = MAXT > 100 ? 1 : cmax(1, 256 / MAXT);
   static constexpr int EB_L = cmax(1, 65536 / (SC * 8));
+  // table rows per scalar-load group (measured: 3 at N = 5, 2 at N = 6..8;
+  // one row per load is faster below N = 5, and larger groups at large N, Q
+  // spill SGPRs into VGPR lanes)
+  static constexpr int TK = N <= 4 ? 1 : (N == 5 ? 3 : (N <= 8 && Q <= N + 2 ? 2 : 1));
   static constexpr int EB = EB_T < EB_L ? EB_T : EB_L;
   static constexpr int NT = round_up(EB * MAXT, 64);
   // LC: one region of RC values -- the x-pass output (A, EB x 2 SA1) at its
@@ -205,15 +209,15 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
 #pragma unroll
     for (int j = 0; j < N; ++j) u[j] = li[j] != kNoDof ? regX[li[j]] : T(0);
     T* sA = regA + e1 * 2 * S::SA1;
+    TableRows<T, S::TK> t_(tB, tD);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      const auto b_ = table_row(tB);
-      const auto d_ = table_row(tD);
+      t_.at(i);
       T bq = T(0), dq = T(0);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        bq = fma(b_[i * N + j], u[j], bq);
-        dq = fma(d_[i * N + j], u[j], dq);
+        bq = fma(t_.b[i * N + j], u[j], bq);
+        dq = fma(t_.d[i * N + j], u[j], dq);
       }
       sA[r1 * S::QP + i] = bq;
       sA[S::SA1 + r1 * S::QP + i] = dq;
@@ -236,17 +240,17 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
       d[j] = sA[S::SA1 + (z * N + j) * S::QP + i];
     }
     if constexpr (S::LC) __syncthreads();
+    TableRows<T, S::TK> t_(tB, tD);
     if (on2)
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-      const auto b_ = table_row(tB);
-      const auto d_ = table_row(tD);
+      t_.at(k);
       T bb = T(0), bd = T(0), db = T(0);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        bb = fma(b_[k * N + j], a[j], bb);
-        bd = fma(d_[k * N + j], a[j], bd);
-        db = fma(b_[k * N + j], d[j], db);
+        bb = fma(t_.b[k * N + j], a[j], bb);
+        bd = fma(t_.d[k * N + j], a[j], bd);
+        db = fma(t_.b[k * N + j], d[j], db);
       }
       const int o = z * S::QQ + k * Q + i;
       sB[o] = db;               // x-gradient path
@@ -267,16 +271,16 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
       cz[z] = sB[2 * S::SB1 + z * S::QQ + r3];
     }
     T tx[Q], ty[Q], tz[Q];
+    TableRows<T, S::TK> t_(tB, tD);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
-      const auto b_ = table_row(tB);
-      const auto d_ = table_row(tD);
+      t_.at(l);
       T gx = T(0), gy = T(0), gz = T(0);
 #pragma unroll
       for (int z = 0; z < N; ++z) {
-        gx = fma(b_[l * N + z], cx[z], gx);
-        gy = fma(b_[l * N + z], cy[z], gy);
-        gz = fma(d_[l * N + z], cz[z], gz);
+        gx = fma(t_.b[l * N + z], cx[z], gx);
+        gy = fma(t_.b[l * N + z], cy[z], gy);
+        gz = fma(t_.d[l * N + z], cz[z], gz);
       }
       tx[l] = gq[0][l] * gx + gq[1][l] * gy + gq[2][l] * gz;
       ty[l] = gq[1][l] * gx + gq[3][l] * gy + gq[4][l] * gz;
@@ -284,14 +288,13 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     }
 #pragma unroll
     for (int z = 0; z < N; ++z) {
-      const auto b_ = table_row(tB);
-      const auto d_ = table_row(tD);
+      t_.at(z);
       T rx = T(0), ry = T(0), rz = T(0);
 #pragma unroll
       for (int l = 0; l < Q; ++l) {
-        rx = fma(b_[l * N + z], tx[l], rx);
-        ry = fma(b_[l * N + z], ty[l], ry);
-        rz = fma(d_[l * N + z], tz[l], rz);
+        rx = fma(t_.b[l * N + z], tx[l], rx);
+        ry = fma(t_.b[l * N + z], ty[l], ry);
+        rz = fma(t_.d[l * N + z], tz[l], rz);
       }
       sB[z * S::QQ + r3] = rx;
       sB[S::SB1 + z * S::QQ + r3] = ry;
@@ -316,17 +319,17 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
       rz[k] = sB[2 * S::SB1 + o];
     }
     if constexpr (S::LC) __syncthreads();
+    TableRows<T, S::TK> t_(tB, tD);
     if (on4)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      const auto b_ = table_row(tB);
-      const auto d_ = table_row(tD);
+      t_.at(j);
       T sx = T(0), so = T(0);
 #pragma unroll
       for (int k = 0; k < Q; ++k) {
-        sx = fma(b_[k * N + j], rx[k], sx);
-        so = fma(d_[k * N + j], ry[k], so);
-        so = fma(b_[k * N + j], rz[k], so);
+        sx = fma(t_.b[k * N + j], rx[k], sx);
+        so = fma(t_.d[k * N + j], ry[k], so);
+        so = fma(t_.b[k * N + j], rz[k], so);
       }
       sA[(z * N + j) * S::QP + i] = sx;           // then D_x^T
       sA[S::SA1 + (z * N + j) * S::QP + i] = so;  // then B_x^T
@@ -352,15 +355,15 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
       sx[i] = sA[r1 * S::QP + i];
       so[i] = sA[S::SA1 + r1 * S::QP + i];
     }
+    TableRows<T, S::TK> t_(tB, tD);
 #pragma unroll
     for (int m = 0; m < N; ++m) {
-      const auto b_ = table_row(tB);
-      const auto d_ = table_row(tD);
+      t_.at(m);
       T v = T(0);
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
-        v = fma(d_[i * N + m], sx[i], v);
-        v = fma(b_[i * N + m], so[i], v);
+        v = fma(t_.d[i * N + m], sx[i], v);
+        v = fma(t_.b[i * N + m], so[i], v);
       }
       if (li[m] != kNoDof) lds_atomic_add(regY + li[m], v);
     }
@@ -396,6 +399,7 @@ struct Apply2Shape {
   static constexpr int MAXT = cmax(N, Q);
   static constexpr int EB = cmax(1, 256 / MAXT);
   static constexpr int NT = round_up(EB * MAXT, 64);
+  static constexpr int TK = 1;  // table rows per scalar-load group (G loads share the SGPRs)
 };
 
 template <int N, int Q, typename T, typename TG>
@@ -433,15 +437,15 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
       for (int j = 0; j < N; ++j) u[j] = T(0);
     }
     T* sA = smem + e1 * S::SE;
+    TableRows<T, S::TK> t_(tB, tD);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      const auto b_ = table_row(tB);
-      const auto d_ = table_row(tD);
+      t_.at(i);
       T b = T(0), d = T(0);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        b = fma(b_[i * N + j], u[j], b);
-        d = fma(d_[i * N + j], u[j], d);
+        b = fma(t_.b[i * N + j], u[j], b);
+        d = fma(t_.d[i * N + j], u[j], d);
       }
       sA[r1 * S::QP + i] = b;
       sA[S::SA1 + r1 * S::QP + i] = d;
@@ -464,15 +468,15 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
     const bool valid = ge < ncell;
     const TG* g = G + ge * 3 * S::QQ + i;
     T tx[Q], ty[Q];
+    TableRows<T, S::TK> t_(tB, tD);
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-      const auto b_ = table_row(tB);
-      const auto d_ = table_row(tD);
+      t_.at(k);
       T gx = T(0), gy = T(0);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        gx = fma(b_[k * N + j], d[j], gx);
-        gy = fma(d_[k * N + j], a[j], gy);
+        gx = fma(t_.b[k * N + j], d[j], gx);
+        gy = fma(t_.d[k * N + j], a[j], gy);
       }
       T g0 = T(0), g1 = T(0), g2 = T(0);
       if (valid) {
@@ -485,13 +489,12 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      const auto b_ = table_row(tB);
-      const auto d_ = table_row(tD);
+      t_.at(j);
       T rx = T(0), ry = T(0);
 #pragma unroll
       for (int k = 0; k < Q; ++k) {
-        rx = fma(b_[k * N + j], tx[k], rx);
-        ry = fma(d_[k * N + j], ty[k], ry);
+        rx = fma(t_.b[k * N + j], tx[k], rx);
+        ry = fma(t_.d[k * N + j], ty[k], ry);
       }
       sA[j * S::QP + i] = rx;            // then D_x^T
       sA[S::SA1 + j * S::QP + i] = ry;   // then B_x^T
@@ -508,15 +511,15 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
       sx[i] = sA[r1 * S::QP + i];
       so[i] = sA[S::SA1 + r1 * S::QP + i];
     }
+    TableRows<T, S::TK> t_(tB, tD);
 #pragma unroll
     for (int m = 0; m < N; ++m) {
-      const auto b_ = table_row(tB);
-      const auto d_ = table_row(tD);
+      t_.at(m);
       T v = T(0);
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
-        v = fma(d_[i * N + m], sx[i], v);
-        v = fma(b_[i * N + m], so[i], v);
+        v = fma(t_.d[i * N + m], sx[i], v);
+        v = fma(t_.b[i * N + m], so[i], v);
       }
       if (ids[m] >= 0) atomic_add(y + ids[m], v);
     }

@@ -59,6 +59,25 @@ __device__ __forceinline__ cptr<P> table_row(const P* p) {
   return c;
 }
 
+// Rows of the B and D tables for an unrolled contraction, re-fetched every
+// TK rows: at(r) hands out a fresh opaque pointer pair when r % TK == 0, so
+// the scalar loads of TK rows are issued together and waited for once
+// (one wait per row leaves the wave stalled on the scalar cache between
+// short bursts of FMAs; all rows at once overflow the SGPRs).
+template <typename T, int TK>
+struct TableRows {
+  const T* tb;
+  const T* td;
+  cptr<T> b, d;
+  __device__ __forceinline__ TableRows(const T* tb_, const T* td_) : tb(tb_), td(td_) {}
+  __device__ __forceinline__ void at(int r) {
+    if (r % TK == 0) {
+      b = table_row(tb);
+      d = table_row(td);
+    }
+  }
+};
+
 // bf16 storage for the geometric factor (round-to-nearest-even on store).
 struct bf16s {
   uint16_t bits;
