@@ -3,6 +3,7 @@
 // the other translation units.
 #include <math.h>
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -549,6 +550,28 @@ int dot_host(sf_plan* P, const double* a, const double* b, double* out,
   return SF_OK;
 }
 
+double now_s();
+
+// Solver progress on stderr when SFHIP_PROGRESS=<seconds> is set: one line
+// per interval from the host convergence checks (iterations, residual).
+struct Progress {
+  double every = -1.0, last = 0.0;
+  const char* what;
+  explicit Progress(const char* w) : what(w) {
+    const char* e = getenv("SFHIP_PROGRESS");
+    if (e && *e) every = atof(e);
+    last = now_s();
+  }
+  void tick(int iters, double rel) {
+    if (every < 0.0) return;
+    const double t = now_s();
+    if (t - last < every) return;
+    last = t;
+    fprintf(stderr, "sfhip %s: %d iterations, rel. residual %.3e\n", what, iters, rel);
+    fflush(stderr);
+  }
+};
+
 // Plain CG on the FP64 operator, device vectors.  Chunks of `chunk`
 // iterations are queued between host checks of the device scalars; once
 // converged the remaining queued updates are no-ops.
@@ -566,8 +589,10 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   CGScalars h{};
   rc = read_scalars(P, &h, s);
   if (rc) return rc;
+  Progress prog("cg");
   const int chunk = 8;
   while (!h.done && h.iters < maxit) {
+    prog.tick(h.iters, bb > 0 ? sqrt(h.rr / bb) : sqrt(h.rr));
     const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
     for (int k = 0; k < todo; ++k) {
       rc = do_apply_f64(P, P->w_q.p, P->w_p.p, s, 0, true);
@@ -601,8 +626,10 @@ int cg_f32_inner(sf_plan* P, int precision, double inner_rtol, int maxit,
   CGScalars h{};
   int rc = read_scalars(P, &h, s);
   if (rc) return rc;
+  Progress prog("refinement inner cg");
   const int chunk = 8;
   while (!h.done && h.iters < maxit) {
+    prog.tick(h.iters, sqrt(h.rr));
     const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
     for (int k = 0; k < todo; ++k) {
       rc = do_apply_f32(P, q, p, precision, s, true);
@@ -691,8 +718,10 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
   rc = read_scalars(P, &h, s);
   if (rc) return rc;
   const int chunk = 8;
+  Progress prog("mixed cg");
   for (;;) {
     while (!h.done && h.iters < maxit) {
+      prog.tick(h.iters, (bnorm > 0 ? rn / bnorm : rn) * sqrt(h.rr));
       const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
       for (int k = 0; k < todo; ++k) {
         rc = do_apply_f32(P, q, p, precision, s, true);
