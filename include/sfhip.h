@@ -54,8 +54,10 @@ enum sf_precision {
   SF_PREC_F32_GBF16 = 2, /* as F32, but G stored in bfloat16 (half the G bytes)   */
   SF_PREC_MFMA_BF16 = 3, /* 3D: contractions as bf16 MFMA GEMMs (operands rounded
                             to bf16), FP32 accumulation, G in bf16               */
-  SF_PREC_MFMA_BF16X3 = 4 /* 3D: as MFMA_BF16 with every operand split into three
+  SF_PREC_MFMA_BF16X3 = 4, /* 3D: as MFMA_BF16 with every operand split into three
                             bf16 terms (six MFMAs per tile): FP32-level accuracy */
+  SF_PREC_MFMA_BF16X2 = 5  /* 3D: two bf16 terms per operand (three MFMAs per
+                            tile): about 16 significant bits                     */
 };
 
 typedef struct sf_plan sf_plan; /* opaque */

@@ -32,9 +32,11 @@ SF_PREC_F32 = 1
 SF_PREC_F32_GBF16 = 2
 SF_PREC_MFMA_BF16 = 3
 SF_PREC_MFMA_BF16X3 = 4
+SF_PREC_MFMA_BF16X2 = 5
 
 PRECISIONS = {"f64": SF_PREC_F64, "f32": SF_PREC_F32, "f32_gbf16": SF_PREC_F32_GBF16,
-              "mfma_bf16": SF_PREC_MFMA_BF16, "mfma_bf16x3": SF_PREC_MFMA_BF16X3}
+              "mfma_bf16": SF_PREC_MFMA_BF16, "mfma_bf16x2": SF_PREC_MFMA_BF16X2,
+              "mfma_bf16x3": SF_PREC_MFMA_BF16X3}
 
 
 class SFError(RuntimeError):

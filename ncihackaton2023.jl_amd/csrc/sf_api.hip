@@ -323,13 +323,15 @@ int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
 
 bool is_reduced(int precision) {
   return precision == SF_PREC_F32 || precision == SF_PREC_F32_GBF16 ||
-         precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X3;
+         precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X2 ||
+         precision == SF_PREC_MFMA_BF16X3;
 }
 
 int prepare(sf_plan* P, int precision) {
   const int64_t ng = P->ncell * P->ng * P->nqc;
   if (precision == SF_PREC_F64) return SF_OK;
-  if (precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X3) {
+  if (precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X2 ||
+      precision == SF_PREC_MFMA_BF16X3) {
     if (P->dim != 3)
       return fail(SF_ERR_UNSUPPORTED, "the MFMA apply is 3D only (dim=%d)", P->dim);
     if (!P->gm_valid) {
@@ -383,8 +385,10 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
   } else if (precision == SF_PREC_F32_GBF16) {
     e = apply_f32_gbf16(P->dim, P->n, P->q, x, y, P->cells.p, P->Gbf.p,
                         P->dB32.p, P->dD32.p, P->ncell, P->blocks(), 0, s);
-  } else if (precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X3) {
-    e = apply_mfma(P->n, P->q, precision == SF_PREC_MFMA_BF16 ? 1 : 3, x, y,
+  } else if (precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X2 ||
+             precision == SF_PREC_MFMA_BF16X3) {
+    const int split = precision == SF_PREC_MFMA_BF16 ? 1 : (precision == SF_PREC_MFMA_BF16X2 ? 2 : 3);
+    e = apply_mfma(P->n, P->q, split, x, y,
                    P->Gm.p, P->dB32.p, P->dD32.p, P->blocks(), s);
   } else {
     return fail(SF_ERR_UNSUPPORTED, "precision %d has no FP32 apply", precision);

@@ -20,7 +20,8 @@
 // bits).  SPLIT = 3 writes each FP32 operand as the sum of three bf16
 // terms (hi + mid + lo, each the rounded remainder of the previous) and
 // keeps the six partial products of order >= 2^-24, so the contraction is
-// as accurate as FP32 arithmetic at six MFMAs per tile.
+// as accurate as FP32 arithmetic at six MFMAs per tile.  SPLIT = 2 (hi +
+// lo, three products) carries about 16 bits.
 //
 // MFMA 16x16x16 bf16 lane layout (wave64, lane l, g = l / 16, j = 0..3):
 //   A[m][k]: m = l % 16, k = 4 g + j      B[k][n]: n = l % 16, k = 4 g + j
@@ -450,12 +451,13 @@ hipError_t apply_mfma(int n, int q, int split, const float* x, float* y,
                       const uint32_t* Gm, const float* tB, const float* tD,
                       const BlockArgs* k, hipStream_t s) {
   if (!k) return hipErrorInvalidValue;
-  if (split != 1 && split != 3) return hipErrorNotSupported;
+  if (split < 1 || split > 3) return hipErrorNotSupported;
   switch (n * 16 + q) {
-#define SF_CASE(N_, Q_)                                                  \
-  case N_ * 16 + Q_:                                                     \
-    return split == 1 ? launch_mfma_nq<N_, Q_, 1>(x, y, Gm, tB, tD, *k, s) \
-                      : launch_mfma_nq<N_, Q_, 3>(x, y, Gm, tB, tD, *k, s);
+#define SF_CASE(N_, Q_)                                                      \
+  case N_ * 16 + Q_:                                                         \
+    return split == 1   ? launch_mfma_nq<N_, Q_, 1>(x, y, Gm, tB, tD, *k, s) \
+           : split == 2 ? launch_mfma_nq<N_, Q_, 2>(x, y, Gm, tB, tD, *k, s) \
+                        : launch_mfma_nq<N_, Q_, 3>(x, y, Gm, tB, tD, *k, s);
     SF_FOR_EACH_NQ(SF_CASE)
 #undef SF_CASE
     default:
