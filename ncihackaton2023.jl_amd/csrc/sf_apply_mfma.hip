@@ -36,12 +36,25 @@ namespace {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef unsigned u2 __attribute__((ext_vector_type(2)));
 
 template <int SPLIT>
 struct Op {
   s4 t[SPLIT];
 };
+
+// Two FP32 values -> one packed bf16 pair (v_cvt_pk_bf16_f32, round to
+// nearest).  The empty asm keeps the pair opaque, so the remainders below
+// reuse it instead of converting each value again.
+__device__ __forceinline__ unsigned pk_bf16(float a, float b) {
+  bf2 h;
+  h.x = (__bf16)a;
+  h.y = (__bf16)b;
+  unsigned u = __builtin_bit_cast(unsigned, h);
+  asm volatile("" : "+v"(u));
+  return u;
+}
 
 // FP32 -> SPLIT bf16 terms (round to nearest; each remainder is exact).
 template <int SPLIT>
@@ -49,11 +62,32 @@ __device__ __forceinline__ Op<SPLIT> split4(f4 v) {
   Op<SPLIT> o;
 #pragma unroll
   for (int s = 0; s < SPLIT; ++s) {
-    const bf4 h = __builtin_convertvector(v, bf4);
-    o.t[s] = __builtin_bit_cast(s4, h);
-    if (s + 1 < SPLIT) v = v - __builtin_convertvector(h, f4);
+    const unsigned h0 = pk_bf16(v.x, v.y), h1 = pk_bf16(v.z, v.w);
+    const u2 hh = {h0, h1};
+    o.t[s] = __builtin_bit_cast(s4, hh);
+    if (s + 1 < SPLIT) {
+      v.x -= __uint_as_float(h0 << 16);
+      v.y -= __uint_as_float(h0 & 0xffff0000u);
+      v.z -= __uint_as_float(h1 << 16);
+      v.w -= __uint_as_float(h1 & 0xffff0000u);
+    }
   }
   return o;
+}
+
+// Branch-free masked accesses.  A read is always issued (an out-of-range
+// slot reads element 0 of the same region) and its value replaced by zero,
+// so the reads of a tile go out back to back and share one wait; a masked
+// write goes to the lane's own dummy word.  (A conditional access compiles
+// to a branch per element, each read then waiting for its own result.)
+__device__ __forceinline__ float ld_or0(const float* base, int idx, bool ok) {
+  const float v = base[ok ? idx : 0];
+  return ok ? v : 0.f;
+}
+__device__ __forceinline__ void st_or_dummy(float* base, int idx, bool ok, float v,
+                                            float* dummy) {
+  float* p = ok ? base + idx : dummy;
+  *p = v;
 }
 
 __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) {
@@ -78,19 +112,20 @@ __device__ __forceinline__ f4 mma(const Op<SPLIT>& a, const Op<SPLIT>& b, f4 c) 
 template <int N, int Q, int SPLIT>
 struct MfmaShape {
   static constexpr int EB = Apply3Shape<N, Q, double>::EB;  // block structure
-  static constexpr int NT = 256, W = NT / 64;
   static constexpr int NN = N * N, NQ = N * Q, QQ = Q * Q;
   static constexpr int NNN = N * N * N, QQQ = Q * Q * Q;
   static constexpr int TX = (NN + 15) / 16, TY = (NQ + 15) / 16, TZ = (QQ + 15) / 16;
+  static constexpr int W = 4;  // waves per workgroup (more measured slower)
+  static constexpr int NT = 64 * W;
   static constexpr int UX = (EB * TX + W - 1) / W;  // tiles per wave, X and X^T
   static constexpr int UY = (EB * TY + W - 1) / W;
   static constexpr int UZ = (EB * TZ + W - 1) / W;
   static constexpr int S1 = N * Q * N;  // one X-pass (or Y^T-pass) tensor
   static constexpr int S2 = Q * Q * N;  // one Y-pass tensor
   static constexpr int RX = EB * NNN;   // the block's x values, later its y
-  static constexpr int LDS_FLOATS = RX + EB * (2 * S1 + 3 * S2);
+  static constexpr int LDS_FLOATS = RX + EB * (2 * S1 + 3 * S2) + 64;  // + dummies
   static constexpr int KU = (EB * NNN + NT - 1) / NT;
-  static constexpr bool GPRE = UZ <= 4;  // G of every Z tile loaded up front
+  static constexpr bool GPRE = UZ <= 4;  // G of every z tile loaded up front
   static_assert(N <= 16 && Q <= 16, "one MFMA tile per contraction");
   static_assert(LDS_FLOATS * 4 <= 65536, "apply3d_mfma: LDS above 64 KiB");
 };
@@ -103,19 +138,21 @@ __device__ __forceinline__ void load_gz(const uint32_t* __restrict__ Gm,
                                         int64_t cell, int col, int g, bool ok,
                                         uint32_t gv[12]) {
   constexpr int QQ = Q * Q;
-  const uint32_t* p = Gm + cell * 3 * QQ * Q;
+  const uint32_t* p = Gm + (ok ? cell * 3 * QQ * Q : 0);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int i3 = 4 * g + j;
     const bool v = ok && i3 < Q && col < QQ;
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      gv[j * 3 + k] = v ? __builtin_nontemporal_load(p + (i3 * 3 + k) * QQ + col) : 0u;
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t w = __builtin_nontemporal_load(p + (v ? (i3 * 3 + k) * QQ + col : 0));
+      gv[j * 3 + k] = v ? w : 0u;
+    }
   }
 }
 
 template <int N, int Q, int SPLIT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__((MfmaShape<N, Q, SPLIT>::NT))
     k_apply3d_mfma(const float* __restrict__ x, float* __restrict__ y,
                    const uint16_t* __restrict__ pidx,
                    const int32_t* __restrict__ bdofs,
@@ -131,6 +168,7 @@ __global__ void __launch_bounds__(256)
   float* const bx = smem;
   float* const T1 = smem + M::RX;
   float* const T2 = T1 + M::EB * 2 * S1;
+  float* const dummy = smem + (M::LDS_FLOATS - 64) + (threadIdx.x & 63);
 
   const int b = xcd_remap(blockIdx.x, nblk);
   const int64_t base = bcell[b];
@@ -166,7 +204,9 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int a = 4 * g + j;
-      pv[i][j] = (ok && a < N) ? pidx[(base + e) * NNN + a + N * col] : kNoDof;
+      const bool v = ok && a < N;
+      const uint16_t l = pidx[v ? (base + e) * NNN + a + N * col : 0];
+      pv[i][j] = v ? l : kNoDof;
     }
   }
 
@@ -189,10 +229,10 @@ __global__ void __launch_bounds__(256)
     for (int j = 0; j < 4; ++j) {
       const int k = 4 * g + j;
       const bool f = cl < Q && k < N, r = cl < N && k < Q;
-      vb[j] = f ? tB[cl * N + k] : 0.f;
-      vd[j] = f ? tD[cl * N + k] : 0.f;
-      vbt[j] = r ? tB[k * N + cl] : 0.f;
-      vdt[j] = r ? tD[k * N + cl] : 0.f;
+      vb[j] = ld_or0(tB, cl * N + k, f);
+      vd[j] = ld_or0(tD, cl * N + k, f);
+      vbt[j] = ld_or0(tB, k * N + cl, r);
+      vdt[j] = ld_or0(tD, k * N + cl, r);
     }
     aB = split4<SPLIT>(vb);
     aD = split4<SPLIT>(vd);
@@ -219,20 +259,18 @@ __global__ void __launch_bounds__(256)
     const int col = (u - e * M::TX) * 16 + cl;
     f4 v;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) v[j] = pv[i][j] != kNoDof ? bx[pv[i][j]] : 0.f;
+    for (int j = 0; j < 4; ++j) v[j] = ld_or0(bx, pv[i][j], pv[i][j] != kNoDof);
     const Op<SPLIT> o = split4<SPLIT>(v);
     const f4 cb = mma(aB, o, zero), cd = mma(aD, o, zero);
-    if (col < NN) {
-      const int bb = col % N, c = col / N;
-      float* t = T1 + e * 2 * S1 + bb * NQ + Q * c;
+    const int bb = col % N, c = col / N;
+    float* t = T1 + e * 2 * S1;
+    const int o0 = bb * NQ + Q * c;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int i1 = 4 * g + j;
-        if (i1 < Q) {
-          t[i1] = cb[j];
-          t[S1 + i1] = cd[j];
-        }
-      }
+    for (int j = 0; j < 4; ++j) {
+      const int i1 = 4 * g + j;
+      const bool ok = col < NN && i1 < Q;
+      st_or_dummy(t, o0 + i1, ok, cb[j], dummy);
+      st_or_dummy(t, S1 + o0 + i1, ok, cd[j], dummy);
     }
   }
   __syncthreads();
@@ -258,23 +296,21 @@ __global__ void __launch_bounds__(256)
     for (int j = 0; j < 4; ++j) {
       const int bb = 4 * g + j;
       const bool ok = bb < N && col < NQ;
-      vx[j] = ok ? t[bb * NQ + col] : 0.f;
-      vd[j] = ok ? t[S1 + bb * NQ + col] : 0.f;
+      vx[j] = ld_or0(t, bb * NQ + col, ok);
+      vd[j] = ld_or0(t, S1 + bb * NQ + col, ok);
     }
     const Op<SPLIT> ox = split4<SPLIT>(vx), od = split4<SPLIT>(vd);
     const f4 cbb = mma(aB, ox, zero), cbd = mma(aD, ox, zero), cdb = mma(aB, od, zero);
-    if (col < NQ) {
-      const int i1 = col % Q, c = col / Q;
-      float* o = T2 + e * 3 * S2 + c * QQ + i1;
+    const int i1 = col % Q, c = col / Q;
+    float* o = T2 + e * 3 * S2;
+    const int o0 = c * QQ + i1;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int i2 = 4 * g + j;
-        if (i2 < Q) {
-          o[Q * i2] = cdb[j];           // x-gradient path
-          o[S2 + Q * i2] = cbd[j];      // y-gradient path
-          o[2 * S2 + Q * i2] = cbb[j];  // z-gradient path
-        }
-      }
+    for (int j = 0; j < 4; ++j) {
+      const int i2 = 4 * g + j;
+      const bool ok = col < NQ && i2 < Q;
+      st_or_dummy(o, o0 + Q * i2, ok, cdb[j], dummy);           // x-gradient path
+      st_or_dummy(o, S2 + o0 + Q * i2, ok, cbd[j], dummy);      // y-gradient path
+      st_or_dummy(o, 2 * S2 + o0 + Q * i2, ok, cbb[j], dummy);  // z-gradient path
     }
   }
   __syncthreads();
@@ -287,7 +323,7 @@ __global__ void __launch_bounds__(256)
     const int e = u / M::TZ;
     if (e >= ncb) break;
     const int col = (u - e * M::TZ) * 16 + cl;  // i1 + Q i2
-    float* t = T2 + e * 3 * S2 + col;
+    float* t = T2 + e * 3 * S2;
     uint32_t gl[12];
     if constexpr (M::GPRE) {
 #pragma unroll
@@ -300,9 +336,9 @@ __global__ void __launch_bounds__(256)
     for (int j = 0; j < 4; ++j) {
       const int c = 4 * g + j;
       const bool ok = c < N && col < QQ;
-      v0[j] = ok ? t[c * QQ] : 0.f;
-      v1[j] = ok ? t[S2 + c * QQ] : 0.f;
-      v2[j] = ok ? t[2 * S2 + c * QQ] : 0.f;
+      v0[j] = ld_or0(t, c * QQ + col, ok);
+      v1[j] = ld_or0(t, S2 + c * QQ + col, ok);
+      v2[j] = ld_or0(t, 2 * S2 + c * QQ + col, ok);
     }
     const f4 gx = mma(aB, split4<SPLIT>(v0), zero);
     const f4 gy = mma(aB, split4<SPLIT>(v1), zero);
@@ -324,16 +360,13 @@ __global__ void __launch_bounds__(256)
     const f4 ax = mma(aBt, split4<SPLIT>(h0), zero);
     const f4 ay = mma(aBt, split4<SPLIT>(h1), zero);
     const f4 az = mma(aDt, split4<SPLIT>(h2), zero);
-    if (col < QQ) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int c = 4 * g + j;
-        if (c < N) {
-          t[c * QQ] = ax[j];
-          t[S2 + c * QQ] = ay[j];
-          t[2 * S2 + c * QQ] = az[j];
-        }
-      }
+    for (int j = 0; j < 4; ++j) {
+      const int c = 4 * g + j;
+      const bool ok = col < QQ && c < N;
+      st_or_dummy(t, c * QQ + col, ok, ax[j], dummy);
+      st_or_dummy(t, S2 + c * QQ + col, ok, ay[j], dummy);
+      st_or_dummy(t, 2 * S2 + c * QQ + col, ok, az[j], dummy);
     }
   }
   __syncthreads();
@@ -347,29 +380,28 @@ __global__ void __launch_bounds__(256)
     if (e >= ncb) break;
     const int col = (u - e * M::TY) * 16 + cl;  // i1 + Q c
     const int i1 = col % Q, c = col / Q;
-    const float* t = T2 + e * 3 * S2 + c * QQ + i1;
+    const float* t = T2 + e * 3 * S2;
+    const int o0 = c * QQ + i1;
     f4 v0, v1, v2;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int i2 = 4 * g + j;
       const bool ok = i2 < Q && col < NQ;
-      v0[j] = ok ? t[Q * i2] : 0.f;
-      v1[j] = ok ? t[S2 + Q * i2] : 0.f;
-      v2[j] = ok ? t[2 * S2 + Q * i2] : 0.f;
+      v0[j] = ld_or0(t, o0 + Q * i2, ok);
+      v1[j] = ld_or0(t, S2 + o0 + Q * i2, ok);
+      v2[j] = ld_or0(t, 2 * S2 + o0 + Q * i2, ok);
     }
     const f4 b0 = mma(aBt, split4<SPLIT>(v0), zero);
     f4 b12 = mma(aDt, split4<SPLIT>(v1), zero);
     b12 = mma(aBt, split4<SPLIT>(v2), b12);
-    if (col < NQ) {
-      float* o = T1 + e * 2 * S1 + i1 * NN + N * c;
+    float* o = T1 + e * 2 * S1;
+    const int o1 = i1 * NN + N * c;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int bb = 4 * g + j;
-        if (bb < N) {
-          o[bb] = b0[j];        // then D_x^T
-          o[S1 + bb] = b12[j];  // then B_x^T
-        }
-      }
+    for (int j = 0; j < 4; ++j) {
+      const int bb = 4 * g + j;
+      const bool ok = col < NQ && bb < N;
+      st_or_dummy(o, o1 + bb, ok, b0[j], dummy);        // then D_x^T
+      st_or_dummy(o, S1 + o1 + bb, ok, b12[j], dummy);  // then B_x^T
     }
   }
   __syncthreads();
@@ -382,20 +414,20 @@ __global__ void __launch_bounds__(256)
     const int e = u / M::TX;
     if (e >= ncb) break;
     const int col = (u - e * M::TX) * 16 + cl;  // b + N c
-    const float* t = T1 + e * 2 * S1 + col;
+    const float* t = T1 + e * 2 * S1;
     f4 v0, v1;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int i1 = 4 * g + j;
       const bool ok = i1 < Q && col < NN;
-      v0[j] = ok ? t[i1 * NN] : 0.f;
-      v1[j] = ok ? t[S1 + i1 * NN] : 0.f;
+      v0[j] = ld_or0(t, i1 * NN + col, ok);
+      v1[j] = ld_or0(t, S1 + i1 * NN + col, ok);
     }
     f4 yv = mma(aDt, split4<SPLIT>(v0), zero);
     yv = mma(aBt, split4<SPLIT>(v1), yv);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      if (pv[i][j] != kNoDof) lds_atomic_add(bx + pv[i][j], yv[j]);
+      lds_atomic_add(pv[i][j] != kNoDof ? bx + pv[i][j] : dummy, yv[j]);
   }
   __syncthreads();
 
@@ -417,7 +449,8 @@ hipError_t launch_mfma_nq(const float* x, float* y, const uint32_t* Gm,
                           const float* tB, const float* tD, const BlockArgs& k,
                           hipStream_t s) {
   if (k.nblk <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_apply3d_mfma<N, Q, SPLIT>), dim3(k.nblk), dim3(256), 0, s,
+  hipLaunchKernelGGL((k_apply3d_mfma<N, Q, SPLIT>), dim3(k.nblk),
+                     dim3(MfmaShape<N, Q, SPLIT>::NT), 0, s,
                      x, y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell, k.nblk,
                      Gm, tB, tD);
   return hipGetLastError();
