@@ -259,24 +259,34 @@ struct sf_plan {
 namespace {
 
 int upload_tables(sf_plan* P) {
-  const int64_t tn = (int64_t)P->q * P->n;
-  SF_HIP(P->dB.alloc(tn));
-  SF_HIP(P->dD.alloc(tn));
-  SF_HIP(P->dW.alloc(P->q));
-  SF_HIP(P->dPts.alloc(P->q));
-  SF_HIP(P->dB32.alloc(tn));
-  SF_HIP(P->dD32.alloc(tn));
-  SF_HIP(hipMemcpy(P->dB.p, P->hB.data(), tn * 8, hipMemcpyHostToDevice));
-  SF_HIP(hipMemcpy(P->dD.p, P->hD.data(), tn * 8, hipMemcpyHostToDevice));
-  SF_HIP(hipMemcpy(P->dW.p, P->hW.data(), P->q * 8, hipMemcpyHostToDevice));
-  SF_HIP(hipMemcpy(P->dPts.p, P->hPts.data(), P->q * 8, hipMemcpyHostToDevice));
-  std::vector<float> b32(tn), d32(tn);
-  for (int64_t i = 0; i < tn; ++i) {
-    b32[i] = (float)P->hB[i];
-    d32[i] = (float)P->hD[i];
+  // device tables: the Q x N table followed by its N x Q transpose (the
+  // transposed contractions of the 3D apply read rows of the latter)
+  const int q = P->q, n = P->n;
+  const int64_t tn = (int64_t)q * n;
+  std::vector<double> b64(2 * tn), d64(2 * tn);
+  std::vector<float> b32(2 * tn), d32(2 * tn);
+  for (int i = 0; i < q; ++i)
+    for (int j = 0; j < n; ++j) {
+      const double bv = P->hB[(int64_t)i * n + j], dv = P->hD[(int64_t)i * n + j];
+      b64[(int64_t)i * n + j] = b64[tn + (int64_t)j * q + i] = bv;
+      d64[(int64_t)i * n + j] = d64[tn + (int64_t)j * q + i] = dv;
+    }
+  for (int64_t i = 0; i < 2 * tn; ++i) {
+    b32[i] = (float)b64[i];
+    d32[i] = (float)d64[i];
   }
-  SF_HIP(hipMemcpy(P->dB32.p, b32.data(), tn * 4, hipMemcpyHostToDevice));
-  SF_HIP(hipMemcpy(P->dD32.p, d32.data(), tn * 4, hipMemcpyHostToDevice));
+  SF_HIP(P->dB.alloc(2 * tn));
+  SF_HIP(P->dD.alloc(2 * tn));
+  SF_HIP(P->dW.alloc(q));
+  SF_HIP(P->dPts.alloc(q));
+  SF_HIP(P->dB32.alloc(2 * tn));
+  SF_HIP(P->dD32.alloc(2 * tn));
+  SF_HIP(hipMemcpy(P->dB.p, b64.data(), 2 * tn * 8, hipMemcpyHostToDevice));
+  SF_HIP(hipMemcpy(P->dD.p, d64.data(), 2 * tn * 8, hipMemcpyHostToDevice));
+  SF_HIP(hipMemcpy(P->dW.p, P->hW.data(), q * 8, hipMemcpyHostToDevice));
+  SF_HIP(hipMemcpy(P->dPts.p, P->hPts.data(), q * 8, hipMemcpyHostToDevice));
+  SF_HIP(hipMemcpy(P->dB32.p, b32.data(), 2 * tn * 4, hipMemcpyHostToDevice));
+  SF_HIP(hipMemcpy(P->dD32.p, d32.data(), 2 * tn * 4, hipMemcpyHostToDevice));
   return SF_OK;
 }
 

@@ -102,6 +102,9 @@ __host__ __device__ inline int64_t g3_index(int64_t bstart, int ncb, int64_t cel
 }
 
 // Block-assembled 3D apply, one launch over all cell blocks.
+// tB / tD hold the Q x N table followed by its N x Q transpose: the
+// transposed passes read whole rows of the transpose (one scalar load per
+// row instead of one per element of a column).
 //
 // Workgroup w owns block b = xcd_remap(w): cells bcell[b] .. bcell[b+1] - 1,
 // at most EB of them, consecutive in the caller's order.  The host listed
@@ -286,15 +289,16 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
       ty[l] = gq[1][l] * gx + gq[3][l] * gy + gq[4][l] * gz;
       tz[l] = gq[2][l] * gx + gq[4][l] * gy + gq[5][l] * gz;
     }
+    TableRows<T, S::TK> tt_(tB + Q * N, tD + Q * N);  // transposed tables
 #pragma unroll
     for (int z = 0; z < N; ++z) {
-      t_.at(z);
+      tt_.at(z);
       T rx = T(0), ry = T(0), rz = T(0);
 #pragma unroll
       for (int l = 0; l < Q; ++l) {
-        rx = fma(t_.b[l * N + z], tx[l], rx);
-        ry = fma(t_.b[l * N + z], ty[l], ry);
-        rz = fma(t_.d[l * N + z], tz[l], rz);
+        rx = fma(tt_.b[z * Q + l], tx[l], rx);
+        ry = fma(tt_.b[z * Q + l], ty[l], ry);
+        rz = fma(tt_.d[z * Q + l], tz[l], rz);
       }
       sB[z * S::QQ + r3] = rx;
       sB[S::SB1 + z * S::QQ + r3] = ry;
@@ -319,7 +323,7 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
       rz[k] = sB[2 * S::SB1 + o];
     }
     if constexpr (S::LC) __syncthreads();
-    TableRows<T, S::TK> t_(tB, tD);
+    TableRows<T, S::TK> t_(tB + Q * N, tD + Q * N);  // transposed tables
     if (on4)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
@@ -327,9 +331,9 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
       T sx = T(0), so = T(0);
 #pragma unroll
       for (int k = 0; k < Q; ++k) {
-        sx = fma(t_.b[k * N + j], rx[k], sx);
-        so = fma(t_.d[k * N + j], ry[k], so);
-        so = fma(t_.b[k * N + j], rz[k], so);
+        sx = fma(t_.b[j * Q + k], rx[k], sx);
+        so = fma(t_.d[j * Q + k], ry[k], so);
+        so = fma(t_.b[j * Q + k], rz[k], so);
       }
       sA[(z * N + j) * S::QP + i] = sx;           // then D_x^T
       sA[S::SA1 + (z * N + j) * S::QP + i] = so;  // then B_x^T
@@ -355,15 +359,15 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
       sx[i] = sA[r1 * S::QP + i];
       so[i] = sA[S::SA1 + r1 * S::QP + i];
     }
-    TableRows<T, S::TK> t_(tB, tD);
+    TableRows<T, S::TK> t_(tB + Q * N, tD + Q * N);  // transposed tables
 #pragma unroll
     for (int m = 0; m < N; ++m) {
       t_.at(m);
       T v = T(0);
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
-        v = fma(t_.d[i * N + m], sx[i], v);
-        v = fma(t_.b[i * N + m], so[i], v);
+        v = fma(t_.d[m * Q + i], sx[i], v);
+        v = fma(t_.b[m * Q + i], so[i], v);
       }
       if (li[m] != kNoDof) lds_atomic_add(regY + li[m], v);
     }

@@ -117,33 +117,37 @@ def test_adjoint_vs_oracle_and_fd(pkg, oracle_mod, cuda, D, m, p, q):
     assert rel_l2(kb, kb_ref) < 1e-12
     assert rel_l2(ub, ub_ref) < 1e-12
     # central finite differences, h = 1e-6, FP64, random unit directions.
-    # J is quadratic in kappa and in u, so the only error is cancellation.
-    def Jk(k):
+    # J is quadratic in kappa and in u, so the only error is cancellation;
+    # J(+) - J(-) is evaluated as 1/2 (r+ - r-) . (r+ + r-), which keeps the
+    # subtraction at the level of the residual vectors (two large sums
+    # subtracted would leave rounding of order eps |J| / h).
+    def rk(k):
         A.set_kappa(k)
         y = cuda.empty_like(ud)
         A.mul(y, ud)
-        rr = y - dd
-        return 0.5 * float((rr * rr).sum())
+        return (y - dd).cpu().numpy()
 
-    def Ju(v):
+    def ru(v):
         A.set_kappa(kappa)
         y = cuda.empty_like(ud)
         A.mul(y, cuda.from_numpy(v).cuda())
-        rr = y - dd
-        return 0.5 * float((rr * rr).sum())
+        return (y - dd).cpu().numpy()
+
+    def central(rp, rm):
+        return 0.5 * float((rp - rm) @ (rp + rm)) / (2 * h)
 
     h = 1e-6
     for s in range(4):
         dk = pkg.normal(100 + s, kappa.size)
         dk /= np.linalg.norm(dk)
-        fd = (Jk(kappa + h * dk) - Jk(kappa - h * dk)) / (2 * h)
+        fd = central(rk(kappa + h * dk), rk(kappa - h * dk))
         an = kb @ dk
-        assert abs(fd - an) < 1e-6 * (abs(an) + 1e-3 * np.linalg.norm(kb))
+        assert abs(fd - an) < 1e-6 * (abs(an) + 1e-3 * np.linalg.norm(kb)), (s, fd, an)
         du = pkg.normal(200 + s, u.size)
         du /= np.linalg.norm(du)
-        fd = (Ju(u + h * du) - Ju(u - h * du)) / (2 * h)
+        fd = central(ru(u + h * du), ru(u - h * du))
         an = ub @ du
-        assert abs(fd - an) < 1e-6 * (abs(an) + 1e-3 * np.linalg.norm(ub))
+        assert abs(fd - an) < 1e-6 * (abs(an) + 1e-3 * np.linalg.norm(ub)), (s, fd, an)
     A.free()
 
 
