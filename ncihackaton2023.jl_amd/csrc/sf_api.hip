@@ -202,6 +202,8 @@ struct sf_plan {
   DevBuf<float> dB32, dD32;
   DevBuf<int32_t> cells;
   DevBuf<double> G, Gk, kappa;
+  DevBuf<double> X;   // 3D cell vertices (kept for the fused-geometry apply)
+  bool fused_geom = false;
   DevBuf<float> G32;
   DevBuf<bf16s> Gbf;
   DevBuf<uint32_t> Gm;  // bf16 pairs in the layout of the MFMA apply (3D)
@@ -231,7 +233,7 @@ struct sf_plan {
   void release_all() {
     dB.release(); dD.release(); dW.release(); dPts.release();
     dB32.release(); dD32.release();
-    cells.release(); G.release(); Gk.release(); kappa.release();
+    cells.release(); G.release(); Gk.release(); kappa.release(); X.release();
     G32.release(); Gbf.release(); Gm.release(); flag.release();
     pidx.release(); bdofs.release(); bexcl.release(); cell_block.release();
     boff.release(); bcell.release();
@@ -245,7 +247,7 @@ struct sf_plan {
 
   int64_t device_bytes() const {
     return dB.bytes() + dD.bytes() + dW.bytes() + dPts.bytes() + dB32.bytes() +
-           dD32.bytes() + cells.bytes() + G.bytes() + Gk.bytes() +
+           dD32.bytes() + cells.bytes() + G.bytes() + Gk.bytes() + X.bytes() +
            kappa.bytes() + G32.bytes() + Gbf.bytes() + Gm.bytes() + flag.bytes() +
            pidx.bytes() + bdofs.bytes() + bexcl.bytes() + cell_block.bytes() +
            boff.bytes() + bcell.bytes() +
@@ -323,8 +325,11 @@ int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
                  int accumulate = 0, bool zeroed = false) {
   if (P->nfree == 0) return SF_OK;  // nothing to compute (and x may be empty)
   if (!accumulate && !zeroed) SF_HIP(launch_fill<double>(y, P->nfree, 0.0, s));
+  // the fused-geometry kernel when the plan has vertices and no kappa
+  const bool geo = P->fused_geom && P->dim == 3 && !P->has_kappa && P->X.p;
   hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
-                           P->dB.p, P->dD.p, P->ncell, P->blocks(), accumulate, s);
+                           P->dB.p, P->dD.p, P->ncell, P->blocks(), accumulate, s,
+                           geo ? P->X.p : nullptr, P->dPts.p, P->dW.p);
   if (e == hipErrorNotSupported)
     return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
   SF_HIP(e);
@@ -982,6 +987,8 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
     // new blocks: the block-interleaved geometry must be set again
     P->has_cells = false;
     P->has_geom = false;
+    P->fused_geom = false;
+    P->X.release();
     P->g32_valid = P->gbf_valid = P->gm_valid = false;
     int rc = build_blocks(P, h);
     if (rc) return rc;
@@ -1020,7 +1027,8 @@ int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
   if (need_cells_for_geometry(P)) return SF_ERR_NOT_READY;
   DeviceGuard g(P->device);
   const int64_t nv = P->ncell * (P->dim == 3 ? 24 : 8);
-  DevBuf<double> X;
+  DevBuf<double>& X = P->X;
+  P->fused_geom = false;
   SF_HIP(X.alloc(nv));
   int rc = SF_OK;
   do {
@@ -1047,7 +1055,11 @@ int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
     P->has_geom = true;
     rc = finish_geometry(P);
   } while (0);
-  X.release();
+  // 3D: keep the vertices; the FP64 apply recomputes G from them
+  // (SFHIP_FUSED_GEOMETRY=0 reads the stored G instead)
+  const char* fg = getenv("SFHIP_FUSED_GEOMETRY");
+  P->fused_geom = rc == SF_OK && P->dim == 3 && !(fg && fg[0] == '0');
+  if (!P->fused_geom) X.release();
   return rc;
   SF_CATCH
 }
@@ -1077,6 +1089,8 @@ int sf_plan_set_geometry(sf_plan* P, const double* G) {
     }
   }
   P->has_geom = true;
+  P->fused_geom = false;  // a caller-supplied G is what the apply must use
+  P->X.release();
   return finish_geometry(P);
   SF_CATCH
 }

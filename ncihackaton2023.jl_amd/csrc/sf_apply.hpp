@@ -101,6 +101,33 @@ __host__ __device__ inline int64_t g3_index(int64_t bstart, int ncb, int64_t cel
   return bstart * 6 * QQ * Q + ((int64_t)(c * Q + l) * ncb + (cell - bstart)) * QQ + r;
 }
 
+// Geometric factor of one quadrature point from the columns of J (the
+// formula of the geometry kernel: G = w / |det J| C^T C, C the cofactor
+// matrix; entries xx, xy, xz, yy, yz, zz).
+template <typename T>
+__device__ __forceinline__ void geom_from_jacobian(const T (&j0)[3], const T (&j1)[3],
+                                                   const T (&j2)[3], T w, T (&g)[6]) {
+  // J[a][b] = jb[a]
+  T C[3][3];
+  C[0][0] = j1[1] * j2[2] - j2[1] * j1[2];
+  C[0][1] = j2[1] * j0[2] - j0[1] * j2[2];
+  C[0][2] = j0[1] * j1[2] - j1[1] * j0[2];
+  C[1][0] = j2[0] * j1[2] - j1[0] * j2[2];
+  C[1][1] = j0[0] * j2[2] - j2[0] * j0[2];
+  C[1][2] = j1[0] * j0[2] - j0[0] * j1[2];
+  C[2][0] = j1[0] * j2[1] - j2[0] * j1[1];
+  C[2][1] = j2[0] * j0[1] - j0[0] * j2[1];
+  C[2][2] = j0[0] * j1[1] - j1[0] * j0[1];
+  const T det = j0[0] * C[0][0] + j1[0] * C[0][1] + j2[0] * C[0][2];
+  const T sc = w / fabs(det);
+  g[0] = sc * (C[0][0] * C[0][0] + C[1][0] * C[1][0] + C[2][0] * C[2][0]);
+  g[1] = sc * (C[0][0] * C[0][1] + C[1][0] * C[1][1] + C[2][0] * C[2][1]);
+  g[2] = sc * (C[0][0] * C[0][2] + C[1][0] * C[1][2] + C[2][0] * C[2][2]);
+  g[3] = sc * (C[0][1] * C[0][1] + C[1][1] * C[1][1] + C[2][1] * C[2][1]);
+  g[4] = sc * (C[0][1] * C[0][2] + C[1][1] * C[1][2] + C[2][1] * C[2][2]);
+  g[5] = sc * (C[0][2] * C[0][2] + C[1][2] * C[1][2] + C[2][2] * C[2][2]);
+}
+
 // Block-assembled 3D apply, one launch over all cell blocks.
 // tB / tD hold the Q x N table followed by its N x Q transpose: the
 // transposed passes read whole rows of the transpose (one scalar load per
@@ -117,7 +144,14 @@ __host__ __device__ inline int64_t g3_index(int64_t bstart, int ncb, int64_t cel
 // with other blocks with one hardware atomic add (consecutive lanes hit
 // consecutive ids).  y = A x needs y zeroed first (the caller's fill) unless
 // accumulate is set, in which case exclusive dofs are read-modify-written.
-template <int N, int Q, typename T, typename TG>
+//
+// GEO: the geometric factor is not read but recomputed at every quadrature
+// point from the cell's 8 vertices (Xv, 24 values per cell, trilinear map)
+// and the 1D points / weights (tP, tW): 192 bytes per cell instead of
+// 48 q^3.  The z-column thread of quadrature column (i, k) interpolates
+// dx/dxi and dx/deta linearly in zeta between edge vectors it forms once,
+// and dx/dzeta is constant along its column.
+template <int N, int Q, typename T, typename TG, bool GEO>
 __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     k_apply3d(const T* __restrict__ x, T* __restrict__ y,
               const uint16_t* __restrict__ pidx,
@@ -126,9 +160,11 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
               const int32_t* __restrict__ bexcl,
               const int64_t* __restrict__ bcell, int nblk,
               const TG* __restrict__ G, const T* __restrict__ tB,
-              const T* __restrict__ tD, int accumulate) {
+              const T* __restrict__ tD, const T* __restrict__ Xv,
+              const T* __restrict__ tP, const T* __restrict__ tW, int accumulate) {
   using S = Apply3Shape<N, Q, T>;
-  __shared__ T smem[S::RC];
+  __shared__ T smem[S::RC + (GEO ? S::EB * 24 : 0)];
+  T* const regV = smem + S::RC;  // GEO: the block's cell vertices
   T* const regA = smem;                               // x / y^T pass out
   T* const regB = S::LC ? smem : smem + S::RA;        // y pass out, z in place
   T* const regX = S::LC ? smem + S::RA : regB;        // the block's x values
@@ -172,8 +208,11 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   const bool own3 = tid < S::EB * S::QQ;
   const int e3 = tid / S::QQ, r3 = tid - e3 * S::QQ;  // r3 = k*Q + i
   const bool act3 = own3 && e3 < ncb;
-  T gq[6][Q];
-  if constexpr (sizeof(TG) == 8) {
+  T gq[GEO ? 1 : 6][GEO ? 1 : Q];
+  if constexpr (GEO) {
+    // the block's vertex coordinates into LDS (ncb * 24 consecutive values)
+    if (tid < ncb * 24) regV[tid] = Xv[base * 24 + tid];
+  } else if constexpr (sizeof(TG) == 8) {
     // block-interleaved G: thread (e3, r3) is lane tid of every [c*Q + l] row
     const TG* g = G + base * 6 * S::QQQ + (act3 ? tid : 0);
     const int kst = ncb * S::QQ;
@@ -274,6 +313,32 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
       cz[z] = sB[2 * S::SB1 + z * S::QQ + r3];
     }
     T tx[Q], ty[Q], tz[Q];
+    // GEO: edge vectors of the trilinear map for this quadrature column
+    // (i, k): dx/dxi = E0 + zeta dE, dx/deta = F0 + zeta dF, dx/dzeta = H
+    T E0[3], dE[3], F0[3], dF[3], H[3], wik = T(0);
+    if constexpr (GEO) {
+      const int qi = r3 % Q, qk = r3 / Q;
+      const T xi = tP[qi], et = tP[qk];
+      wik = tW[qi] * tW[qk];
+      const T* V = regV + e3 * 24;  // vertex v = b0 + 2 b1 + 4 b2, 3 coords
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        // dx/dxi on the planes zeta = 0, 1 (interpolated in eta)
+        const T e0 = (T(1) - et) * (V[1 * 3 + a] - V[0 * 3 + a]) + et * (V[3 * 3 + a] - V[2 * 3 + a]);
+        const T e1 = (T(1) - et) * (V[5 * 3 + a] - V[4 * 3 + a]) + et * (V[7 * 3 + a] - V[6 * 3 + a]);
+        // dx/deta on the planes zeta = 0, 1 (interpolated in xi)
+        const T f0 = (T(1) - xi) * (V[2 * 3 + a] - V[0 * 3 + a]) + xi * (V[3 * 3 + a] - V[1 * 3 + a]);
+        const T f1 = (T(1) - xi) * (V[6 * 3 + a] - V[4 * 3 + a]) + xi * (V[7 * 3 + a] - V[5 * 3 + a]);
+        // dx/dzeta, bilinear in (xi, eta)
+        const T h0 = (T(1) - xi) * (V[4 * 3 + a] - V[0 * 3 + a]) + xi * (V[5 * 3 + a] - V[1 * 3 + a]);
+        const T h1 = (T(1) - xi) * (V[6 * 3 + a] - V[2 * 3 + a]) + xi * (V[7 * 3 + a] - V[3 * 3 + a]);
+        E0[a] = e0;
+        dE[a] = e1 - e0;
+        F0[a] = f0;
+        dF[a] = f1 - f0;
+        H[a] = (T(1) - et) * h0 + et * h1;
+      }
+    }
     TableRows<T, S::TK> t_(tB, tD);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
@@ -285,9 +350,23 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
         gy = fma(t_.b[l * N + z], cy[z], gy);
         gz = fma(t_.d[l * N + z], cz[z], gz);
       }
-      tx[l] = gq[0][l] * gx + gq[1][l] * gy + gq[2][l] * gz;
-      ty[l] = gq[1][l] * gx + gq[3][l] * gy + gq[4][l] * gz;
-      tz[l] = gq[2][l] * gx + gq[4][l] * gy + gq[5][l] * gz;
+      if constexpr (GEO) {
+        const T ze = tP[l];
+        T j0[3], j1[3], g[6];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          j0[a] = fma(ze, dE[a], E0[a]);
+          j1[a] = fma(ze, dF[a], F0[a]);
+        }
+        geom_from_jacobian<T>(j0, j1, H, wik * tW[l], g);
+        tx[l] = g[0] * gx + g[1] * gy + g[2] * gz;
+        ty[l] = g[1] * gx + g[3] * gy + g[4] * gz;
+        tz[l] = g[2] * gx + g[4] * gy + g[5] * gz;
+      } else {
+        tx[l] = gq[0][l] * gx + gq[1][l] * gy + gq[2][l] * gz;
+        ty[l] = gq[1][l] * gx + gq[3][l] * gy + gq[4][l] * gz;
+        tz[l] = gq[2][l] * gx + gq[4][l] * gy + gq[5][l] * gz;
+      }
     }
     TableRows<T, S::TK> tt_(tB + Q * N, tD + Q * N);  // transposed tables
 #pragma unroll
@@ -543,6 +622,11 @@ struct ApplyArgs {
   const T* tD;
   int64_t ncell;
   int accumulate;  // 3D: 0 -> y zeroed by the caller, 1 -> y += A x
+  // 3D: cell vertices, 1D points and weights; set -> G recomputed in the
+  // kernel (the G argument is then unused)
+  const T* Xv = nullptr;
+  const T* tP = nullptr;
+  const T* tW = nullptr;
 };
 
 // Host-built block structure of the 3D apply (see k_apply3d).
@@ -560,9 +644,14 @@ hipError_t launch_apply3d_nq(const ApplyArgs<T, TG>& a, const BlockArgs& k,
                              hipStream_t s) {
   using S = Apply3Shape<N, Q, T>;
   if (k.nblk <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_apply3d<N, Q, T, TG>), dim3(k.nblk), dim3(S::NT), 0, s,
-                     a.x, a.y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell,
-                     k.nblk, a.G, a.tB, a.tD, a.accumulate);
+  if (a.Xv)
+    hipLaunchKernelGGL((k_apply3d<N, Q, T, TG, true>), dim3(k.nblk), dim3(S::NT), 0, s,
+                       a.x, a.y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell,
+                       k.nblk, a.G, a.tB, a.tD, a.Xv, a.tP, a.tW, a.accumulate);
+  else
+    hipLaunchKernelGGL((k_apply3d<N, Q, T, TG, false>), dim3(k.nblk), dim3(S::NT), 0, s,
+                       a.x, a.y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell,
+                       k.nblk, a.G, a.tB, a.tD, a.Xv, a.tP, a.tW, a.accumulate);
   return hipGetLastError();
 }
 

@@ -41,10 +41,13 @@ hipError_t launch_to_bf16(const double* a, int64_t n, bf16s* b, hipStream_t s);
 // ---- applies (sf_apply_*.hip) ----
 // 3D (k != nullptr): y = A x, or y += A x when accumulate; no fill needed.
 // 2D (k == nullptr): y += A x by atomics (the caller zeroes y for y = A x).
+// 3D with Xv (cell vertices, 24 per cell) and the 1D points / weights tP,
+// tW: the geometric factor is recomputed in the kernel instead of read.
 hipError_t apply_f64(int dim, int n, int q, const double* x, double* y,
                      const int32_t* cells, const double* G, const double* tB,
                      const double* tD, int64_t ncell, const BlockArgs* k,
-                     int accumulate, hipStream_t s);
+                     int accumulate, hipStream_t s, const double* Xv = nullptr,
+                     const double* tP = nullptr, const double* tW = nullptr);
 hipError_t apply_f32(int dim, int n, int q, const float* x, float* y,
                      const int32_t* cells, const float* G, const float* tB,
                      const float* tD, int64_t ncell, const BlockArgs* k,
