@@ -203,6 +203,7 @@ struct sf_plan {
   DevBuf<int32_t> cells;
   DevBuf<double> G, Gk, kappa;
   DevBuf<double> X;   // 3D cell vertices (kept for the fused-geometry apply)
+  DevBuf<float> X32, Pts32, W32;  // FP32 copies for the FP32 fused apply
   bool fused_geom = false;
   DevBuf<float> G32;
   DevBuf<bf16s> Gbf;
@@ -234,6 +235,7 @@ struct sf_plan {
     dB.release(); dD.release(); dW.release(); dPts.release();
     dB32.release(); dD32.release();
     cells.release(); G.release(); Gk.release(); kappa.release(); X.release();
+    X32.release(); Pts32.release(); W32.release();
     G32.release(); Gbf.release(); Gm.release(); flag.release();
     pidx.release(); bdofs.release(); bexcl.release(); cell_block.release();
     boff.release(); bcell.release();
@@ -248,6 +250,7 @@ struct sf_plan {
   int64_t device_bytes() const {
     return dB.bytes() + dD.bytes() + dW.bytes() + dPts.bytes() + dB32.bytes() +
            dD32.bytes() + cells.bytes() + G.bytes() + Gk.bytes() + X.bytes() +
+           X32.bytes() + Pts32.bytes() + W32.bytes() +
            kappa.bytes() + G32.bytes() + Gbf.bytes() + Gm.bytes() + flag.bytes() +
            pidx.bytes() + bdofs.bytes() + bexcl.bytes() + cell_block.bytes() +
            boff.bytes() + bcell.bytes() +
@@ -359,6 +362,21 @@ int prepare(sf_plan* P, int precision) {
     return SF_OK;
   }
   if (precision == SF_PREC_F32) {
+    if (P->fused_geom && P->dim == 3 && !P->X32.p) {
+      // FP32 vertices and 1D quadrature for the FP32 fused-geometry apply
+      SF_HIP(P->X32.alloc(P->ncell * 24));
+      SF_HIP(P->Pts32.alloc(P->q));
+      SF_HIP(P->W32.alloc(P->q));
+      SF_HIP(launch_to_f32(P->X.p, P->ncell * 24, P->X32.p, nullptr));
+      std::vector<float> pw(2 * P->q);
+      for (int i = 0; i < P->q; ++i) {
+        pw[i] = (float)P->hPts[i];
+        pw[P->q + i] = (float)P->hW[i];
+      }
+      SF_HIP(hipMemcpy(P->Pts32.p, pw.data(), P->q * 4, hipMemcpyHostToDevice));
+      SF_HIP(hipMemcpy(P->W32.p, pw.data() + P->q, P->q * 4, hipMemcpyHostToDevice));
+      SF_HIP(hipDeviceSynchronize());
+    }
     if (!P->g32_valid) {
       SF_HIP(P->G32.alloc(ng));
       if (P->dim == 3)
@@ -395,8 +413,10 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
   if (!zeroed) SF_HIP(launch_fill<float>(y, P->nfree, 0.0f, s));
   hipError_t e;
   if (precision == SF_PREC_F32) {
+    const bool geo = P->fused_geom && P->dim == 3 && !P->has_kappa && P->X32.p;
     e = apply_f32(P->dim, P->n, P->q, x, y, P->cells.p, P->G32.p, P->dB32.p,
-                  P->dD32.p, P->ncell, P->blocks(), 0, s);
+                  P->dD32.p, P->ncell, P->blocks(), 0, s, geo ? P->X32.p : nullptr,
+                  P->Pts32.p, P->W32.p);
   } else if (precision == SF_PREC_F32_GBF16) {
     e = apply_f32_gbf16(P->dim, P->n, P->q, x, y, P->cells.p, P->Gbf.p,
                         P->dB32.p, P->dD32.p, P->ncell, P->blocks(), 0, s);
@@ -989,6 +1009,7 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
     P->has_geom = false;
     P->fused_geom = false;
     P->X.release();
+    P->X32.release();
     P->g32_valid = P->gbf_valid = P->gm_valid = false;
     int rc = build_blocks(P, h);
     if (rc) return rc;
@@ -1029,6 +1050,7 @@ int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
   const int64_t nv = P->ncell * (P->dim == 3 ? 24 : 8);
   DevBuf<double>& X = P->X;
   P->fused_geom = false;
+  P->X32.release();
   SF_HIP(X.alloc(nv));
   int rc = SF_OK;
   do {
@@ -1091,6 +1113,7 @@ int sf_plan_set_geometry(sf_plan* P, const double* G) {
   P->has_geom = true;
   P->fused_geom = false;  // a caller-supplied G is what the apply must use
   P->X.release();
+  P->X32.release();
   return finish_geometry(P);
   SF_CATCH
 }

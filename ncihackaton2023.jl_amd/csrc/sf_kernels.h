@@ -51,7 +51,8 @@ hipError_t apply_f64(int dim, int n, int q, const double* x, double* y,
 hipError_t apply_f32(int dim, int n, int q, const float* x, float* y,
                      const int32_t* cells, const float* G, const float* tB,
                      const float* tD, int64_t ncell, const BlockArgs* k,
-                     int accumulate, hipStream_t s);
+                     int accumulate, hipStream_t s, const float* Xv = nullptr,
+                     const float* tP = nullptr, const float* tW = nullptr);
 hipError_t apply_f32_gbf16(int dim, int n, int q, const float* x, float* y,
                            const int32_t* cells, const bf16s* G,
                            const float* tB, const float* tD, int64_t ncell,
