@@ -101,13 +101,28 @@ __host__ __device__ inline int64_t g3_index(int64_t bstart, int ncb, int64_t cel
   return bstart * 6 * QQ * Q + ((int64_t)(c * Q + l) * ncb + (cell - bstart)) * QQ + r;
 }
 
-// Geometric factor of one quadrature point from the columns of J (the
-// formula of the geometry kernel: G = w / |det J| C^T C, C the cofactor
-// matrix; entries xx, xy, xz, yy, yz, zz).
+// 1 / a for a > 0: the hardware reciprocal estimate refined by two Newton
+// steps r <- r (2 - a r) (each doubles the correct bits), instead of the
+// IEEE division sequence.
+__device__ __forceinline__ double recip_pos(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  r = fma(r, fma(-a, r, 1.0), r);
+  r = fma(r, fma(-a, r, 1.0), r);
+  return r;
+}
+__device__ __forceinline__ float recip_pos(float a) {
+  float r = __builtin_amdgcn_rcpf(a);
+  return fmaf(r, fmaf(-a, r, 1.0f), r);
+}
+
+// t = G g at one quadrature point from the columns of J, with G = w / |det J|
+// C^T C (C the cofactor matrix; the geometry kernel's formula) applied as
+// s C^T (C g): 9 + 9 FMAs and one scaling instead of forming the six G
+// entries.
 template <typename T>
-__device__ __forceinline__ void geom_from_jacobian(const T (&j0)[3], const T (&j1)[3],
-                                                   const T (&j2)[3], T w, T (&g)[6]) {
-  // J[a][b] = jb[a]
+__device__ __forceinline__ void apply_geom(const T (&j0)[3], const T (&j1)[3],
+                                           const T (&j2)[3], T w, T g0, T g1, T g2,
+                                           T& t0, T& t1, T& t2) {
   T C[3][3];
   C[0][0] = j1[1] * j2[2] - j2[1] * j1[2];
   C[0][1] = j2[1] * j0[2] - j0[1] * j2[2];
@@ -119,13 +134,13 @@ __device__ __forceinline__ void geom_from_jacobian(const T (&j0)[3], const T (&j
   C[2][1] = j2[0] * j0[1] - j0[0] * j2[1];
   C[2][2] = j0[0] * j1[1] - j1[0] * j0[1];
   const T det = j0[0] * C[0][0] + j1[0] * C[0][1] + j2[0] * C[0][2];
-  const T sc = w / fabs(det);
-  g[0] = sc * (C[0][0] * C[0][0] + C[1][0] * C[1][0] + C[2][0] * C[2][0]);
-  g[1] = sc * (C[0][0] * C[0][1] + C[1][0] * C[1][1] + C[2][0] * C[2][1]);
-  g[2] = sc * (C[0][0] * C[0][2] + C[1][0] * C[1][2] + C[2][0] * C[2][2]);
-  g[3] = sc * (C[0][1] * C[0][1] + C[1][1] * C[1][1] + C[2][1] * C[2][1]);
-  g[4] = sc * (C[0][1] * C[0][2] + C[1][1] * C[1][2] + C[2][1] * C[2][2]);
-  g[5] = sc * (C[0][2] * C[0][2] + C[1][2] * C[1][2] + C[2][2] * C[2][2]);
+  const T sc = w * recip_pos(fabs(det));
+  const T v0 = C[0][0] * g0 + C[0][1] * g1 + C[0][2] * g2;
+  const T v1 = C[1][0] * g0 + C[1][1] * g1 + C[1][2] * g2;
+  const T v2 = C[2][0] * g0 + C[2][1] * g1 + C[2][2] * g2;
+  t0 = sc * (C[0][0] * v0 + C[1][0] * v1 + C[2][0] * v2);
+  t1 = sc * (C[0][1] * v0 + C[1][1] * v1 + C[2][1] * v2);
+  t2 = sc * (C[0][2] * v0 + C[1][2] * v1 + C[2][2] * v2);
 }
 
 // Block-assembled 3D apply, one launch over all cell blocks.
@@ -352,16 +367,13 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
       }
       if constexpr (GEO) {
         const T ze = tP[l];
-        T j0[3], j1[3], g[6];
+        T j0[3], j1[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           j0[a] = fma(ze, dE[a], E0[a]);
           j1[a] = fma(ze, dF[a], F0[a]);
         }
-        geom_from_jacobian<T>(j0, j1, H, wik * tW[l], g);
-        tx[l] = g[0] * gx + g[1] * gy + g[2] * gz;
-        ty[l] = g[1] * gx + g[3] * gy + g[4] * gz;
-        tz[l] = g[2] * gx + g[4] * gy + g[5] * gz;
+        apply_geom<T>(j0, j1, H, wik * tW[l], gx, gy, gz, tx[l], ty[l], tz[l]);
       } else {
         tx[l] = gq[0][l] * gx + gq[1][l] * gy + gq[2][l] * gz;
         ty[l] = gq[1][l] * gx + gq[3][l] * gy + gq[4][l] * gz;
