@@ -295,14 +295,16 @@ def mode_cg(args, torch, pkg):
     res["f64"] = st
     log("f64", st)
     for spec in [t for t in args.mixed.split(",") if t]:
-        inner, delta = spec.split(":")
+        parts = spec.split(":")
+        inner, delta = parts[0], parts[1]
+        rec = parts[2] if len(parts) > 2 else "f32"
         x = torch.zeros_like(b)
         st = pkg.cg_mixed_(x, A, b, rtol=1e-10, inner=inner, delta=float(delta),
-                           maxiter=args.maxit)
+                           maxiter=args.maxit, recurrence=rec)
         st["err_vs_f64"] = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
         st["speedup_vs_f64"] = res["f64"]["seconds"] / st["seconds"]
-        res["mixed_%s_delta%s" % (inner, delta)] = st
-        log("mixed", inner, delta, st)
+        res["mixed_%s_delta%s_rec%s" % (inner, delta, rec)] = st
+        log("mixed", inner, delta, rec, st)
     for spec in [t for t in args.inner.split(",") if t]:
         inner, irt = spec.split(":")
         irt = float(irt)
@@ -477,9 +479,9 @@ def main():
     ap.add_argument("--inner", default="f32:1e-2,f32_gbf16:1e-2,mfma_bf16x3:1e-2,mfma_bf16:1e-2",
                     help="cg mode: restarted refinement runs, comma list of "
                          "inner_precision:inner_rtol")
-    ap.add_argument("--mixed", default="f32:0.1,f32_gbf16:0.1,mfma_bf16x3:0.1,mfma_bf16:0.1",
+    ap.add_argument("--mixed", default="f32:0.1:f64,f32:0.1,f32_gbf16:0.1:f64,mfma_bf16x3:0.1",
                     help="cg mode: reliable-update mixed CG runs, comma list of "
-                         "inner_precision:delta")
+                         "inner_precision:delta[:recurrence f32|f64]")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); the apply "

@@ -225,17 +225,20 @@ int sf_cg_ir(sf_plan* plan, const double* b, double* x, double rtol,
              int inner_maxit, sf_solve_stats* stats, void* stream);
 
 /* Mixed-precision CG with reliable updates: one CG recurrence on the
- * reduced-precision operator `inner_precision` (FP32 vectors, dot products
- * accumulated in FP64).  Whenever its recurred residual has fallen by the
- * factor delta (0 < delta < 1) since the last update, x (FP64) absorbs the
- * FP32 partial solution and the residual is replaced by the FP64 b - A x;
- * the search direction is kept, so no Krylov information is lost (unlike
- * sf_cg_ir's restarts).  Stops when the FP64 ||b - A x|| <= rtol ||b|| or
- * after maxit iterations; stats->outer_iterations counts the replacements.
- * x is the initial guess.  No reference counterpart. */
+ * reduced-precision operator `inner_precision`.  fp64_recurrence = 0: FP32
+ * vectors (dot products accumulated in FP64), x (FP64) absorbing the FP32
+ * partial solution at each update; fp64_recurrence = 1: FP64 vectors and
+ * updates, only the operator in reduced precision (an FP32 copy of the
+ * direction goes into it).  Whenever the recurred residual has fallen by
+ * the factor delta (0 < delta < 1) since the last update, the residual is
+ * replaced by the FP64 b - A x; the search direction is kept, so no Krylov
+ * information is lost (unlike sf_cg_ir's restarts).  Stops when the FP64
+ * ||b - A x|| <= rtol ||b|| or after maxit iterations;
+ * stats->outer_iterations counts the replacements.  x is the initial guess.
+ * No reference counterpart. */
 int sf_cg_mixed(sf_plan* plan, const double* b, double* x, double rtol,
                 int maxit, int inner_precision, double delta,
-                sf_solve_stats* stats, void* stream);
+                int fp64_recurrence, sf_solve_stats* stats, void* stream);
 
 #ifdef __cplusplus
 }

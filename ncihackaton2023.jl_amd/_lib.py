@@ -94,7 +94,7 @@ SIGNATURES = [
     ("sf_adjoint", _i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("sf_cg", _i, [_vp, _vp, _vp, _d, _i, ctypes.POINTER(SolveStats), _vp]),
     ("sf_cg_ir", _i, [_vp, _vp, _vp, _d, _i, _i, _d, _i, ctypes.POINTER(SolveStats), _vp]),
-    ("sf_cg_mixed", _i, [_vp, _vp, _vp, _d, _i, _i, _d, ctypes.POINTER(SolveStats), _vp]),
+    ("sf_cg_mixed", _i, [_vp, _vp, _vp, _d, _i, _i, _d, _i, ctypes.POINTER(SolveStats), _vp]),
 ]
 
 _lib = None

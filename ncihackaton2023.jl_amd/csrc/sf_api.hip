@@ -808,6 +808,86 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
   }
 }
 
+// Mixed CG with the Krylov recurrence in FP64 and only the operator in
+// reduced precision: q32 = A32 p32 with p32 the FP32 copy of the FP64
+// direction, every vector update and dot in FP64.  The recurred residual
+// drifts from b - A x by the operator's rounding, so it is replaced by the
+// FP64 b - A x whenever it has dropped by delta (search direction kept,
+// beta' = ||r_true||^2 / ||r_{k-1}||^2).  Fewer iterations than the FP32
+// recurrence (which loses orthogonality faster), FP64 vector traffic.
+int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
+                       double rtol, int maxit, double delta, hipStream_t s,
+                       int* iters_out, int* updates_out, int* nonfinite_out,
+                       double* rel_out) {
+  Reducer red{P->red_part.p, P->red_counter.p};
+  double* r = P->w_r.p;
+  double* p = P->w_p.p;
+  float* p32 = P->w_p32.p;
+  float* q32 = P->w_q32.p;
+  double bb = 0.0;
+  int rc = dot_host(P, b, b, &bb, s);
+  if (rc) return rc;
+  const double bnorm = sqrt(bb), target = rtol * bnorm;
+  double rn = 0.0;
+  rc = true_residual(P, b, x, &rn, s);  // w_t = A x, w_r = b - A x
+  if (rc) return rc;
+  *iters_out = 0;
+  *updates_out = 0;
+  *nonfinite_out = !isfinite(rn);
+  *rel_out = bnorm > 0 ? rn / bnorm : rn;
+  if (!isfinite(rn) || rn <= target || rn == 0.0) return SF_OK;
+  auto threshold = [&](double rnow) { return (delta * rnow > target) ? delta * rnow : target; };
+  const double t0 = threshold(rn);
+  SF_HIP(launch_cg_start<double>(b, P->w_t.p, r, p, P->w_q.p, P->nfree, red, P->scal.p,
+                                 t0 * t0, s));
+  SF_HIP(launch_to_f32(p, P->nfree, p32, s));
+  SF_HIP(launch_fill<float>(q32, P->nfree, 0.0f, s));
+  CGScalars h{};
+  rc = read_scalars(P, &h, s);
+  if (rc) return rc;
+  const int chunk = 8;
+  Progress prog("mixed cg (fp64 recurrence)");
+  for (;;) {
+    while (!h.done && h.iters < maxit) {
+      prog.tick(h.iters, bnorm > 0 ? sqrt(h.rr) / bnorm : sqrt(h.rr));
+      const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
+      for (int k = 0; k < todo; ++k) {
+        rc = do_apply_f32(P, q32, p32, precision, s, true);
+        if (rc) return rc;
+        SF_HIP(launch_cg_alpha_mixed(p, q32, P->nfree, red, P->scal.p, s));
+        SF_HIP(launch_cg_update_mixed(x, r, p, q32, P->nfree, red, P->scal.p, s));
+        SF_HIP(launch_cg_dir_mixed(p, p32, r, q32, P->nfree, P->scal.p, s));
+      }
+      rc = read_scalars(P, &h, s);
+      if (rc) return rc;
+    }
+    *iters_out = h.iters;
+    if (h.nonfinite) {
+      *nonfinite_out = 1;
+      return SF_OK;
+    }
+    // reliable update: r <- b - A x in FP64
+    double rn_new = 0.0;
+    rc = true_residual(P, b, x, &rn_new, s);
+    if (rc) return rc;
+    *updates_out += 1;
+    *rel_out = bnorm > 0 ? rn_new / bnorm : rn_new;
+    if (!isfinite(rn_new)) {
+      *nonfinite_out = 1;
+      return SF_OK;
+    }
+    if (rn_new <= target || rn_new == 0.0 || h.iters >= maxit) return SF_OK;
+    const double c = (h.rr_new > 0.0) ? rn_new * rn_new * h.beta / h.rr_new : 0.0;
+    SF_HIP(launch_dir_coef_mixed(p, p32, r, q32, c, P->nfree, s));
+    const double t = threshold(rn_new);
+    h.rr = rn_new * rn_new;
+    h.stop_rr = t * t;
+    h.done = 0;
+    rc = write_scalars(P, &h, s);
+    if (rc) return rc;
+  }
+}
+
 double now_s() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -1294,8 +1374,8 @@ int sf_cg(sf_plan* P, const double* b, double* x, double rtol, int maxit,
 }
 
 int sf_cg_mixed(sf_plan* P, const double* b, double* x, double rtol, int maxit,
-                int inner_precision, double delta, sf_solve_stats* stats,
-                void* stream) {
+                int inner_precision, double delta, int fp64_recurrence,
+                sf_solve_stats* stats, void* stream) {
   SF_TRY
   int rc = check_ready(P);
   if (rc) return rc;
@@ -1314,8 +1394,11 @@ int sf_cg_mixed(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   const double t0 = now_s();
   int iters = 0, updates = 0, nonfinite = 0;
   double rel = 0.0;
-  rc = cg_reliable(P, inner_precision, b, x, rtol, maxit, delta, s, &iters,
-                   &updates, &nonfinite, &rel);
+  rc = fp64_recurrence
+           ? cg_reliable_f64rec(P, inner_precision, b, x, rtol, maxit, delta, s, &iters,
+                                &updates, &nonfinite, &rel)
+           : cg_reliable(P, inner_precision, b, x, rtol, maxit, delta, s, &iters,
+                         &updates, &nonfinite, &rel);
   if (rc) return rc;
   SF_HIP(hipStreamSynchronize(s));
   const double t1 = now_s();

@@ -112,9 +112,9 @@ __global__ void __launch_bounds__(kRedThreads)
   }
 }
 
-template <typename V>
+template <typename V, typename W = V>
 __global__ void __launch_bounds__(kRedThreads)
-    k_cg_alpha(const V* __restrict__ p, const V* __restrict__ q, int64_t n,
+    k_cg_alpha(const V* __restrict__ p, const W* __restrict__ q, int64_t n,
                Reducer red, CGScalars* __restrict__ sc) {
   if (sc->done) return;  // uniform over the grid: set only by other kernels
   double s = 0.0;
@@ -136,10 +136,10 @@ __global__ void __launch_bounds__(kRedThreads)
   }
 }
 
-template <typename V>
+template <typename V, typename W = V>
 __global__ void __launch_bounds__(kRedThreads)
     k_cg_update(V* __restrict__ x, V* __restrict__ r,
-                const V* __restrict__ p, const V* __restrict__ q, int64_t n,
+                const V* __restrict__ p, const W* __restrict__ q, int64_t n,
                 Reducer red, CGScalars* __restrict__ sc) {
   if (sc->done) return;
   const V a = (V)sc->alpha;
@@ -147,7 +147,7 @@ __global__ void __launch_bounds__(kRedThreads)
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     x[i] = fma(a, p[i], x[i]);
-    const V ri = fma(-a, q[i], r[i]);
+    const V ri = fma(-a, (V)q[i], r[i]);
     r[i] = ri;
     s += (double)ri * (double)ri;
   }
@@ -179,6 +179,38 @@ __global__ void __launch_bounds__(256)
        i += (int64_t)gridDim.x * blockDim.x) {
     p[i] = fma(b, p[i], r[i]);
     q[i] = V(0);
+  }
+}
+
+// FP64 recurrence with an FP32 operator: p = r + beta p (FP64), its FP32
+// copy for the next apply, and q32 = 0 (the apply adds into it).
+__global__ void __launch_bounds__(256)
+    k_cg_dir_mixed(double* __restrict__ p, float* __restrict__ p32,
+                   const double* __restrict__ r, float* __restrict__ q32,
+                   int64_t n, const CGScalars* __restrict__ sc) {
+  if (sc->done) return;
+  const double b = sc->beta;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double pi = fma(b, p[i], r[i]);
+    p[i] = pi;
+    p32[i] = (float)pi;
+    q32[i] = 0.f;
+  }
+}
+
+// The same with the coefficient given by the host (after a residual
+// replacement).
+__global__ void __launch_bounds__(256)
+    k_dir_coef_mixed(double* __restrict__ p, float* __restrict__ p32,
+                     const double* __restrict__ r, float* __restrict__ q32,
+                     double c, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double pi = fma(c, p[i], r[i]);
+    p[i] = pi;
+    p32[i] = (float)pi;
+    q32[i] = 0.f;
   }
 }
 
@@ -301,6 +333,38 @@ hipError_t launch_cg_dir(V* p, const V* r, V* q, int64_t n, CGScalars* sc,
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_cg_dir<V>, dim3(stream_grid(n)), dim3(256), 0, s, p, r,
                      q, n, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_alpha_mixed(const double* p, const float* q, int64_t n,
+                                 Reducer red, CGScalars* sc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_cg_alpha<double, float>), dim3(red_grid(n)), dim3(kRedThreads), 0, s,
+                     p, q, n, red, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_update_mixed(double* x, double* r, const double* p, const float* q,
+                                  int64_t n, Reducer red, CGScalars* sc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_cg_update<double, float>), dim3(red_grid(n)), dim3(kRedThreads), 0, s,
+                     x, r, p, q, n, red, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_dir_mixed(double* p, float* p32, const double* r, float* q32,
+                               int64_t n, CGScalars* sc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cg_dir_mixed, dim3(stream_grid(n)), dim3(256), 0, s, p, p32, r, q32,
+                     n, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_dir_coef_mixed(double* p, float* p32, const double* r, float* q32,
+                                 double c, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dir_coef_mixed, dim3(stream_grid(n)), dim3(256), 0, s, p, p32, r,
+                     q32, c, n);
   return hipGetLastError();
 }
 

@@ -141,6 +141,19 @@ hipError_t launch_residual(const double* b, const double* Ax, double* r,
 // r32 = r * inv_scale
 hipError_t launch_scale_to_f32(const double* r, double inv_scale, float* r32,
                                int64_t n, hipStream_t s);
+// FP64 CG vectors with an FP32 operator (q = A32 p32):
+//   alpha_mixed : pq = p . q32, alpha = rr / pq
+//   update_mixed: x += alpha p, r -= alpha q32, rr_new = r.r, beta, done
+//   dir_mixed   : p = r + beta p, p32 = (float)p, q32 = 0
+//   dir_coef_mixed: the same with the coefficient c given by the host
+hipError_t launch_cg_alpha_mixed(const double* p, const float* q, int64_t n,
+                                 Reducer red, CGScalars* sc, hipStream_t s);
+hipError_t launch_cg_update_mixed(double* x, double* r, const double* p, const float* q,
+                                  int64_t n, Reducer red, CGScalars* sc, hipStream_t s);
+hipError_t launch_cg_dir_mixed(double* p, float* p32, const double* r, float* q32,
+                               int64_t n, CGScalars* sc, hipStream_t s);
+hipError_t launch_dir_coef_mixed(double* p, float* p32, const double* r, float* q32,
+                                 double c, int64_t n, hipStream_t s);
 // p = r + c p, q = 0 (FP32)
 hipError_t launch_dir_coef(float* p, const float* r, float* q, float c,
                            int64_t n, hipStream_t s);

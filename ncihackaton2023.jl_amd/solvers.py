@@ -53,18 +53,21 @@ def cg_ir_(x, A, b, rtol=1e-10, inner="f32", inner_rtol=1e-3, inner_maxiter=None
 
 
 def cg_mixed_(x, A, b, rtol=1e-10, inner="f32", delta=0.1, maxiter=None,
-              raise_on_fail=False):
-    """Mixed-precision CG with reliable updates: one FP32 CG recurrence on
-    the reduced-precision operator, residual replaced by the FP64 b - A x
-    whenever it has fallen by `delta` (search direction kept)."""
+              recurrence="f32", raise_on_fail=False):
+    """Mixed-precision CG with reliable updates: one CG recurrence on the
+    reduced-precision operator (recurrence 'f32': FP32 vectors; 'f64': FP64
+    vectors, only the operator reduced), residual replaced by the FP64
+    b - A x whenever it has fallen by `delta` (search direction kept)."""
     _vec_ok(A, x)
     _vec_ok(A, b)
     st = _lib.SolveStats()
     prec = _lib.PRECISIONS[inner] if isinstance(inner, str) else int(inner)
     maxiter = 10 * A.n_free if maxiter is None else int(maxiter)
+    if recurrence not in ("f32", "f64"):
+        raise ValueError("recurrence must be 'f32' or 'f64'")
     rc = A._L.sf_cg_mixed(A.plan, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
-                          float(rtol), maxiter, prec, float(delta), ctypes.byref(st),
-                          _torch_stream(x))
+                          float(rtol), maxiter, prec, float(delta), int(recurrence == "f64"),
+                          ctypes.byref(st), _torch_stream(x))
     if rc != _lib.SF_OK and (raise_on_fail or rc != _lib.SF_ERR_NOT_CONVERGED):
         _lib.check(rc)
     return st.as_dict()
