@@ -18,12 +18,15 @@
 // same B[i][j] at the same time, so they live in SGPRs / the scalar cache and
 // cost no LDS bandwidth.  LDS holds only the per-cell intermediate tensors.
 //
-// LDS per cell (3D):  A = 2 tensors N x N x QP   (x-pass output, and the
-//                                                 y^T-pass output later)
-//                     B = 3 tensors N x Q x Q    (y-pass output, rewritten in
-//                                                 place by the z column pass)
-// QP = Q rounded up to an odd number, so the x-pencil rows written by
-// consecutive lanes (stride QP doubles) fall into distinct LDS banks.
+// LDS per cell (3D):  A = 2 tensors N x N x Q   (x-pass output, and the
+//                                                y^T-pass output later)
+//                     B = 3 tensors N x Q x Q   (y-pass output, rewritten in
+//                                                place by the z column pass)
+// The strides of both groups (LdsLayout) are chosen per (N, Q) so that the
+// lanes of each stage hit distinct LDS banks: FP64 from sf_lds_layout.h
+// (gen_lds_layout.py scores every stage's lane pattern against the 64-bit
+// bank model of gfx950: 64 banks of 4 bytes, 16 lanes per pass); otherwise
+// dense tensors with the x-pencil rows padded to an odd length QP.
 // From N = 6 on, A and B share one region: the y pass and the y^T pass
 // read their input pencils into registers, cross a barrier, then overwrite
 // them, so a workgroup holds max(B, A + x values) instead of A + B (more
@@ -34,8 +37,26 @@
 #pragma once
 
 #include "sf_common.hpp"
+#include "sf_lds_layout.h"
 
 namespace sf {
+
+// LDS strides, in elements, of the per-cell tensors of the 3D apply:
+//   A (2 tensors): e*CA + t*N*ZA + z*ZA + y*YA + i
+//   B (3 tensors): e*CB + t*N*ZB + z*ZB + k*Q + i
+template <int N, int Q, int BYTES>
+struct LdsLayout {
+  static constexpr int QP = (Q % 2 == 0) ? Q + 1 : Q;
+  static constexpr int YA = QP, ZA = N * QP, CA = 2 * N * N * QP;
+  static constexpr int ZB = Q * Q, CB = 3 * N * Q * Q;
+};
+#define SF_LDS_ROW(N_, Q_, YA_, ZA_, CA_, ZB_, CB_)                        \
+  template <>                                                             \
+  struct LdsLayout<N_, Q_, 8> {                                           \
+    static constexpr int YA = YA_, ZA = ZA_, CA = CA_, ZB = ZB_, CB = CB_; \
+  };
+SF_LDS_LAYOUT_F64(SF_LDS_ROW)
+#undef SF_LDS_ROW
 
 template <int N, int Q, typename T>
 struct Apply3Shape {
@@ -61,14 +82,27 @@ struct Apply3Shape {
   static constexpr int TK = N <= 4 ? 1 : (N == 5 ? 3 : (N <= 8 && Q <= N + 2 ? 2 : 1));
   static constexpr int EB = EB_T < EB_L ? EB_T : EB_L;
   static constexpr int NT = round_up(EB * MAXT, 64);
-  // LC: one region of RC values -- the x-pass output (A, EB x 2 SA1) at its
+  // minimum waves per SIMD the compiler must fit the FP64 kernel into.  At
+  // N = 7 and (N = 8, Q <= 9) it would use 170-190 VGPRs (2 waves per SIMD)
+  // while the LDS allows 3 workgroups per CU; held to 168 (a few spills at
+  // N = 8) the third wave wins (measured: Q6 -13%, Q7 -6%).  Elsewhere the
+  // compiler's own choice was as fast or faster.
+  static constexpr int WPE = (sizeof(T) == 8 && (N == 7 || (N == 8 && Q <= 9))) ? 3 : 1;
+  // the apply kernel's LDS strides (SA1 / SB1 / QP above size EB and serve
+  // the adjoint kernel's dense layout)
+  using L = LdsLayout<N, Q, (int)sizeof(T)>;
+  static constexpr int YA = L::YA, ZA = L::ZA, TA = N * ZA, CA = L::CA;
+  static constexpr int ZB = L::ZB, TB = N * ZB, CB = L::CB;
+  static_assert(YA >= Q && ZA >= N * YA && CA >= 2 * TA && ZB >= QQ && CB >= 3 * TB,
+                "apply3d: overlapping LDS tensors");
+  // LC: one region of RC values -- the x-pass output (A, EB x CA) at its
   // start with the block's gathered x values (at most EB * N^3 distinct
-  // dofs) behind it; later the y-pass output (B, EB x 3 SB1) over both;
+  // dofs) behind it; later the y-pass output (B, EB x CB) over both;
   // later the y^T-pass output over A and the assembled y values at the end.
   // Otherwise A, then B; B also holds the x values and later the y values.
-  static constexpr int RA = EB * 2 * SA1;
+  static constexpr int RA = EB * CA;
   static constexpr int RX = EB * NNN;
-  static constexpr int RC = LC ? cmax(EB * 3 * SB1, RA + RX) : RA + cmax(EB * 3 * SB1, RX);
+  static constexpr int RC = LC ? cmax(EB * CB, RA + RX) : RA + cmax(EB * CB, RX);
   static constexpr int LDS_BYTES = RC * (int)sizeof(T);
   // distinct dofs of a block handled per thread in the gather / write-back
   static constexpr int KU = (EB * NNN + NT - 1) / NT;
@@ -168,6 +202,7 @@ __device__ __forceinline__ void apply_geom(const T (&j0)[3], const T (&j1)[3],
 // and dx/dzeta is constant along its column.
 template <int N, int Q, typename T, typename TG, bool GEO>
 __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
+__attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
     k_apply3d(const T* __restrict__ x, T* __restrict__ y,
               const uint16_t* __restrict__ pidx,
               const int32_t* __restrict__ bdofs,
@@ -210,6 +245,7 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
   uint16_t li[N];
   const bool own1 = tid < S::EB * S::NN;
   const int e1 = tid / S::NN, r1 = tid - e1 * S::NN;  // r1 = z*N + y
+  const int o1 = (r1 / N) * S::ZA + (r1 % N) * S::YA;  // its x-pencil in A
   const bool act1 = own1 && e1 < ncb;
   if (act1) {
     const uint16_t* lp = pidx + ((base + e1) * S::NN + r1) * N;
@@ -265,7 +301,7 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     T u[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) u[j] = li[j] != kNoDof ? regX[li[j]] : T(0);
-    T* sA = regA + e1 * 2 * S::SA1;
+    T* sA = regA + e1 * S::CA + o1;
     TableRows<T, S::TK> t_(tB, tD);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -276,8 +312,8 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
         bq = fma(t_.b[i * N + j], u[j], bq);
         dq = fma(t_.d[i * N + j], u[j], dq);
       }
-      sA[r1 * S::QP + i] = bq;
-      sA[S::SA1 + r1 * S::QP + i] = dq;
+      sA[i] = bq;
+      sA[S::TA + i] = dq;
     }
   }
   __syncthreads();
@@ -288,13 +324,13 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     const bool on2 = tid < S::EB * S::NQ;
     const int e = on2 ? tid / S::NQ : 0, r = on2 ? tid - e * S::NQ : 0;
     const int z = r / Q, i = r - z * Q;
-    const T* sA = regA + e * 2 * S::SA1;
-    T* sB = regB + e * 3 * S::SB1;
+    const T* sA = regA + e * S::CA + z * S::ZA + i;
+    T* sB = regB + e * S::CB + z * S::ZB + i;
     T a[N], d[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      a[j] = sA[(z * N + j) * S::QP + i];
-      d[j] = sA[S::SA1 + (z * N + j) * S::QP + i];
+      a[j] = sA[j * S::YA];
+      d[j] = sA[S::TA + j * S::YA];
     }
     if constexpr (S::LC) __syncthreads();
     TableRows<T, S::TK> t_(tB, tD);
@@ -309,23 +345,22 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
         bd = fma(t_.d[k * N + j], a[j], bd);
         db = fma(t_.b[k * N + j], d[j], db);
       }
-      const int o = z * S::QQ + k * Q + i;
-      sB[o] = db;               // x-gradient path
-      sB[S::SB1 + o] = bd;      // y-gradient path
-      sB[2 * S::SB1 + o] = bb;  // z-gradient path
+      sB[k * Q] = db;                 // x-gradient path
+      sB[S::TB + k * Q] = bd;         // y-gradient path
+      sB[2 * S::TB + k * Q] = bb;     // z-gradient path
     }
   }
   __syncthreads();
 
   // ---- stage 3: z-columns: gradients, G, transposed z contraction --------
   if (own3) {
-    T* sB = regB + e3 * 3 * S::SB1;
+    T* sB = regB + e3 * S::CB + r3;
     T cx[N], cy[N], cz[N];
 #pragma unroll
     for (int z = 0; z < N; ++z) {
-      cx[z] = sB[z * S::QQ + r3];
-      cy[z] = sB[S::SB1 + z * S::QQ + r3];
-      cz[z] = sB[2 * S::SB1 + z * S::QQ + r3];
+      cx[z] = sB[z * S::ZB];
+      cy[z] = sB[S::TB + z * S::ZB];
+      cz[z] = sB[2 * S::TB + z * S::ZB];
     }
     T tx[Q], ty[Q], tz[Q];
     // GEO: edge vectors of the trilinear map for this quadrature column
@@ -391,9 +426,9 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
         ry = fma(tt_.b[z * Q + l], ty[l], ry);
         rz = fma(tt_.d[z * Q + l], tz[l], rz);
       }
-      sB[z * S::QQ + r3] = rx;
-      sB[S::SB1 + z * S::QQ + r3] = ry;
-      sB[2 * S::SB1 + z * S::QQ + r3] = rz;
+      sB[z * S::ZB] = rx;
+      sB[S::TB + z * S::ZB] = ry;
+      sB[2 * S::TB + z * S::ZB] = rz;
     }
   }
   __syncthreads();
@@ -403,15 +438,14 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
     const bool on4 = tid < S::EB * S::NQ;
     const int e = on4 ? tid / S::NQ : 0, r = on4 ? tid - e * S::NQ : 0;
     const int z = r / Q, i = r - z * Q;
-    T* sA = regA + e * 2 * S::SA1;
-    const T* sB = regB + e * 3 * S::SB1;
+    T* sA = regA + e * S::CA + z * S::ZA + i;
+    const T* sB = regB + e * S::CB + z * S::ZB + i;
     T rx[Q], ry[Q], rz[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-      const int o = z * S::QQ + k * Q + i;
-      rx[k] = sB[o];
-      ry[k] = sB[S::SB1 + o];
-      rz[k] = sB[2 * S::SB1 + o];
+      rx[k] = sB[k * Q];
+      ry[k] = sB[S::TB + k * Q];
+      rz[k] = sB[2 * S::TB + k * Q];
     }
     if constexpr (S::LC) __syncthreads();
     TableRows<T, S::TK> t_(tB + Q * N, tD + Q * N);  // transposed tables
@@ -426,8 +460,8 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
         so = fma(t_.d[j * Q + k], ry[k], so);
         so = fma(t_.b[j * Q + k], rz[k], so);
       }
-      sA[(z * N + j) * S::QP + i] = sx;           // then D_x^T
-      sA[S::SA1 + (z * N + j) * S::QP + i] = so;  // then B_x^T
+      sA[j * S::YA] = sx;           // then D_x^T
+      sA[S::TA + j * S::YA] = so;   // then B_x^T
     }
   }
   __syncthreads();
@@ -443,12 +477,12 @@ __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
 
   // ---- stage 5: transposed x-pencils, assembled in LDS ----------------------
   if (act1) {
-    const T* sA = regA + e1 * 2 * S::SA1;
+    const T* sA = regA + e1 * S::CA + o1;
     T sx[Q], so[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      sx[i] = sA[r1 * S::QP + i];
-      so[i] = sA[S::SA1 + r1 * S::QP + i];
+      sx[i] = sA[i];
+      so[i] = sA[S::TA + i];
     }
     TableRows<T, S::TK> t_(tB + Q * N, tD + Q * N);  // transposed tables
 #pragma unroll
