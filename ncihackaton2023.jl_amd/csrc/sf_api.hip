@@ -223,6 +223,7 @@ struct sf_plan {
   DevBuf<double> w_r, w_p, w_q, w_t;
   DevBuf<float> w_r32, w_s32, w_d32, w_p32, w_q32;
   DevBuf<double> red_part;
+  DevBuf<double> w_part;  // 3D: the apply's per-workgroup shares of p.Ap
   DevBuf<unsigned> red_counter;
   DevBuf<CGScalars> scal;
   DevBuf<double> dscal;  // dot results
@@ -242,7 +243,8 @@ struct sf_plan {
     xs.release(); ys.release();
     w_r.release(); w_p.release(); w_q.release(); w_t.release();
     w_r32.release(); w_s32.release(); w_d32.release(); w_p32.release(); w_q32.release();
-    red_part.release(); red_counter.release(); scal.release(); dscal.release();
+    red_part.release(); w_part.release(); red_counter.release(); scal.release();
+    dscal.release();
     if (own_stream) (void)hipStreamDestroy(own_stream);
     own_stream = nullptr;
   }
@@ -256,7 +258,7 @@ struct sf_plan {
            boff.bytes() + bcell.bytes() +
            xs.bytes() + ys.bytes() + w_r.bytes() + w_p.bytes() + w_q.bytes() +
            w_t.bytes() + w_r32.bytes() + w_s32.bytes() + w_d32.bytes() + w_p32.bytes() +
-           w_q32.bytes() + red_part.bytes() + red_counter.bytes() +
+           w_q32.bytes() + red_part.bytes() + w_part.bytes() + red_counter.bytes() +
            scal.bytes() + dscal.bytes();
   }
 };
@@ -325,14 +327,14 @@ int clear_flag(sf_plan* P) {
 // guarantees y == 0 already (the CG direction update clears it), so the
 // fill is skipped.
 int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
-                 int accumulate = 0, bool zeroed = false) {
+                 int accumulate = 0, bool zeroed = false, double* dotp = nullptr) {
   if (P->nfree == 0) return SF_OK;  // nothing to compute (and x may be empty)
   if (!accumulate && !zeroed) SF_HIP(launch_fill<double>(y, P->nfree, 0.0, s));
   // the fused-geometry kernel when the plan has vertices and no kappa
   const bool geo = P->fused_geom && P->dim == 3 && !P->has_kappa && P->X.p;
   hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
                            P->dB.p, P->dD.p, P->ncell, P->blocks(), accumulate, s,
-                           geo ? P->X.p : nullptr, P->dPts.p, P->dW.p);
+                           geo ? P->X.p : nullptr, P->dPts.p, P->dW.p, dotp);
   if (e == hipErrorNotSupported)
     return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
   SF_HIP(e);
@@ -408,7 +410,7 @@ int prepare(sf_plan* P, int precision) {
 // y = A x in reduced precision (FP32 vectors), y overwritten (zeroed: y
 // is already zero).
 int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
-                 hipStream_t s, bool zeroed = false) {
+                 hipStream_t s, bool zeroed = false, double* dotp = nullptr) {
   if (P->nfree == 0) return SF_OK;
   if (!zeroed) SF_HIP(launch_fill<float>(y, P->nfree, 0.0f, s));
   hipError_t e;
@@ -416,7 +418,7 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
     const bool geo = P->fused_geom && P->dim == 3 && !P->has_kappa && P->X32.p;
     e = apply_f32(P->dim, P->n, P->q, x, y, P->cells.p, P->G32.p, P->dB32.p,
                   P->dD32.p, P->ncell, P->blocks(), 0, s, geo ? P->X32.p : nullptr,
-                  P->Pts32.p, P->W32.p);
+                  P->Pts32.p, P->W32.p, dotp);
   } else if (precision == SF_PREC_F32_GBF16) {
     e = apply_f32_gbf16(P->dim, P->n, P->q, x, y, P->cells.p, P->Gbf.p,
                         P->dB32.p, P->dD32.p, P->ncell, P->blocks(), 0, s);
@@ -553,6 +555,7 @@ int build_blocks(sf_plan* P, const std::vector<int32_t>& h) {
 }
 
 int ensure_solver_ws(sf_plan* P, bool mixed) {
+  if (P->dim == 3 && P->nblk > 0) SF_HIP(P->w_part.alloc(P->nblk));
   SF_HIP(P->w_r.alloc(P->nfree));
   SF_HIP(P->w_p.alloc(P->nfree));
   SF_HIP(P->w_q.alloc(P->nfree));
@@ -630,10 +633,21 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   if (rc) return rc;
   Progress prog("cg");
   const int chunk = 8;
+  // 3D fused-geometry apply: p.Ap from the apply (alpha_parts / resid / xdir)
+  const bool fused = P->dim == 3 && P->nblk > 0 && P->w_part.p && P->fused_geom &&
+                     !P->has_kappa && P->X.p;
   while (!h.done && h.iters < maxit) {
     prog.tick(h.iters, bb > 0 ? sqrt(h.rr / bb) : sqrt(h.rr));
     const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
     for (int k = 0; k < todo; ++k) {
+      if (fused) {
+        rc = do_apply_f64(P, P->w_q.p, P->w_p.p, s, 0, true, P->w_part.p);
+        if (rc) return rc;
+        SF_HIP(launch_cg_alpha_parts(P->w_part.p, P->nblk, red, P->scal.p, s));
+        SF_HIP(launch_cg_resid<double>(P->w_r.p, P->w_q.p, P->nfree, red, P->scal.p, s));
+        SF_HIP(launch_cg_xdir<double>(x, P->w_p.p, P->w_r.p, P->w_q.p, P->nfree, P->scal.p, s));
+        continue;
+      }
       rc = do_apply_f64(P, P->w_q.p, P->w_p.p, s, 0, true);
       if (rc) return rc;
       SF_HIP(launch_cg_alpha<double>(P->w_p.p, P->w_q.p, P->nfree, red, P->scal.p, s));
@@ -847,11 +861,22 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
   if (rc) return rc;
   const int chunk = 8;
   Progress prog("mixed cg (fp64 recurrence)");
+  // 3D FP32 fused-geometry operator: p.Ap from the apply
+  const bool fused = P->dim == 3 && P->nblk > 0 && P->w_part.p && precision == SF_PREC_F32 &&
+                     P->fused_geom && !P->has_kappa && P->X32.p;
   for (;;) {
     while (!h.done && h.iters < maxit) {
       prog.tick(h.iters, bnorm > 0 ? sqrt(h.rr) / bnorm : sqrt(h.rr));
       const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
       for (int k = 0; k < todo; ++k) {
+        if (fused) {
+          rc = do_apply_f32(P, q32, p32, precision, s, true, P->w_part.p);
+          if (rc) return rc;
+          SF_HIP(launch_cg_alpha_parts(P->w_part.p, P->nblk, red, P->scal.p, s));
+          SF_HIP(launch_cg_resid_mixed(r, q32, P->nfree, red, P->scal.p, s));
+          SF_HIP(launch_cg_xdir_mixed(x, p, p32, r, q32, P->nfree, P->scal.p, s));
+          continue;
+        }
         rc = do_apply_f32(P, q32, p32, precision, s, true);
         if (rc) return rc;
         SF_HIP(launch_cg_alpha_mixed(p, q32, P->nfree, red, P->scal.p, s));
@@ -883,6 +908,7 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
     h.rr = rn_new * rn_new;
     h.stop_rr = t * t;
     h.done = 0;
+    h.xpend = 0;
     rc = write_scalars(P, &h, s);
     if (rc) return rc;
   }

@@ -9,16 +9,10 @@ namespace sf {
 
 namespace {
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
 // Block sum over kRedThreads threads; result valid on thread 0.
 __device__ double block_sum(double v) {
   __shared__ double ws[kRedThreads / 64];
-  v = wave_sum(v);
+  v = wave_sum_f64(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __syncthreads();  // ws may still be read by a previous call
   if (lane == 0) ws[wid] = v;
@@ -106,6 +100,7 @@ __global__ void __launch_bounds__(kRedThreads)
       sc->beta = 0.0;
       sc->stop_rr = stop_rr;
       sc->iters = 0;
+      sc->xpend = 0;
       sc->nonfinite = isfinite(rr) ? 0 : 1;
       sc->done = (rr <= stop_rr || !isfinite(rr)) ? 1 : 0;
     }
@@ -166,6 +161,123 @@ __global__ void __launch_bounds__(kRedThreads)
         sc->done = 1;
       }
     }
+  }
+}
+
+// ---- fused CG step (3D): the apply returns per-workgroup shares of p.Ap,
+// so one step is  apply -> alpha_parts -> resid -> xdir  and p is read
+// once outside the apply:
+//   alpha_parts: alpha = rr / sum(parts)                (reads nblk values)
+//   resid      : r -= alpha q, rr_new = r.r, beta       (reads r, q)
+//   xdir       : x += alpha p, p = r + beta p, q = 0    (reads x, p, r)
+// xpend carries "this step's x update is still to come" from resid to
+// xdir, so the step that converges still moves x, and nothing moves it
+// twice.
+__global__ void __launch_bounds__(kRedThreads)
+    k_cg_alpha_parts(const double* __restrict__ part, int64_t np, Reducer red,
+                     CGScalars* __restrict__ sc) {
+  if (sc->done) {  // uniform: written only by other kernels
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->xpend = 0;
+    return;
+  }
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += part[i];
+  s = block_sum(s);
+  if (arrive(red, s)) {
+    const double pq = gather_partials(red);
+    if (threadIdx.x == 0) {
+      sc->xpend = 0;
+      sc->pq = pq;
+      const double a = sc->rr / pq;
+      sc->alpha = a;
+      if (!isfinite(a)) {
+        sc->nonfinite = 1;
+        sc->done = 1;
+      }
+    }
+  }
+}
+
+template <typename V, typename W = V>
+__global__ void __launch_bounds__(kRedThreads)
+    k_cg_resid(V* __restrict__ r, const W* __restrict__ q, int64_t n, Reducer red,
+               CGScalars* __restrict__ sc) {
+  if (sc->done) return;
+  const V a = (V)sc->alpha;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const V ri = fma(-a, (V)q[i], r[i]);
+    r[i] = ri;
+    s += (double)ri * (double)ri;
+  }
+  s = block_sum(s);
+  if (arrive(red, s)) {
+    const double rr_new = gather_partials(red);
+    if (threadIdx.x == 0) {
+      sc->rr_new = rr_new;
+      sc->beta = rr_new / sc->rr;
+      sc->rr = rr_new;
+      sc->iters += 1;
+      sc->xpend = 1;
+      if (!isfinite(rr_new)) {
+        sc->nonfinite = 1;
+        sc->done = 1;
+      } else if (rr_new <= sc->stop_rr) {
+        sc->done = 1;
+      }
+    }
+  }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256)
+    k_cg_xdir(V* __restrict__ x, V* __restrict__ p, const V* __restrict__ r,
+              V* __restrict__ q, int64_t n, const CGScalars* __restrict__ sc) {
+  if (!sc->xpend) return;
+  const V a = (V)sc->alpha;
+  if (sc->done) {  // the converging step: x only
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+      x[i] = fma(a, p[i], x[i]);
+    return;
+  }
+  const V b = (V)sc->beta;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const V pi = p[i];
+    x[i] = fma(a, pi, x[i]);
+    p[i] = fma(b, pi, r[i]);
+    q[i] = V(0);
+  }
+}
+
+// The same with an FP32 operator: also the FP32 copy of p for the next
+// apply, and q32 = 0.
+__global__ void __launch_bounds__(256)
+    k_cg_xdir_mixed(double* __restrict__ x, double* __restrict__ p,
+                    float* __restrict__ p32, const double* __restrict__ r,
+                    float* __restrict__ q32, int64_t n,
+                    const CGScalars* __restrict__ sc) {
+  if (!sc->xpend) return;
+  const double a = sc->alpha;
+  if (sc->done) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+      x[i] = fma(a, p[i], x[i]);
+    return;
+  }
+  const double b = sc->beta;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double pi = p[i];
+    x[i] = fma(a, pi, x[i]);
+    const double pn = fma(b, pi, r[i]);
+    p[i] = pn;
+    p32[i] = (float)pn;
+    q32[i] = 0.f;
   }
 }
 
@@ -336,6 +448,45 @@ hipError_t launch_cg_dir(V* p, const V* r, V* q, int64_t n, CGScalars* sc,
   return hipGetLastError();
 }
 
+hipError_t launch_cg_alpha_parts(const double* part, int64_t np, Reducer red,
+                                 CGScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_alpha_parts, dim3(red_grid(np)), dim3(kRedThreads), 0, s,
+                     part, np, red, sc);
+  return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_cg_resid(V* r, const V* q, int64_t n, Reducer red, CGScalars* sc,
+                           hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_resid<V, V>), dim3(red_grid(n)), dim3(kRedThreads), 0, s,
+                     r, q, n, red, sc);
+  return hipGetLastError();
+}
+
+template <typename V>
+hipError_t launch_cg_xdir(V* x, V* p, const V* r, V* q, int64_t n, CGScalars* sc,
+                          hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cg_xdir<V>, dim3(stream_grid(n)), dim3(256), 0, s, x, p, r, q,
+                     n, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_resid_mixed(double* r, const float* q, int64_t n, Reducer red,
+                                 CGScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL((k_cg_resid<double, float>), dim3(red_grid(n)), dim3(kRedThreads), 0,
+                     s, r, q, n, red, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_xdir_mixed(double* x, double* p, float* p32, const double* r,
+                                float* q32, int64_t n, CGScalars* sc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cg_xdir_mixed, dim3(stream_grid(n)), dim3(256), 0, s, x, p, p32,
+                     r, q32, n, sc);
+  return hipGetLastError();
+}
+
 hipError_t launch_cg_alpha_mixed(const double* p, const float* q, int64_t n,
                                  Reducer red, CGScalars* sc, hipStream_t s) {
   if (n <= 0) return hipSuccess;
@@ -415,7 +566,11 @@ hipError_t launch_axpy_f32_to_f64(double* x, const float* d32, double scale,
                                           int64_t, Reducer, CGScalars*,       \
                                           hipStream_t);                       \
   template hipError_t launch_cg_dir<V>(V*, const V*, V*, int64_t,             \
-                                       CGScalars*, hipStream_t);
+                                       CGScalars*, hipStream_t);              \
+  template hipError_t launch_cg_resid<V>(V*, const V*, int64_t, Reducer,      \
+                                         CGScalars*, hipStream_t);            \
+  template hipError_t launch_cg_xdir<V>(V*, V*, const V*, V*, int64_t,        \
+                                        CGScalars*, hipStream_t);
 SF_INST(double)
 SF_INST(float)
 #undef SF_INST

@@ -151,4 +151,11 @@ __device__ __forceinline__ void atomic_add(float* p, float v) {
   unsafeAtomicAdd(p, v);
 }
 
+// Sum over the 64 lanes of a wave (valid in lane 0).
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
 }  // namespace sf

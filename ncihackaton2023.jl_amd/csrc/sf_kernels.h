@@ -43,16 +43,19 @@ hipError_t launch_to_bf16(const double* a, int64_t n, bf16s* b, hipStream_t s);
 // 2D (k == nullptr): y += A x by atomics (the caller zeroes y for y = A x).
 // 3D with Xv (cell vertices, 24 per cell) and the 1D points / weights tP,
 // tW: the geometric factor is recomputed in the kernel instead of read.
+// 3D with dotp: per-workgroup shares of x.Ax into dotp[0 .. k->nblk).
 hipError_t apply_f64(int dim, int n, int q, const double* x, double* y,
                      const int32_t* cells, const double* G, const double* tB,
                      const double* tD, int64_t ncell, const BlockArgs* k,
                      int accumulate, hipStream_t s, const double* Xv = nullptr,
-                     const double* tP = nullptr, const double* tW = nullptr);
+                     const double* tP = nullptr, const double* tW = nullptr,
+                     double* dotp = nullptr);
 hipError_t apply_f32(int dim, int n, int q, const float* x, float* y,
                      const int32_t* cells, const float* G, const float* tB,
                      const float* tD, int64_t ncell, const BlockArgs* k,
                      int accumulate, hipStream_t s, const float* Xv = nullptr,
-                     const float* tP = nullptr, const float* tW = nullptr);
+                     const float* tP = nullptr, const float* tW = nullptr,
+                     double* dotp = nullptr);
 hipError_t apply_f32_gbf16(int dim, int n, int q, const float* x, float* y,
                            const int32_t* cells, const bf16s* G,
                            const float* tB, const float* tD, int64_t ncell,
@@ -90,7 +93,8 @@ struct CGScalars {
   int done;        // 1 once rr <= stop_rr (later updates are no-ops)
   int iters;       // iterations taken while not done
   int nonfinite;   // a NaN / Inf was met
-  int pad;
+  int xpend;       // fused step: x += alpha p still to apply (set by the
+                   // residual update, cleared by the next alpha)
 };
 
 struct Reducer {
@@ -141,6 +145,24 @@ hipError_t launch_residual(const double* b, const double* Ax, double* r,
 // r32 = r * inv_scale
 hipError_t launch_scale_to_f32(const double* r, double inv_scale, float* r32,
                                int64_t n, hipStream_t s);
+// Fused CG step (3D; the apply writes per-workgroup shares of p.Ap):
+//   alpha_parts: alpha = rr / sum(part[0 .. np)), clears xpend
+//   resid      : r -= alpha q, rr_new = r.r, beta, done; sets xpend
+//   xdir       : if xpend: x += alpha p; unless done: p = r + beta p, q = 0
+hipError_t launch_cg_alpha_parts(const double* part, int64_t np, Reducer red,
+                                 CGScalars* sc, hipStream_t s);
+template <typename V>
+hipError_t launch_cg_resid(V* r, const V* q, int64_t n, Reducer red, CGScalars* sc,
+                           hipStream_t s);
+template <typename V>
+hipError_t launch_cg_xdir(V* x, V* p, const V* r, V* q, int64_t n, CGScalars* sc,
+                          hipStream_t s);
+// ... with FP64 vectors and an FP32 operator (q32 = A32 p32; xdir also
+// refreshes p32 = (float)p)
+hipError_t launch_cg_resid_mixed(double* r, const float* q, int64_t n, Reducer red,
+                                 CGScalars* sc, hipStream_t s);
+hipError_t launch_cg_xdir_mixed(double* x, double* p, float* p32, const double* r,
+                                float* q32, int64_t n, CGScalars* sc, hipStream_t s);
 // FP64 CG vectors with an FP32 operator (q = A32 p32):
 //   alpha_mixed : pq = p . q32, alpha = rr / pq
 //   update_mixed: x += alpha p, r -= alpha q32, rr_new = r.r, beta, done

@@ -53,7 +53,7 @@ def cg_ir_(x, A, b, rtol=1e-10, inner="f32", inner_rtol=1e-3, inner_maxiter=None
 
 
 def cg_mixed_(x, A, b, rtol=1e-10, inner="f32", delta=0.1, maxiter=None,
-              recurrence="f32", raise_on_fail=False):
+              recurrence="f64", raise_on_fail=False):
     """Mixed-precision CG with reliable updates: one CG recurrence on the
     reduced-precision operator (recurrence 'f32': FP32 vectors; 'f64': FP64
     vectors, only the operator reduced), residual replaced by the FP64

@@ -204,6 +204,55 @@ def test_cg_maxiter_and_zero_rhs(pkg, cuda):
     A.free()
 
 
+def test_cg_fused_step_matches_unfused(pkg, oracle_mod, cuda, monkeypatch):
+    """3D CG with the fused-geometry apply takes p.Ap from the apply (energy
+    form) and fuses x += alpha p into the direction update; the stored-G
+    plan runs the unfused kernels.  Same iterations (to rounding) and the
+    same solution; in 2D (unfused) CG also reaches the oracle residual."""
+    say("cg fused vs unfused step")
+    model, space, mp, A = make(pkg, 3, 6, 4, 6)
+    b = cuda.from_numpy(pkg.normal(5, space.n_free)).cuda()
+    xf = cuda.zeros_like(b)
+    sf = pkg.cg_(xf, A, b, rtol=1e-10, maxiter=20000)
+    A.free()
+    monkeypatch.setenv("SFHIP_FUSED_GEOMETRY", "0")
+    model, space, mp, A = make(pkg, 3, 6, 4, 6)
+    xu = cuda.zeros_like(b)
+    su = pkg.cg_(xu, A, b, rtol=1e-10, maxiter=20000)
+    A.free()
+    monkeypatch.delenv("SFHIP_FUSED_GEOMETRY")
+    say("fused", sf, "unfused", su)
+    assert sf["converged"] == 1 and su["converged"] == 1
+    assert abs(sf["iterations"] - su["iterations"]) <= max(3, 0.01 * su["iterations"])
+    bh = b.cpu().numpy()
+    for xx in (xf, xu):
+        r = bh - oracle_mod.apply(3, 5, 6, space.cell_dofs, model.cell_vertices(),
+                                  xx.cpu().numpy(), mp.tables)
+        assert np.linalg.norm(r) / np.linalg.norm(bh) < 2e-10
+    assert rel_l2(xf.cpu().numpy(), xu.cpu().numpy()) < 1e-8
+    # a capped fused run still applies the last step's x update
+    x3 = cuda.zeros_like(b)
+    model, space, mp, A = make(pkg, 3, 6, 4, 6)
+    s3 = pkg.cg_(x3, A, b, rtol=1e-14, maxiter=1)
+    assert s3["iterations"] == 1 and float(x3.abs().max()) > 0.0
+    # one CG step from x = 0: x = (r.r / r.Ar) r with r = b
+    Ab = oracle_mod.apply(3, 5, 6, space.cell_dofs, model.cell_vertices(), bh, mp.tables)
+    x1 = (bh @ bh) / (bh @ Ab) * bh
+    assert rel_l2(x3.cpu().numpy(), x1) < 1e-12
+    A.free()
+    # 2D: no blocks, unfused kernels
+    model, space, mp, A = make(pkg, 2, 12, 4, 6)
+    b2 = cuda.from_numpy(pkg.normal(6, space.n_free)).cuda()
+    x2 = cuda.zeros_like(b2)
+    s2 = pkg.cg_(x2, A, b2, rtol=1e-10, maxiter=20000)
+    assert s2["converged"] == 1
+    b2h = b2.cpu().numpy()
+    r2 = b2h - oracle_mod.apply(2, 5, 6, space.cell_dofs, model.cell_vertices(),
+                                x2.cpu().numpy(), mp.tables)
+    assert np.linalg.norm(r2) / np.linalg.norm(b2h) < 2e-10
+    A.free()
+
+
 @pytest.mark.parametrize("inner", ["f32", "f32_gbf16", "mfma_bf16x3"])
 def test_mixed_precision_refinement(pkg, oracle_mod, cuda, inner):
     say("IR inner=%s" % inner)
@@ -239,7 +288,7 @@ def test_mixed_cg_reliable_updates(pkg, oracle_mod, cuda, inner):
     s64 = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=20000)
     assert s64["converged"] == 1
     x = cuda.zeros_like(b)
-    st = pkg.cg_mixed_(x, A, b, rtol=1e-10, inner=inner, delta=0.1)
+    st = pkg.cg_mixed_(x, A, b, rtol=1e-10, inner=inner, delta=0.1, recurrence="f32")
     say("f64", s64, "mixed", st)
     assert st["converged"] == 1 and st["rel_residual"] <= 1e-10
     assert st["outer_iterations"] >= 5  # 1e-10 in steps of at most 0.1
@@ -250,7 +299,7 @@ def test_mixed_cg_reliable_updates(pkg, oracle_mod, cuda, inner):
     assert np.linalg.norm(r) / np.linalg.norm(bh) < 2e-10
     assert rel_l2(xh, x64.cpu().numpy()) < 1e-7
     # a converged start returns at once
-    st2 = pkg.cg_mixed_(x, A, b, rtol=1e-9, inner=inner)
+    st2 = pkg.cg_mixed_(x, A, b, rtol=1e-9, inner=inner, recurrence="f32")
     assert st2["iterations"] == 0 and st2["converged"] == 1
     A.free()
 
