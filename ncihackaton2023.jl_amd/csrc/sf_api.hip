@@ -213,6 +213,7 @@ struct sf_plan {
   DevBuf<int> flag;
   // 3D block structure (block-assembled apply)
   int eb = 0, nblk = 0;
+  int64_t npart = 0;  // shares of p.Ap the 3D apply writes: one per wave
   DevBuf<uint16_t> pidx;
   DevBuf<int32_t> bdofs, bexcl, cell_block;
   DevBuf<int64_t> boff, bcell;
@@ -223,7 +224,7 @@ struct sf_plan {
   DevBuf<double> w_r, w_p, w_q, w_t;
   DevBuf<float> w_r32, w_s32, w_d32, w_p32, w_q32;
   DevBuf<double> red_part;
-  DevBuf<double> w_part;  // 3D: the apply's per-workgroup shares of p.Ap
+  DevBuf<double> w_part;  // 3D: the apply's per-wave shares of p.Ap
   DevBuf<unsigned> red_counter;
   DevBuf<CGScalars> scal;
   DevBuf<double> dscal;  // dot results
@@ -550,12 +551,13 @@ int build_blocks(sf_plan* P, const std::vector<int32_t>& h) {
   if (E > 0) SF_HIP(hipMemcpy(P->cell_block.p, cblk.data(), E * 4, hipMemcpyHostToDevice));
   P->eb = eb;
   P->nblk = (int)nb;
+  P->npart = nb * (nt / 64);
   P->kargs = BlockArgs{P->pidx.p, P->bdofs.p, P->boff.p, P->bexcl.p, P->bcell.p, (int)nb};
   return SF_OK;
 }
 
 int ensure_solver_ws(sf_plan* P, bool mixed) {
-  if (P->dim == 3 && P->nblk > 0) SF_HIP(P->w_part.alloc(P->nblk));
+  if (P->dim == 3 && P->npart > 0) SF_HIP(P->w_part.alloc(P->npart));
   SF_HIP(P->w_r.alloc(P->nfree));
   SF_HIP(P->w_p.alloc(P->nfree));
   SF_HIP(P->w_q.alloc(P->nfree));
@@ -634,7 +636,7 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   Progress prog("cg");
   const int chunk = 8;
   // 3D fused-geometry apply: p.Ap from the apply (alpha_parts / resid / xdir)
-  const bool fused = P->dim == 3 && P->nblk > 0 && P->w_part.p && P->fused_geom &&
+  const bool fused = P->dim == 3 && P->npart > 0 && P->w_part.p && P->fused_geom &&
                      !P->has_kappa && P->X.p;
   while (!h.done && h.iters < maxit) {
     prog.tick(h.iters, bb > 0 ? sqrt(h.rr / bb) : sqrt(h.rr));
@@ -643,7 +645,7 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
       if (fused) {
         rc = do_apply_f64(P, P->w_q.p, P->w_p.p, s, 0, true, P->w_part.p);
         if (rc) return rc;
-        SF_HIP(launch_cg_alpha_parts(P->w_part.p, P->nblk, red, P->scal.p, s));
+        SF_HIP(launch_cg_alpha_parts(P->w_part.p, P->npart, red, P->scal.p, s));
         SF_HIP(launch_cg_resid<double>(P->w_r.p, P->w_q.p, P->nfree, red, P->scal.p, s));
         SF_HIP(launch_cg_xdir<double>(x, P->w_p.p, P->w_r.p, P->w_q.p, P->nfree, P->scal.p, s));
         continue;
@@ -862,7 +864,7 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
   const int chunk = 8;
   Progress prog("mixed cg (fp64 recurrence)");
   // 3D FP32 fused-geometry operator: p.Ap from the apply
-  const bool fused = P->dim == 3 && P->nblk > 0 && P->w_part.p && precision == SF_PREC_F32 &&
+  const bool fused = P->dim == 3 && P->npart > 0 && P->w_part.p && precision == SF_PREC_F32 &&
                      P->fused_geom && !P->has_kappa && P->X32.p;
   for (;;) {
     while (!h.done && h.iters < maxit) {
@@ -872,7 +874,7 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
         if (fused) {
           rc = do_apply_f32(P, q32, p32, precision, s, true, P->w_part.p);
           if (rc) return rc;
-          SF_HIP(launch_cg_alpha_parts(P->w_part.p, P->nblk, red, P->scal.p, s));
+          SF_HIP(launch_cg_alpha_parts(P->w_part.p, P->npart, red, P->scal.p, s));
           SF_HIP(launch_cg_resid_mixed(r, q32, P->nfree, red, P->scal.p, s));
           SF_HIP(launch_cg_xdir_mixed(x, p, p32, r, q32, P->nfree, P->scal.p, s));
           continue;

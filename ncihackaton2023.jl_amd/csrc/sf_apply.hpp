@@ -152,11 +152,11 @@ __device__ __forceinline__ float recip_pos(float a) {
 // t = G g at one quadrature point from the columns of J, with G = w / |det J|
 // C^T C (C the cofactor matrix; the geometry kernel's formula) applied as
 // s C^T (C g): 9 + 9 FMAs and one scaling instead of forming the six G
-// entries.  en = g . t = s |C g|^2, the point's energy density.
+// entries.
 template <typename T>
 __device__ __forceinline__ void apply_geom(const T (&j0)[3], const T (&j1)[3],
                                            const T (&j2)[3], T w, T g0, T g1, T g2,
-                                           T& t0, T& t1, T& t2, T& en) {
+                                           T& t0, T& t1, T& t2) {
   T C[3][3];
   C[0][0] = j1[1] * j2[2] - j2[1] * j1[2];
   C[0][1] = j2[1] * j0[2] - j0[1] * j2[2];
@@ -175,7 +175,6 @@ __device__ __forceinline__ void apply_geom(const T (&j0)[3], const T (&j1)[3],
   t0 = sc * (C[0][0] * v0 + C[1][0] * v1 + C[2][0] * v2);
   t1 = sc * (C[0][1] * v0 + C[1][1] * v1 + C[2][1] * v2);
   t2 = sc * (C[0][2] * v0 + C[1][2] * v1 + C[2][2] * v2);
-  en = sc * (v0 * v0 + v1 * v1 + v2 * v2);
 }
 
 // Block-assembled 3D apply, one launch over all cell blocks.
@@ -202,10 +201,12 @@ __device__ __forceinline__ void apply_geom(const T (&j0)[3], const T (&j1)[3],
 // dx/dxi and dx/deta linearly in zeta between edge vectors it forms once,
 // and dx/dzeta is constant along its column.
 //
-// DOT (with GEO): workgroup b also writes dotp[b] = its cells' share of x.Ax,
-// the sum over their quadrature points of grad u . (G grad u) (the energy
-// form, equal to x.Ax: Dirichlet dofs enter as zero), so a CG step gets
-// p.Ap without reading p and q again.
+// DOT (with GEO): wave w of workgroup b also writes dotp[b * NT / 64 + w],
+// its share of x.Ax = sum over blocks of x . (the block's contribution):
+// in the write-back each lane adds x_d times the assembled value of its
+// distinct dofs d (FP32: x kept in registers from the prologue; FP64: x
+// re-read, which costs fewer registers).  A CG step then gets p.Ap without
+// reading p and q again.
 template <int N, int Q, typename T, typename TG, bool GEO, bool DOT>
 __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
 __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
@@ -221,7 +222,6 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
               double* __restrict__ dotp) {
   using S = Apply3Shape<N, Q, T>;
   __shared__ T smem[S::RC + (GEO ? S::EB * 24 : 0)];
-  __shared__ double wdot[DOT ? S::NT / 64 : 1];  // per-wave shares of x.Ax
   T* const regV = smem + S::RC;  // GEO: the block's cell vertices
   T* const regA = smem;                               // x / y^T pass out
   T* const regB = S::LC ? smem : smem + S::RA;        // y pass out, z in place
@@ -361,7 +361,6 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
   __syncthreads();
 
   // ---- stage 3: z-columns: gradients, G, transposed z contraction --------
-  double ep = 0.0;  // this column's share of x.Ax (DOT)
   if (own3) {
     T* sB = regB + e3 * S::CB + r3;
     T cx[N], cy[N], cz[N];
@@ -417,9 +416,7 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
           j0[a] = fma(ze, dE[a], E0[a]);
           j1[a] = fma(ze, dF[a], F0[a]);
         }
-        T en;
-        apply_geom<T>(j0, j1, H, wik * tW[l], gx, gy, gz, tx[l], ty[l], tz[l], en);
-        if constexpr (DOT) ep += (double)en;
+        apply_geom<T>(j0, j1, H, wik * tW[l], gx, gy, gz, tx[l], ty[l], tz[l]);
       } else {
         tx[l] = gq[0][l] * gx + gq[1][l] * gy + gq[2][l] * gz;
         ty[l] = gq[1][l] * gx + gq[3][l] * gy + gq[4][l] * gz;
@@ -441,11 +438,6 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
       sB[S::TB + z * S::ZB] = ry;
       sB[2 * S::TB + z * S::ZB] = rz;
     }
-  }
-  if constexpr (DOT) {
-    // lanes of cells past ncb ran on unset LDS: their values are dropped
-    ep = wave_sum_f64(act3 ? ep : 0.0);
-    if ((tid & 63) == 0) wdot[tid >> 6] = ep;
   }
   __syncthreads();
 
@@ -515,19 +507,20 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
   }
   __syncthreads();
 
-  if (DOT && tid == 0) {
-    double s = 0.0;
-#pragma unroll
-    for (int w = 0; w < S::NT / 64; ++w) s += wdot[w];
-    dotp[b] = s;
-  }
-
   // ---- write-back: one global access per distinct dof of the block --------
+  T xr[DOT ? S::KU : 1];
+  if constexpr (DOT) {
+#pragma unroll
+    for (int k = 0; k < S::KU; ++k)
+      xr[k] = sizeof(T) == 4 ? xv[k] : (dv[k] >= 0 ? x[dv[k]] : T(0));
+  }
+  double ep = 0.0;
 #pragma unroll
   for (int k = 0; k < S::KU; ++k) {
     const int i = tid + k * S::NT;
     if (dv[k] < 0) continue;
     const T v = regY[i];
+    if constexpr (DOT) ep += (double)xr[k] * (double)v;
     if (i < nexcl) {
       if (accumulate)
         y[dv[k]] += v;
@@ -536,6 +529,10 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
     } else {
       atomic_add(y + dv[k], v);
     }
+  }
+  if constexpr (DOT) {
+    ep = wave_sum_f64(ep);
+    if ((tid & 63) == 0) dotp[(int64_t)b * (S::NT / 64) + (tid >> 6)] = ep;
   }
 }
 
@@ -696,7 +693,7 @@ struct ApplyArgs {
   const T* Xv = nullptr;
   const T* tP = nullptr;
   const T* tW = nullptr;
-  // 3D: per-workgroup shares of x.Ax (nblk values), or none
+  // 3D with Xv: shares of x.Ax, one per wave (nblk * NT / 64 values), or none
   double* dotp = nullptr;
 };
 

@@ -164,10 +164,10 @@ __global__ void __launch_bounds__(kRedThreads)
   }
 }
 
-// ---- fused CG step (3D): the apply returns per-workgroup shares of p.Ap,
+// ---- fused CG step (3D): the apply returns per-wave shares of p.Ap,
 // so one step is  apply -> alpha_parts -> resid -> xdir  and p is read
 // once outside the apply:
-//   alpha_parts: alpha = rr / sum(parts)                (reads nblk values)
+//   alpha_parts: alpha = rr / sum(parts)     (one value per wave of the apply)
 //   resid      : r -= alpha q, rr_new = r.r, beta       (reads r, q)
 //   xdir       : x += alpha p, p = r + beta p, q = 0    (reads x, p, r)
 // xpend carries "this step's x update is still to come" from resid to

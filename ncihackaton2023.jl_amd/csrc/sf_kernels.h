@@ -43,7 +43,8 @@ hipError_t launch_to_bf16(const double* a, int64_t n, bf16s* b, hipStream_t s);
 // 2D (k == nullptr): y += A x by atomics (the caller zeroes y for y = A x).
 // 3D with Xv (cell vertices, 24 per cell) and the 1D points / weights tP,
 // tW: the geometric factor is recomputed in the kernel instead of read.
-// 3D with dotp: per-workgroup shares of x.Ax into dotp[0 .. k->nblk).
+// 3D with dotp (and Xv): shares of x.Ax, one per wave, into
+// dotp[0 .. k->nblk * NT / 64) (NT from apply_shape_f64).
 hipError_t apply_f64(int dim, int n, int q, const double* x, double* y,
                      const int32_t* cells, const double* G, const double* tB,
                      const double* tD, int64_t ncell, const BlockArgs* k,
@@ -145,7 +146,7 @@ hipError_t launch_residual(const double* b, const double* Ax, double* r,
 // r32 = r * inv_scale
 hipError_t launch_scale_to_f32(const double* r, double inv_scale, float* r32,
                                int64_t n, hipStream_t s);
-// Fused CG step (3D; the apply writes per-workgroup shares of p.Ap):
+// Fused CG step (3D; the apply writes per-wave shares of p.Ap):
 //   alpha_parts: alpha = rr / sum(part[0 .. np)), clears xpend
 //   resid      : r -= alpha q, rr_new = r.r, beta, done; sets xpend
 //   xdir       : if xpend: x += alpha p; unless done: p = r + beta p, q = 0

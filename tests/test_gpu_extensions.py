@@ -205,8 +205,9 @@ def test_cg_maxiter_and_zero_rhs(pkg, cuda):
 
 
 def test_cg_fused_step_matches_unfused(pkg, oracle_mod, cuda, monkeypatch):
-    """3D CG with the fused-geometry apply takes p.Ap from the apply (energy
-    form) and fuses x += alpha p into the direction update; the stored-G
+    """3D CG with the fused-geometry apply takes p.Ap from the apply (per-wave
+    shares from its write-back) and fuses x += alpha p into the direction
+    update; the stored-G
     plan runs the unfused kernels.  Same iterations (to rounding) and the
     same solution; in 2D (unfused) CG also reaches the oracle residual."""
     say("cg fused vs unfused step")
