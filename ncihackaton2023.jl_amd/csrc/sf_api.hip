@@ -570,7 +570,7 @@ int ensure_solver_ws(sf_plan* P, bool mixed) {
     SF_HIP(P->w_q32.alloc(P->nfree));
   }
   if (!P->red_part.p) {
-    SF_HIP(P->red_part.alloc(kRedBlocks));
+    SF_HIP(P->red_part.alloc(kPartSlots));
     SF_HIP(P->red_counter.alloc(1));
     SF_HIP(hipMemset(P->red_counter.p, 0, sizeof(unsigned)));
     SF_HIP(P->scal.alloc(1));

@@ -107,68 +107,67 @@ __global__ void __launch_bounds__(kRedThreads)
   }
 }
 
+// p.q on the streaming grid, per-workgroup shares (finished by
+// k_cg_alpha_finish; see k_cg_resid for why there is no arrival counter).
 template <typename V, typename W = V>
-__global__ void __launch_bounds__(kRedThreads)
+__global__ void __launch_bounds__(256)
     k_cg_alpha(const V* __restrict__ p, const W* __restrict__ q, int64_t n,
-               Reducer red, CGScalars* __restrict__ sc) {
+               double* __restrict__ part, const CGScalars* __restrict__ sc) {
   if (sc->done) return;  // uniform over the grid: set only by other kernels
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     s += (double)p[i] * (double)q[i];
   s = block_sum(s);
-  if (arrive(red, s)) {
-    const double pq = gather_partials(red);
-    if (threadIdx.x == 0) {
-      sc->pq = pq;
-      const double a = sc->rr / pq;
-      sc->alpha = a;
-      if (!isfinite(a)) {
-        sc->nonfinite = 1;
-        sc->done = 1;
-      }
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// One workgroup: pq = sum of the np shares, alpha = rr / pq.
+__global__ void __launch_bounds__(kRedThreads)
+    k_cg_alpha_finish(const double* __restrict__ part, int np,
+                      CGScalars* __restrict__ sc) {
+  if (sc->done) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    sc->pq = s;
+    const double a = sc->rr / s;
+    sc->alpha = a;
+    if (!isfinite(a)) {
+      sc->nonfinite = 1;
+      sc->done = 1;
     }
   }
 }
 
+// x += alpha p, r -= alpha q on the streaming grid, per-workgroup shares of
+// r.r (finished by k_cg_resid_finish).
 template <typename V, typename W = V>
-__global__ void __launch_bounds__(kRedThreads)
+__global__ void __launch_bounds__(256)
     k_cg_update(V* __restrict__ x, V* __restrict__ r,
                 const V* __restrict__ p, const W* __restrict__ q, int64_t n,
-                Reducer red, CGScalars* __restrict__ sc) {
+                double* __restrict__ part, const CGScalars* __restrict__ sc) {
   if (sc->done) return;
   const V a = (V)sc->alpha;
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    x[i] = fma(a, p[i], x[i]);
+    __builtin_nontemporal_store(fma(a, p[i], x[i]), x + i);
     const V ri = fma(-a, (V)q[i], r[i]);
-    r[i] = ri;
+    __builtin_nontemporal_store(ri, r + i);
     s += (double)ri * (double)ri;
   }
   s = block_sum(s);
-  if (arrive(red, s)) {
-    const double rr_new = gather_partials(red);
-    if (threadIdx.x == 0) {
-      sc->rr_new = rr_new;
-      sc->beta = rr_new / sc->rr;
-      sc->rr = rr_new;
-      sc->iters += 1;
-      if (!isfinite(rr_new)) {
-        sc->nonfinite = 1;
-        sc->done = 1;
-      } else if (rr_new <= sc->stop_rr) {
-        sc->done = 1;
-      }
-    }
-  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // ---- fused CG step (3D): the apply returns per-wave shares of p.Ap,
 // so one step is  apply -> alpha_parts -> resid -> xdir  and p is read
 // once outside the apply:
 //   alpha_parts: alpha = rr / sum(parts)     (one value per wave of the apply)
-//   resid      : r -= alpha q, rr_new = r.r, beta       (reads r, q)
+//   resid      : r -= alpha q, rr_new = r.r, beta       (reads r, q; two
+//                launches: per-workgroup shares, then a one-workgroup sum)
 //   xdir       : x += alpha p, p = r + beta p, q = 0    (reads x, p, r)
 // xpend carries "this step's x update is still to come" from resid to
 // xdir, so the step that converges still moves x, and nothing moves it
@@ -200,34 +199,47 @@ __global__ void __launch_bounds__(kRedThreads)
   }
 }
 
+// r -= alpha q on the streaming grid; each workgroup stores its share of
+// r.r (plain store: the finish kernel runs after the grid has ended, so no
+// fence or arrival counter -- on this chip each such fence waits for the
+// L2 to write back the r lines just stored).
 template <typename V, typename W = V>
-__global__ void __launch_bounds__(kRedThreads)
-    k_cg_resid(V* __restrict__ r, const W* __restrict__ q, int64_t n, Reducer red,
-               CGScalars* __restrict__ sc) {
+__global__ void __launch_bounds__(256)
+    k_cg_resid(V* __restrict__ r, const W* __restrict__ q, int64_t n,
+               double* __restrict__ part, const CGScalars* __restrict__ sc) {
   if (sc->done) return;
   const V a = (V)sc->alpha;
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const V ri = fma(-a, (V)q[i], r[i]);
-    r[i] = ri;
+    __builtin_nontemporal_store(ri, r + i);
     s += (double)ri * (double)ri;
   }
   s = block_sum(s);
-  if (arrive(red, s)) {
-    const double rr_new = gather_partials(red);
-    if (threadIdx.x == 0) {
-      sc->rr_new = rr_new;
-      sc->beta = rr_new / sc->rr;
-      sc->rr = rr_new;
-      sc->iters += 1;
-      sc->xpend = 1;
-      if (!isfinite(rr_new)) {
-        sc->nonfinite = 1;
-        sc->done = 1;
-      } else if (rr_new <= sc->stop_rr) {
-        sc->done = 1;
-      }
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// One workgroup: rr_new = sum of the np shares, beta, the stop test.
+__global__ void __launch_bounds__(kRedThreads)
+    k_cg_resid_finish(const double* __restrict__ part, int np,
+                      CGScalars* __restrict__ sc) {
+  if (sc->done) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    const double rr_new = s;
+    sc->rr_new = rr_new;
+    sc->beta = rr_new / sc->rr;
+    sc->rr = rr_new;
+    sc->iters += 1;
+    sc->xpend = 1;
+    if (!isfinite(rr_new)) {
+      sc->nonfinite = 1;
+      sc->done = 1;
+    } else if (rr_new <= sc->stop_rr) {
+      sc->done = 1;
     }
   }
 }
@@ -375,7 +387,7 @@ inline unsigned stream_grid(int64_t n) {
   // Enough workgroups to fill 256 CUs several times over; grid-stride loops
   // cover the rest.
   int64_t g = (n + 255) / 256;
-  if (g > 4096) g = 4096;
+  if (g > kStreamBlocks) g = kStreamBlocks;
   if (g < 1) g = 1;
   return (unsigned)g;
 }
@@ -423,20 +435,41 @@ hipError_t launch_cg_start(const V* b, const V* Ax, V* r, V* p, V* q,
   return hipGetLastError();
 }
 
+template <typename V, typename W>
+hipError_t alpha_two_phase(const V* p, const W* q, int64_t n, Reducer red, CGScalars* sc,
+                           hipStream_t s) {
+  const unsigned g = stream_grid(n);
+  hipLaunchKernelGGL((k_cg_alpha<V, W>), dim3(g), dim3(256), 0, s, p, q, n, red.part, sc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cg_alpha_finish, dim3(1), dim3(kRedThreads), 0, s, red.part,
+                     (int)g, sc);
+  return hipGetLastError();
+}
+
+template <typename V, typename W>
+hipError_t update_two_phase(V* x, V* r, const V* p, const W* q, int64_t n, Reducer red,
+                            CGScalars* sc, hipStream_t s) {
+  const unsigned g = stream_grid(n);
+  hipLaunchKernelGGL((k_cg_update<V, W>), dim3(g), dim3(256), 0, s, x, r, p, q, n,
+                     red.part, sc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cg_resid_finish, dim3(1), dim3(kRedThreads), 0, s, red.part,
+                     (int)g, sc);
+  return hipGetLastError();
+}
+
 template <typename V>
 hipError_t launch_cg_alpha(const V* p, const V* q, int64_t n, Reducer red,
                            CGScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_alpha<V>, dim3(red_grid(n)), dim3(kRedThreads), 0, s,
-                     p, q, n, red, sc);
-  return hipGetLastError();
+  return alpha_two_phase<V, V>(p, q, n, red, sc, s);
 }
 
 template <typename V>
 hipError_t launch_cg_update(V* x, V* r, const V* p, const V* q, int64_t n,
                             Reducer red, CGScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_update<V>, dim3(red_grid(n)), dim3(kRedThreads), 0,
-                     s, x, r, p, q, n, red, sc);
-  return hipGetLastError();
+  return update_two_phase<V, V>(x, r, p, q, n, red, sc, s);
 }
 
 template <typename V>
@@ -455,12 +488,22 @@ hipError_t launch_cg_alpha_parts(const double* part, int64_t np, Reducer red,
   return hipGetLastError();
 }
 
+template <typename V, typename W>
+hipError_t resid_two_phase(V* r, const W* q, int64_t n, Reducer red, CGScalars* sc,
+                           hipStream_t s) {
+  const unsigned g = stream_grid(n);
+  hipLaunchKernelGGL((k_cg_resid<V, W>), dim3(g), dim3(256), 0, s, r, q, n, red.part, sc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cg_resid_finish, dim3(1), dim3(kRedThreads), 0, s, red.part,
+                     (int)g, sc);
+  return hipGetLastError();
+}
+
 template <typename V>
 hipError_t launch_cg_resid(V* r, const V* q, int64_t n, Reducer red, CGScalars* sc,
                            hipStream_t s) {
-  hipLaunchKernelGGL((k_cg_resid<V, V>), dim3(red_grid(n)), dim3(kRedThreads), 0, s,
-                     r, q, n, red, sc);
-  return hipGetLastError();
+  return resid_two_phase<V, V>(r, q, n, red, sc, s);
 }
 
 template <typename V>
@@ -474,9 +517,7 @@ hipError_t launch_cg_xdir(V* x, V* p, const V* r, V* q, int64_t n, CGScalars* sc
 
 hipError_t launch_cg_resid_mixed(double* r, const float* q, int64_t n, Reducer red,
                                  CGScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL((k_cg_resid<double, float>), dim3(red_grid(n)), dim3(kRedThreads), 0,
-                     s, r, q, n, red, sc);
-  return hipGetLastError();
+  return resid_two_phase<double, float>(r, q, n, red, sc, s);
 }
 
 hipError_t launch_cg_xdir_mixed(double* x, double* p, float* p32, const double* r,
@@ -489,18 +530,12 @@ hipError_t launch_cg_xdir_mixed(double* x, double* p, float* p32, const double* 
 
 hipError_t launch_cg_alpha_mixed(const double* p, const float* q, int64_t n,
                                  Reducer red, CGScalars* sc, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_cg_alpha<double, float>), dim3(red_grid(n)), dim3(kRedThreads), 0, s,
-                     p, q, n, red, sc);
-  return hipGetLastError();
+  return alpha_two_phase<double, float>(p, q, n, red, sc, s);
 }
 
 hipError_t launch_cg_update_mixed(double* x, double* r, const double* p, const float* q,
                                   int64_t n, Reducer red, CGScalars* sc, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_cg_update<double, float>), dim3(red_grid(n)), dim3(kRedThreads), 0, s,
-                     x, r, p, q, n, red, sc);
-  return hipGetLastError();
+  return update_two_phase<double, float>(x, r, p, q, n, red, sc, s);
 }
 
 hipError_t launch_cg_dir_mixed(double* p, float* p32, const double* r, float* q32,

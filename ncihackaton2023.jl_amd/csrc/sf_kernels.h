@@ -99,11 +99,13 @@ struct CGScalars {
 };
 
 struct Reducer {
-  double* part;        // kRedBlocks partial sums
+  double* part;        // kPartSlots partial sums
   unsigned* counter;   // zero at rest, reset by the last block
 };
 
-constexpr int kRedBlocks = 1024;
+constexpr int kRedBlocks = 1024;     // grid of the single-pass reductions
+constexpr int kStreamBlocks = 4096;  // grid of the streaming kernels
+constexpr int kPartSlots = kStreamBlocks;
 constexpr int kRedThreads = 256;
 
 template <typename V>
