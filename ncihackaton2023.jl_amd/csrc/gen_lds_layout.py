@@ -15,7 +15,9 @@ ds_read_b64 loops): 64 banks of 4 bytes, 16 lanes served per pass, a pass
 costs as many cycles as the most distinct addresses any bank receives.
 For every (N, Q) the script picks (YA, ZA, CA, ZB, CB) minimising the
 cycles of those accesses, keeping the cells per workgroup and the number
-of workgroups per CU that the LDS allows, and never above 64 KiB.
+of workgroups per CU that the LDS allows (from N = 5 on, at most the 12
+waves per CU that the kernels' register use admits), and never above
+64 KiB.
 
     python3 gen_lds_layout.py > sf_lds_layout.h
 """
@@ -41,11 +43,12 @@ def old_shape(N, Q):
     NN, NQ, QQ, NNN = N * N, N * Q, Q * Q, N * N * N
     QP = Q + 1 if Q % 2 == 0 else Q
     SA1, SB1 = NN * QP, N * QQ
-    LC = N >= 6
+    LC = N >= 5
     SC = max(3 * SB1, 2 * SA1 + NNN) if LC else 2 * SA1 + 3 * SB1
     MAXT = max(NN, NQ, QQ)
     EB = min(1 if MAXT > 100 else max(1, 256 // MAXT), max(1, 65536 // (SC * 8)))
-    return dict(YA=QP, ZA=N * QP, CA=2 * SA1, ZB=QQ, CB=3 * SB1, EB=EB, LC=LC)
+    NT = (EB * MAXT + 63) // 64 * 64
+    return dict(YA=QP, ZA=N * QP, CA=2 * SA1, ZB=QQ, CB=3 * SB1, EB=EB, LC=LC, NT=NT)
 
 
 def lds_bytes(N, Q, L):
@@ -89,6 +92,10 @@ def best(N, Q):
     EB = base['EB']
     b0 = lds_bytes(N, Q, base)
     wg0 = min(LDS_CU // rup(b0, GRANULE), 8)
+    if N >= 5:
+        # the FP64 kernels use more than 128 VGPRs from N = 5 on: at most 3
+        # waves per SIMD (12 per CU) run, whatever the LDS allows
+        wg0 = min(wg0, max(1, 12 // (base['NT'] // 64)))
     # A and B are scored independently; the joint LDS limit is checked after
     As = []
     for YA in range(Q, Q + 4):

@@ -27,13 +27,13 @@
 // (gen_lds_layout.py scores every stage's lane pattern against the 64-bit
 // bank model of gfx950: 64 banks of 4 bytes, 16 lanes per pass); otherwise
 // dense tensors with the x-pencil rows padded to an odd length QP.
-// From N = 6 on, A and B share one region: the y pass and the y^T pass
+// From N = 5 on, A and B share one region: the y pass and the y^T pass
 // read their input pencils into registers, cross a barrier, then overwrite
 // them, so a workgroup holds max(B, A + x values) instead of A + B (more
 // cells per workgroup and more workgroups per CU -- the occupancy is what
-// hides the gather and G latencies at high order).  Below N = 6 the LDS is
-// not what limits the occupancy (the registers are) and the two extra
-// barriers would only cost.
+// hides the gather latency at high order; at N = 5 measured 2% faster).
+// Below N = 5 the LDS is not what limits the occupancy (the registers are)
+// and the two extra barriers would only cost.
 #pragma once
 
 #include "sf_common.hpp"
@@ -67,9 +67,9 @@ struct Apply3Shape {
   static constexpr int SB1 = N * QQ;   // one y-pass tensor [z][k][i]
   static constexpr int SE = 2 * SA1 + 3 * SB1;
   static constexpr int MAXT = cmax(NN, cmax(NQ, QQ));
-  // shared A / B region (N >= 6): LDS per cell is the y-pass output, or
+  // shared A / B region (N >= 5): LDS per cell is the y-pass output, or
   // the x-pass output next to the cell's x values
-  static constexpr bool LC = N >= 6;
+  static constexpr bool LC = N >= 5;
   static constexpr int SC = LC ? cmax(3 * SB1, 2 * SA1 + NNN) : SE;
   // cells per workgroup: fill ~256 lanes in the widest stage, within 64 KiB
   // of FP64 LDS (the same EB for every T, so FP64 and FP32 applies share the

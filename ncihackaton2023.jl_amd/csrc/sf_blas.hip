@@ -253,16 +253,16 @@ __global__ void __launch_bounds__(256)
   if (sc->done) {  // the converging step: x only
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-      x[i] = fma(a, p[i], x[i]);
+      __builtin_nontemporal_store(fma(a, p[i], x[i]), x + i);
     return;
   }
   const V b = (V)sc->beta;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const V pi = p[i];
-    x[i] = fma(a, pi, x[i]);
-    p[i] = fma(b, pi, r[i]);
-    q[i] = V(0);
+    __builtin_nontemporal_store(fma(a, pi, x[i]), x + i);
+    __builtin_nontemporal_store(fma(b, pi, r[i]), p + i);
+    __builtin_nontemporal_store(V(0), q + i);
   }
 }
 
@@ -278,18 +278,18 @@ __global__ void __launch_bounds__(256)
   if (sc->done) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-      x[i] = fma(a, p[i], x[i]);
+      __builtin_nontemporal_store(fma(a, p[i], x[i]), x + i);
     return;
   }
   const double b = sc->beta;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const double pi = p[i];
-    x[i] = fma(a, pi, x[i]);
+    __builtin_nontemporal_store(fma(a, pi, x[i]), x + i);
     const double pn = fma(b, pi, r[i]);
-    p[i] = pn;
-    p32[i] = (float)pn;
-    q32[i] = 0.f;
+    __builtin_nontemporal_store(pn, p + i);
+    __builtin_nontemporal_store((float)pn, p32 + i);
+    __builtin_nontemporal_store(0.f, q32 + i);
   }
 }
 
