@@ -13,6 +13,8 @@ lane patterns (lane = thread id, consecutive tasks on consecutive lanes):
 Bank model of 64-bit LDS accesses on gfx950 (measured with strided
 ds_read_b64 loops): 64 banks of 4 bytes, 16 lanes served per pass, a pass
 costs as many cycles as the most distinct addresses any bank receives.
+The shared A / B region is assumed from N = 5 on (the kernel's stored-G
+form at N = 5 splits them and keeps the dense layout).
 For every (N, Q) the script picks (YA, ZA, CA, ZB, CB) minimising the
 cycles of those accesses, keeping the cells per workgroup and the number
 of workgroups per CU that the LDS allows (from N = 5 on, at most the 12

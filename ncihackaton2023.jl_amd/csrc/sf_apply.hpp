@@ -27,13 +27,14 @@
 // (gen_lds_layout.py scores every stage's lane pattern against the 64-bit
 // bank model of gfx950: 64 banks of 4 bytes, 16 lanes per pass); otherwise
 // dense tensors with the x-pencil rows padded to an odd length QP.
-// From N = 5 on, A and B share one region: the y pass and the y^T pass
-// read their input pencils into registers, cross a barrier, then overwrite
-// them, so a workgroup holds max(B, A + x values) instead of A + B (more
-// cells per workgroup and more workgroups per CU -- the occupancy is what
-// hides the gather latency at high order; at N = 5 measured 2% faster).
-// Below N = 5 the LDS is not what limits the occupancy (the registers are)
-// and the two extra barriers would only cost.
+// From N = 6 on (N = 5 in the fused-geometry kernel), A and B share one
+// region: the y pass and the y^T pass read their input pencils into
+// registers, cross a barrier, then overwrite them, so a workgroup holds
+// max(B, A + x values) instead of A + B (more cells per workgroup and more
+// workgroups per CU -- the occupancy is what hides the gather latency at
+// high order; at N = 5 measured 2-4% faster with the fused geometry, 10%
+// slower with the stored G).  Below that the LDS is not what limits the
+// occupancy (the registers are) and the two extra barriers would only cost.
 #pragma once
 
 #include "sf_common.hpp"
@@ -58,7 +59,7 @@ struct LdsLayout {
 SF_LDS_LAYOUT_F64(SF_LDS_ROW)
 #undef SF_LDS_ROW
 
-template <int N, int Q, typename T>
+template <int N, int Q, typename T, bool GEO = true>
 struct Apply3Shape {
   static constexpr int NN = N * N, NQ = N * Q, QQ = Q * Q, QQQ = Q * Q * Q;
   static constexpr int NNN = N * N * N;
@@ -67,9 +68,11 @@ struct Apply3Shape {
   static constexpr int SB1 = N * QQ;   // one y-pass tensor [z][k][i]
   static constexpr int SE = 2 * SA1 + 3 * SB1;
   static constexpr int MAXT = cmax(NN, cmax(NQ, QQ));
-  // shared A / B region (N >= 5): LDS per cell is the y-pass output, or
-  // the x-pass output next to the cell's x values
-  static constexpr bool LC = N >= 5;
+  // shared A / B region (N >= 6, and N = 5 with the fused geometry; the
+  // stored-G kernel at N = 5 is faster without the extra barriers): LDS per
+  // cell is the y-pass output, or the x-pass output next to the cell's x
+  // values
+  static constexpr bool LC = N >= 6 || (N == 5 && GEO);
   static constexpr int SC = LC ? cmax(3 * SB1, 2 * SA1 + NNN) : SE;
   // cells per workgroup: fill ~256 lanes in the widest stage, within 64 KiB
   // of FP64 LDS (the same EB for every T, so FP64 and FP32 applies share the
@@ -89,8 +92,10 @@ struct Apply3Shape {
   // compiler's own choice was as fast or faster.
   static constexpr int WPE = (sizeof(T) == 8 && (N == 7 || (N == 8 && Q <= 9))) ? 3 : 1;
   // the apply kernel's LDS strides (SA1 / SB1 / QP above size EB and serve
-  // the adjoint kernel's dense layout)
-  using L = LdsLayout<N, Q, (int)sizeof(T)>;
+  // the adjoint kernel's dense layout).  The FP64 table assumes the shared
+  // region from N = 5 on; the stored-G kernel at N = 5 (split regions)
+  // keeps the dense layout, which fits three workgroups per CU.
+  using L = LdsLayout<N, Q, (LC || N < 5) ? (int)sizeof(T) : 0>;
   static constexpr int YA = L::YA, ZA = L::ZA, TA = N * ZA, CA = L::CA;
   static constexpr int ZB = L::ZB, TB = N * ZB, CB = L::CB;
   static_assert(YA >= Q && ZA >= N * YA && CA >= 2 * TA && ZB >= QQ && CB >= 3 * TB,
@@ -220,7 +225,9 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
               const T* __restrict__ tD, const T* __restrict__ Xv,
               const T* __restrict__ tP, const T* __restrict__ tW, int accumulate,
               double* __restrict__ dotp) {
-  using S = Apply3Shape<N, Q, T>;
+  using S = Apply3Shape<N, Q, T, GEO>;
+  static_assert(S::EB == Apply3Shape<N, Q, T>::EB && S::NT == Apply3Shape<N, Q, T>::NT,
+                "apply3d: one block structure for both forms");
   __shared__ T smem[S::RC + (GEO ? S::EB * 24 : 0)];
   T* const regV = smem + S::RC;  // GEO: the block's cell vertices
   T* const regA = smem;                               // x / y^T pass out
