@@ -4,8 +4,9 @@
 Default (the contract line): 3D unit cube, 64^3 hexahedra with interior
 vertices perturbed by U[-0.2h, 0.2h] (seed 1), Q4 (5 nodes/dir, 16,581,375
 free dofs), 6 Gauss points/dir, FP64 apply y = A u with u ~ U[-1,1] (seed 0)
-resident in HBM.  A step is one y = A u over all cells: one dispatch of the
-apply kernel per colour of cell blocks, no fill.  Prints one JSON line with
+resident in HBM.  A step is one y = A u over all cells: the zero fill of y
+(its shared dofs receive atomic adds) and one dispatch of the apply kernel.
+Prints one JSON line with
 GDOF/s, the roofline of the apply (HIP events on its stream around each
 apply) and the CPU oracle timed on the host cores.
 
@@ -68,8 +69,8 @@ def build_problem(pkg, D, m, p, q, perturb=0.2, kappa=None):
 
 def time_apply(torch, A, x, y, steps, warmup, reduced=None):
     """Wall time per step and mean apply time from HIP events recorded on
-    the launch stream around each y = A x (all colour-phase dispatches of
-    the 3D apply, no separate fill)."""
+    the launch stream around each y = A x (the fill of y and the apply
+    dispatch)."""
     stream = torch.cuda.current_stream()
 
     def step():
