@@ -327,42 +327,68 @@ def mode_cg(args, torch, pkg):
 
 
 def mode_adjoint(args, torch, pkg):
+    """Differentiation workload: 3D 48^3 perturbed hexes, Q5, 7 Gauss
+    pts/dir, log-normal kappa per quadrature point.  The pullback of
+    J = 1/2 |A(kappa) u - d|^2 (dJ/dkappa and dJ/du = A(kappa) r) by the
+    block-assembled fused-geometry kernel (the default) and by the per-cell
+    stored-G kernel (a plan built with SFHIP_FUSED_GEOMETRY=0), and the
+    kappa apply in both forms."""
+    import ctypes
     import numpy as np
     D, m, p, q = 3, 48, 5, 7
+    n = p + 1
     log("setting up 3D 48^3 Q5 q=7 perturbed with kappa ...")
     model = pkg.CartesianModel(D, m, perturb=0.2, seed=1)
     space = pkg.FESpace(model, p)
     mp = pkg.SumFactorizationMap.with_points(D, p, q)
     nq = q ** D
-    kappa = np.exp(pkg.normal(3, space.n_cells * nq))
-    A = pkg.LazyMatrix(mp, space, kappa=kappa)
-    u = torch.from_numpy(pkg.uniform(0, space.n_free, -1, 1)).cuda()
-    d = torch.from_numpy(pkg.uniform(4, space.n_free, -1, 1)).cuda()
-    y = torch.empty_like(u)
-    A.mul(y, u)
-    r = y - d
-    kb = torch.empty(space.n_cells * nq, dtype=torch.float64, device="cuda")
-    ub = torch.empty_like(u)
-    import ctypes
-    for _ in range(args.warmup):
-        pkg._lib.check(A._L.sf_adjoint(A.plan, ctypes.c_void_p(u.data_ptr()),
-                                       ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(kb.data_ptr()),
-                                       ctypes.c_void_p(ub.data_ptr()), None))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        pkg._lib.check(A._L.sf_adjoint(A.plan, ctypes.c_void_p(u.data_ptr()),
-                                       ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(kb.data_ptr()),
-                                       ctypes.c_void_p(ub.data_ptr()), None))
-    torch.cuda.synchronize()
-    t = (time.perf_counter() - t0) / args.steps
-    n = p + 1
-    Badj = (8 * 3 * space.n_free + 4 * n ** 3 * space.n_cells
-            + 8 * (6 + 1 + 1) * nq * space.n_cells)
-    A.free()
-    return {"metric": "adjoint (dJ/dkappa + dJ/du) 3D 48^3 Q5 q=7", "ms": t * 1e3,
-            "GB/s": Badj / t / 1e9, "frac_hbm": Badj / t / 1e9 / HBM_PEAK_GBS,
-            "n_free": space.n_free, "n_gpus": 1}
+    nf, E = space.n_free, space.n_cells
+    kappa = np.exp(pkg.normal(3, E * nq))
+    u = torch.from_numpy(pkg.uniform(0, nf, -1, 1)).cuda()
+    d = torch.from_numpy(pkg.uniform(4, nf, -1, 1)).cuda()
+    res = {"metric": "adjoint (dJ/dkappa + dJ/du) 3D 48^3 Q5 q=7, log-normal kappa",
+           "n_free": nf, "n_cells": E, "n_gpus": 1}
+    outs = {}
+    for form in ("fused", "stored_g"):
+        if form == "stored_g":
+            os.environ["SFHIP_FUSED_GEOMETRY"] = "0"
+        A = pkg.LazyMatrix(mp, space, kappa=kappa)
+        os.environ.pop("SFHIP_FUSED_GEOMETRY", None)
+        y = torch.empty_like(u)
+        wall, kern = time_apply(torch, A, u, y, args.steps, args.warmup)
+        r = y - d
+        kb = torch.empty(E * nq, dtype=torch.float64, device="cuda")
+        ub = torch.empty_like(u)
+        call = lambda: pkg._lib.check(A._L.sf_adjoint(
+            A.plan, ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(r.data_ptr()),
+            ctypes.c_void_p(kb.data_ptr()), ctypes.c_void_p(ub.data_ptr()), None))
+        for _ in range(args.warmup):
+            call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            call()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / args.steps
+        if form == "fused":
+            # u, r read and dJ/du written; cell dofs; 24 vertex coordinates
+            # per cell; kappa read and dJ/dkappa written per quadrature point
+            B = 8 * 3 * nf + 4 * n ** 3 * E + 8 * 24 * E + 8 * 2 * nq * E
+        else:
+            # the same with the six G entries per quadrature point
+            B = 8 * 3 * nf + 4 * n ** 3 * E + 8 * (6 + 1 + 1) * nq * E
+        res[form] = {"adjoint_ms": t * 1e3, "alg_bytes": B, "GB/s": B / t / 1e9,
+                     "frac_hbm": B / t / 1e9 / HBM_PEAK_GBS,
+                     "kappa_apply_kernel_ms": kern * 1e3,
+                     "adjoint_over_apply": t / kern}
+        outs[form] = (y.clone(), kb.clone(), ub.clone())
+        log(form, res[form])
+        A.free()
+    for i, name in enumerate(("Au", "dJdkappa", "dJdu")):
+        a, b = outs["fused"][i], outs["stored_g"][i]
+        res["rel_l2_fused_vs_stored_" + name] = float(torch.linalg.norm(a - b)
+                                                      / torch.linalg.norm(b))
+    return res
 
 
 def mode_q9(args, torch, pkg):
@@ -444,8 +470,10 @@ def mode_q9(args, torch, pkg):
         call()
     torch.cuda.synchronize()
     ta = (time.perf_counter() - t0) / 10
-    Badj = 8 * 3 * nf + 4 * n ** 3 * E + 8 * (6 + 1 + 1) * nq * E
-    res["adjoint"] = {"ms": ta * 1e3, "hbm_frac": Badj / ta / 1e9 / HBM_PEAK_GBS}
+    # fused-geometry adjoint: vectors, cell dofs, vertices, kappa, dJ/dkappa
+    Badj = 8 * 3 * nf + 4 * n ** 3 * E + 8 * 24 * E + 8 * 2 * nq * E
+    res["adjoint"] = {"ms": ta * 1e3, "hbm_frac": Badj / ta / 1e9 / HBM_PEAK_GBS,
+                      "over_f64_apply": ta / kern}
     log("adjoint", res["adjoint"])
     A.free()
     return res

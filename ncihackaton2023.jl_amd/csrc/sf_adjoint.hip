@@ -428,6 +428,440 @@ __global__ void __launch_bounds__((Adj2Shape<N, Q>::NT))
   }
 }
 
+// ---------------------------------------------------------------------------
+// Block-assembled 3D adjoint with the geometry recomputed from the cell
+// vertices (the fused-geometry apply's design, sf_apply.hpp: same cell
+// blocks, distinct-dof gather, LDS assembly and exclusive-store / atomic
+// write-back).  At every quadrature point, with C the cofactor matrix of J
+// and s = w / |det J|:
+//   G grad = s C^T (C grad),  so  dJdk = grad(lam) . G grad(u)
+//                                      = s (C grad(lam)) . (C grad(u))
+//   and the lam-path flux is kappa s C^T (C grad(lam)).
+// u and lam run through the x and y passes side by side (twice the apply's
+// LDS per cell, so half the workgroups per CU); the z-column pass forms both
+// gradients, writes dJdk (non-temporal, one value per quadrature point) and
+// contracts only the lam flux back.  WANT_Y = false stops after the z pass.
+template <int N, int Q>
+struct AdjB3Shape {
+  using AS = Apply3Shape<N, Q, double>;
+  static constexpr int EB = AS::EB, NT = AS::NT, KU = AS::KU, TK = AS::TK;
+  static constexpr int NN = N * N, NQ = N * Q, QQ = Q * Q, QQQ = Q * Q * Q;
+  static constexpr int NNN = N * N * N;
+  static constexpr int QP = (Q % 2 == 0) ? Q + 1 : Q;
+  // x-pass outputs [t][z][y][i], t = (B u, D u, B lam, D lam)
+  static constexpr int YA = QP, ZA = N * QP, TA = N * ZA, CA = 4 * TA;
+  // y-pass outputs [t][z][k][i], t = (u: x, y, z paths; lam: x, y, z paths)
+  static constexpr int ZB = QQ, TB = N * QQ, CB = 6 * TB;
+  static constexpr int RA = EB * CA;
+  static constexpr int RX = EB * NNN;  // distinct dofs of a block, at most
+  // one region: x-pass outputs with the u and lam values behind them; then
+  // the y-pass outputs over both; then the y^T-pass output (two tensors per
+  // cell slot) with the assembled values at the end
+  static constexpr int RC = cmax(EB * CB, RA + 2 * RX);
+  static_assert((RC + EB * 24) * 8 <= 163840, "adjoint3d: LDS above 160 KiB");
+  static_assert(NT >= EB * cmax(NN, cmax(NQ, QQ)), "one task per thread per stage");
+};
+
+template <int N, int Q, bool KAP, bool WANT_Y>
+__global__ void __launch_bounds__((AdjB3Shape<N, Q>::NT))
+__attribute__((amdgpu_waves_per_eu(2)))
+    k_adjoint3d_blk(const double* __restrict__ u, const double* __restrict__ lam,
+                    double* __restrict__ dJdk, double* __restrict__ ybar,
+                    const uint16_t* __restrict__ pidx,
+                    const int32_t* __restrict__ bdofs,
+                    const int64_t* __restrict__ boff,
+                    const int32_t* __restrict__ bexcl,
+                    const int64_t* __restrict__ bcell, int nblk,
+                    const double* __restrict__ tB, const double* __restrict__ tD,
+                    const double* __restrict__ Xv, const double* __restrict__ tP,
+                    const double* __restrict__ tW, const double* __restrict__ kap) {
+  using S = AdjB3Shape<N, Q>;
+  __shared__ double smem[S::RC + S::EB * 24];
+  double* const regV = smem + S::RC;          // the block's cell vertices
+  double* const regA = smem;                  // x / y^T pass outputs
+  double* const regB = smem;                  // y pass outputs, z^T in place
+  double* const regXu = smem + S::RA;         // the block's u values
+  double* const regXl = smem + S::RA + S::RX; // ... and lam values
+  double* const regY = smem + (S::RC - S::RX);  // assembled A(kappa) lam
+  const int b = xcd_remap(blockIdx.x, nblk);
+  const int64_t base = bcell[b];
+  const int ncb = (int)(bcell[b + 1] - base);
+  const int tid = threadIdx.x;
+  const int64_t d0 = boff[b];
+  const int nu = (int)(boff[b + 1] - d0);
+  const int nexcl = bexcl[b];
+  const int32_t* __restrict__ dofs = bdofs + d0;
+
+  // ---- prologue: dof ids, u and lam values, local indices, vertices, kappa
+  int32_t dv[S::KU];
+  double uv[S::KU], lv[S::KU];
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    dv[k] = i < nu ? dofs[i] : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int j = dv[k] >= 0 ? dv[k] : 0;
+    uv[k] = u[j];
+    lv[k] = lam[j];
+  }
+  uint16_t li[N];
+  const bool own1 = tid < S::EB * S::NN;
+  const int e1 = tid / S::NN, r1 = tid - e1 * S::NN;  // r1 = z*N + y
+  const int o1 = (r1 / N) * S::ZA + (r1 % N) * S::YA;
+  const bool act1 = own1 && e1 < ncb;
+  if (act1) {
+    const uint16_t* lp = pidx + ((base + e1) * S::NN + r1) * N;
+#pragma unroll
+    for (int j = 0; j < N; ++j) li[j] = lp[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) li[j] = kNoDof;
+  }
+  const bool own3 = tid < S::EB * S::QQ;
+  const int e3 = tid / S::QQ, r3 = tid - e3 * S::QQ;  // r3 = k*Q + i
+  const bool act3 = own3 && e3 < ncb;
+  if (tid < ncb * 24) regV[tid] = Xv[base * 24 + tid];
+  double kq[KAP ? Q : 1];
+  if constexpr (KAP) {
+    const double* kp = kap + (base + (act3 ? e3 : 0)) * S::QQQ + (act3 ? r3 : 0);
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+      kq[l] = act3 ? __builtin_nontemporal_load(kp + l * S::QQ) : 0.0;
+  }
+
+  // ---- stage 0: the block's distinct u and lam values into LDS ----------
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    if (i < nu) {
+      regXu[i] = uv[k];
+      regXl[i] = lv[k];
+    }
+  }
+  __syncthreads();
+
+  // ---- stage 1: x-pencils of u and lam ----------------------------------
+  if (own1) {
+    double uu[N], ll[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      uu[j] = li[j] != kNoDof ? regXu[li[j]] : 0.0;
+      ll[j] = li[j] != kNoDof ? regXl[li[j]] : 0.0;
+    }
+    double* sA = regA + e1 * S::CA + o1;
+    TableRows<double, S::TK> t_(tB, tD);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      t_.at(i);
+      double bu = 0, du = 0, bl = 0, dl = 0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        bu = fma(t_.b[i * N + j], uu[j], bu);
+        du = fma(t_.d[i * N + j], uu[j], du);
+        bl = fma(t_.b[i * N + j], ll[j], bl);
+        dl = fma(t_.d[i * N + j], ll[j], dl);
+      }
+      sA[i] = bu;
+      sA[S::TA + i] = du;
+      sA[2 * S::TA + i] = bl;
+      sA[3 * S::TA + i] = dl;
+    }
+  }
+  __syncthreads();
+
+  // ---- stage 2: y-pencils of both (into registers, barrier, over A) -------
+  {
+    const bool on2 = tid < S::EB * S::NQ;
+    const int e = on2 ? tid / S::NQ : 0, r = on2 ? tid - e * S::NQ : 0;
+    const int z = r / Q, i = r - z * Q;
+    const double* sA = regA + e * S::CA + z * S::ZA + i;
+    double* sB = regB + e * S::CB + z * S::ZB + i;
+    double au[N], du[N], al[N], dl[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      au[j] = sA[j * S::YA];
+      du[j] = sA[S::TA + j * S::YA];
+      al[j] = sA[2 * S::TA + j * S::YA];
+      dl[j] = sA[3 * S::TA + j * S::YA];
+    }
+    __syncthreads();
+    TableRows<double, S::TK> t_(tB, tD);
+    if (on2)
+#pragma unroll
+      for (int k = 0; k < Q; ++k) {
+        t_.at(k);
+        double ubb = 0, ubd = 0, udb = 0, lbb = 0, lbd = 0, ldb = 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          ubb = fma(t_.b[k * N + j], au[j], ubb);
+          ubd = fma(t_.d[k * N + j], au[j], ubd);
+          udb = fma(t_.b[k * N + j], du[j], udb);
+          lbb = fma(t_.b[k * N + j], al[j], lbb);
+          lbd = fma(t_.d[k * N + j], al[j], lbd);
+          ldb = fma(t_.b[k * N + j], dl[j], ldb);
+        }
+        sB[k * Q] = udb;                 // u: x-gradient path
+        sB[S::TB + k * Q] = ubd;         // u: y-gradient path
+        sB[2 * S::TB + k * Q] = ubb;     // u: z-gradient path
+        sB[3 * S::TB + k * Q] = ldb;     // lam: x
+        sB[4 * S::TB + k * Q] = lbd;     // lam: y
+        sB[5 * S::TB + k * Q] = lbb;     // lam: z
+      }
+  }
+  __syncthreads();
+
+  // ---- stage 3: z-columns: both gradients, dJ/dkappa, the lam flux --------
+  // grad u at the column's Q points first (its z pencils then die), then the
+  // lam pencils; the lam flux overwrites grad u point by point, so the
+  // column holds 3Q + 3N values at a time, as in the apply
+  if (own3) {
+    double* sB = regB + e3 * S::CB + r3;
+    double g0[Q], g1[Q], g2[Q];
+    {
+      double cx[N], cy[N], cz[N];
+#pragma unroll
+      for (int z = 0; z < N; ++z) {
+        cx[z] = sB[z * S::ZB];
+        cy[z] = sB[S::TB + z * S::ZB];
+        cz[z] = sB[2 * S::TB + z * S::ZB];
+      }
+      TableRows<double, S::TK> t_(tB, tD);
+#pragma unroll
+      for (int l = 0; l < Q; ++l) {
+        t_.at(l);
+        double a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+        for (int z = 0; z < N; ++z) {
+          a0 = fma(t_.b[l * N + z], cx[z], a0);
+          a1 = fma(t_.b[l * N + z], cy[z], a1);
+          a2 = fma(t_.d[l * N + z], cz[z], a2);
+        }
+        g0[l] = a0;
+        g1[l] = a1;
+        g2[l] = a2;
+      }
+    }
+    // pin grad u here: otherwise its FMAs are sunk into the point loop
+    // below, which then keeps the u pencils live next to the lam pencils
+    // and the flux (about twice the registers)
+#pragma unroll
+    for (int l = 0; l < Q; ++l) asm volatile("" : "+v"(g0[l]), "+v"(g1[l]), "+v"(g2[l]));
+    double lx[N], ly[N], lz[N];
+#pragma unroll
+    for (int z = 0; z < N; ++z) {
+      lx[z] = sB[3 * S::TB + z * S::ZB];
+      ly[z] = sB[4 * S::TB + z * S::ZB];
+      lz[z] = sB[5 * S::TB + z * S::ZB];
+    }
+    // edge vectors of the trilinear map along quadrature column (i, k), as
+    // in the fused-geometry apply
+    double E0[3], dE[3], F0[3], dF[3], H[3];
+    const int qi = r3 % Q, qk = r3 / Q;
+    const double xi = tP[qi], et = tP[qk];
+    const double wik = tW[qi] * tW[qk];
+    const double* V = regV + e3 * 24;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double e0 = (1 - et) * (V[3 + a] - V[a]) + et * (V[9 + a] - V[6 + a]);
+      const double e1 = (1 - et) * (V[15 + a] - V[12 + a]) + et * (V[21 + a] - V[18 + a]);
+      const double f0 = (1 - xi) * (V[6 + a] - V[a]) + xi * (V[9 + a] - V[3 + a]);
+      const double f1 = (1 - xi) * (V[18 + a] - V[12 + a]) + xi * (V[21 + a] - V[15 + a]);
+      const double h0 = (1 - xi) * (V[12 + a] - V[a]) + xi * (V[15 + a] - V[3 + a]);
+      const double h1 = (1 - xi) * (V[18 + a] - V[6 + a]) + xi * (V[21 + a] - V[9 + a]);
+      E0[a] = e0;
+      dE[a] = e1 - e0;
+      F0[a] = f0;
+      dF[a] = f1 - f0;
+      H[a] = (1 - et) * h0 + et * h1;
+    }
+    double* kout = dJdk + (base + (act3 ? e3 : 0)) * S::QQQ + r3;
+    TableRows<double, S::TK> t_(tB, tD);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      t_.at(l);
+      double gl0 = 0, gl1 = 0, gl2 = 0;
+#pragma unroll
+      for (int z = 0; z < N; ++z) {
+        gl0 = fma(t_.b[l * N + z], lx[z], gl0);
+        gl1 = fma(t_.b[l * N + z], ly[z], gl1);
+        gl2 = fma(t_.d[l * N + z], lz[z], gl2);
+      }
+      const double ze = tP[l];
+      double j0[3], j1[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        j0[a] = fma(ze, dE[a], E0[a]);
+        j1[a] = fma(ze, dF[a], F0[a]);
+      }
+      const double* j2 = H;
+      double C[3][3];
+      C[0][0] = j1[1] * j2[2] - j2[1] * j1[2];
+      C[0][1] = j2[1] * j0[2] - j0[1] * j2[2];
+      C[0][2] = j0[1] * j1[2] - j1[1] * j0[2];
+      C[1][0] = j2[0] * j1[2] - j1[0] * j2[2];
+      C[1][1] = j0[0] * j2[2] - j2[0] * j0[2];
+      C[1][2] = j1[0] * j0[2] - j0[0] * j1[2];
+      C[2][0] = j1[0] * j2[1] - j2[0] * j1[1];
+      C[2][1] = j2[0] * j0[1] - j0[0] * j2[1];
+      C[2][2] = j0[0] * j1[1] - j1[0] * j0[1];
+      const double det = j0[0] * C[0][0] + j1[0] * C[0][1] + j2[0] * C[0][2];
+      const double sc = wik * tW[l] * recip_pos(fabs(det));
+      const double vu0 = C[0][0] * g0[l] + C[0][1] * g1[l] + C[0][2] * g2[l];
+      const double vu1 = C[1][0] * g0[l] + C[1][1] * g1[l] + C[1][2] * g2[l];
+      const double vu2 = C[2][0] * g0[l] + C[2][1] * g1[l] + C[2][2] * g2[l];
+      const double vl0 = C[0][0] * gl0 + C[0][1] * gl1 + C[0][2] * gl2;
+      const double vl1 = C[1][0] * gl0 + C[1][1] * gl1 + C[1][2] * gl2;
+      const double vl2 = C[2][0] * gl0 + C[2][1] * gl1 + C[2][2] * gl2;
+      if (act3)
+        __builtin_nontemporal_store(sc * (vl0 * vu0 + vl1 * vu1 + vl2 * vu2),
+                                    kout + l * S::QQ);
+      if constexpr (WANT_Y) {
+        const double sk = KAP ? sc * kq[l] : sc;
+        g0[l] = sk * (C[0][0] * vl0 + C[1][0] * vl1 + C[2][0] * vl2);
+        g1[l] = sk * (C[0][1] * vl0 + C[1][1] * vl1 + C[2][1] * vl2);
+        g2[l] = sk * (C[0][2] * vl0 + C[1][2] * vl1 + C[2][2] * vl2);
+      }
+      // keep the scheduler from overlapping point iterations (their
+      // geometry temporaries otherwise overflow the register file)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (WANT_Y) {
+      TableRows<double, S::TK> tt_(tB + Q * N, tD + Q * N);  // transposed tables
+#pragma unroll
+      for (int z = 0; z < N; ++z) {
+        tt_.at(z);
+        double rx = 0, ry = 0, rz = 0;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+          rx = fma(tt_.b[z * Q + l], g0[l], rx);
+          ry = fma(tt_.b[z * Q + l], g1[l], ry);
+          rz = fma(tt_.d[z * Q + l], g2[l], rz);
+        }
+        sB[z * S::ZB] = rx;
+        sB[S::TB + z * S::ZB] = ry;
+        sB[2 * S::TB + z * S::ZB] = rz;
+      }
+    }
+  }
+  if constexpr (!WANT_Y) return;
+  __syncthreads();
+
+  // ---- stage 4: transposed y-pencils (into registers, barrier, over A) ----
+  {
+    const bool on4 = tid < S::EB * S::NQ;
+    const int e = on4 ? tid / S::NQ : 0, r = on4 ? tid - e * S::NQ : 0;
+    const int z = r / Q, i = r - z * Q;
+    double* sA = regA + e * S::CA + z * S::ZA + i;
+    const double* sB = regB + e * S::CB + z * S::ZB + i;
+    double rx[Q], ry[Q], rz[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      rx[k] = sB[k * Q];
+      ry[k] = sB[S::TB + k * Q];
+      rz[k] = sB[2 * S::TB + k * Q];
+    }
+    __syncthreads();
+    TableRows<double, S::TK> t_(tB + Q * N, tD + Q * N);
+    if (on4)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        t_.at(j);
+        double sx = 0, so = 0;
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+          sx = fma(t_.b[j * Q + k], rx[k], sx);
+          so = fma(t_.d[j * Q + k], ry[k], so);
+          so = fma(t_.b[j * Q + k], rz[k], so);
+        }
+        sA[j * S::YA] = sx;           // then D_x^T
+        sA[S::TA + j * S::YA] = so;   // then B_x^T
+      }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    if (i < nu) regY[i] = 0.0;
+  }
+  __syncthreads();
+
+  // ---- stage 5: transposed x-pencils, assembled in LDS --------------------
+  if (act1) {
+    const double* sA = regA + e1 * S::CA + o1;
+    double sx[Q], so[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      sx[i] = sA[i];
+      so[i] = sA[S::TA + i];
+    }
+    TableRows<double, S::TK> t_(tB + Q * N, tD + Q * N);
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      t_.at(m);
+      double v = 0;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        v = fma(t_.d[m * Q + i], sx[i], v);
+        v = fma(t_.b[m * Q + i], so[i], v);
+      }
+      if (li[m] != kNoDof) lds_atomic_add(regY + li[m], v);
+    }
+  }
+  __syncthreads();
+
+  // ---- write-back: exclusive dofs stored, shared dofs added ---------------
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    if (dv[k] < 0) continue;
+    const double v = regY[i];
+    if (i < nexcl)
+      ybar[dv[k]] = v;
+    else
+      atomic_add(ybar + dv[k], v);
+  }
+}
+
+template <int N, int Q>
+hipError_t launch_adj3_blk(const double* u, const double* lam, double* dJdk,
+                           double* ybar, const BlockArgs& k, const double* tB,
+                           const double* tD, const double* Xv, const double* tP,
+                           const double* tW, const double* kap, hipStream_t s) {
+  using S = AdjB3Shape<N, Q>;
+  if (k.nblk <= 0) return hipSuccess;
+#define SF_ADJ_LAUNCH(KAP_, WY_)                                                     \
+  hipLaunchKernelGGL((k_adjoint3d_blk<N, Q, KAP_, WY_>), dim3(k.nblk), dim3(S::NT), 0, \
+                     s, u, lam, dJdk, ybar, k.pidx, k.bdofs, k.boff, k.bexcl,        \
+                     k.bcell, k.nblk, tB, tD, Xv, tP, tW, kap)
+  if (kap && ybar)
+    SF_ADJ_LAUNCH(true, true);
+  else if (ybar)
+    SF_ADJ_LAUNCH(false, true);
+  else
+    SF_ADJ_LAUNCH(false, false);  // kappa scales only the lam flux
+#undef SF_ADJ_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t adjoint3d_fused_f64(int n, int q, const double* u, const double* lam,
+                               double* dJdk, double* ybar, const BlockArgs* k,
+                               const double* tB, const double* tD, const double* Xv,
+                               const double* tP, const double* tW, const double* kappa,
+                               hipStream_t s) {
+  if (!k || !dJdk || !Xv) return hipErrorInvalidValue;
+  switch (n * 16 + q) {
+#define SF_CASE(N_, Q_)                                                            \
+  case N_ * 16 + Q_:                                                               \
+    return launch_adj3_blk<N_, Q_>(u, lam, dJdk, ybar, *k, tB, tD, Xv, tP, tW, kappa, s);
+    SF_FOR_EACH_NQ(SF_CASE)
+#undef SF_CASE
+    default:
+      return hipErrorNotSupported;
+  }
+}
+
 template <int N, int Q>
 hipError_t launch_adj3(const double* u, const double* lam, double* dJdk,
                        double* ybar, const int32_t* cells, const double* G,

@@ -202,6 +202,8 @@ struct sf_plan {
   DevBuf<float> dB32, dD32;
   DevBuf<int32_t> cells;
   DevBuf<double> G, Gk, kappa;
+  DevBuf<float> kappa32;  // FP32 copy for the FP32 fused-geometry apply
+  bool k32_valid = false;
   DevBuf<double> X;   // 3D cell vertices (kept for the fused-geometry apply)
   DevBuf<float> X32, Pts32, W32;  // FP32 copies for the FP32 fused apply
   bool fused_geom = false;
@@ -236,7 +238,8 @@ struct sf_plan {
   void release_all() {
     dB.release(); dD.release(); dW.release(); dPts.release();
     dB32.release(); dD32.release();
-    cells.release(); G.release(); Gk.release(); kappa.release(); X.release();
+    cells.release(); G.release(); Gk.release(); kappa.release(); kappa32.release();
+    X.release();
     X32.release(); Pts32.release(); W32.release();
     G32.release(); Gbf.release(); Gm.release(); flag.release();
     pidx.release(); bdofs.release(); bexcl.release(); cell_block.release();
@@ -254,7 +257,7 @@ struct sf_plan {
     return dB.bytes() + dD.bytes() + dW.bytes() + dPts.bytes() + dB32.bytes() +
            dD32.bytes() + cells.bytes() + G.bytes() + Gk.bytes() + X.bytes() +
            X32.bytes() + Pts32.bytes() + W32.bytes() +
-           kappa.bytes() + G32.bytes() + Gbf.bytes() + Gm.bytes() + flag.bytes() +
+           kappa.bytes() + kappa32.bytes() + G32.bytes() + Gbf.bytes() + Gm.bytes() + flag.bytes() +
            pidx.bytes() + bdofs.bytes() + bexcl.bytes() + cell_block.bytes() +
            boff.bytes() + bcell.bytes() +
            xs.bytes() + ys.bytes() + w_r.bytes() + w_p.bytes() + w_q.bytes() +
@@ -324,6 +327,12 @@ int clear_flag(sf_plan* P) {
   return SF_OK;
 }
 
+// The 3D fused-geometry applies (vertices kept, FP32 copies made).
+bool fused_f64(const sf_plan* P) { return P->fused_geom && P->dim == 3 && P->X.p; }
+bool fused_f32(const sf_plan* P) {
+  return P->fused_geom && P->dim == 3 && P->X32.p && (!P->has_kappa || P->k32_valid);
+}
+
 // y = A x (FP64) on stream s.  accumulate: y += A x.  zeroed: the caller
 // guarantees y == 0 already (the CG direction update clears it), so the
 // fill is skipped.
@@ -331,11 +340,13 @@ int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
                  int accumulate = 0, bool zeroed = false, double* dotp = nullptr) {
   if (P->nfree == 0) return SF_OK;  // nothing to compute (and x may be empty)
   if (!accumulate && !zeroed) SF_HIP(launch_fill<double>(y, P->nfree, 0.0, s));
-  // the fused-geometry kernel when the plan has vertices and no kappa
-  const bool geo = P->fused_geom && P->dim == 3 && !P->has_kappa && P->X.p;
+  // the fused-geometry kernel when the plan has vertices (kappa, if set,
+  // read per quadrature point)
+  const bool geo = fused_f64(P);
   hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
                            P->dB.p, P->dD.p, P->ncell, P->blocks(), accumulate, s,
-                           geo ? P->X.p : nullptr, P->dPts.p, P->dW.p, dotp);
+                           geo ? P->X.p : nullptr, P->dPts.p, P->dW.p, dotp,
+                           geo && P->has_kappa ? P->kappa.p : nullptr);
   if (e == hipErrorNotSupported)
     return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
   SF_HIP(e);
@@ -380,6 +391,12 @@ int prepare(sf_plan* P, int precision) {
       SF_HIP(hipMemcpy(P->W32.p, pw.data() + P->q, P->q * 4, hipMemcpyHostToDevice));
       SF_HIP(hipDeviceSynchronize());
     }
+    if (P->fused_geom && P->dim == 3 && P->has_kappa && !P->k32_valid) {
+      SF_HIP(P->kappa32.alloc(P->ncell * P->nqc));
+      SF_HIP(launch_to_f32(P->kappa.p, P->ncell * P->nqc, P->kappa32.p, nullptr));
+      SF_HIP(hipDeviceSynchronize());
+      P->k32_valid = true;
+    }
     if (!P->g32_valid) {
       SF_HIP(P->G32.alloc(ng));
       if (P->dim == 3)
@@ -416,10 +433,11 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
   if (!zeroed) SF_HIP(launch_fill<float>(y, P->nfree, 0.0f, s));
   hipError_t e;
   if (precision == SF_PREC_F32) {
-    const bool geo = P->fused_geom && P->dim == 3 && !P->has_kappa && P->X32.p;
+    const bool geo = fused_f32(P);
     e = apply_f32(P->dim, P->n, P->q, x, y, P->cells.p, P->G32.p, P->dB32.p,
                   P->dD32.p, P->ncell, P->blocks(), 0, s, geo ? P->X32.p : nullptr,
-                  P->Pts32.p, P->W32.p, dotp);
+                  P->Pts32.p, P->W32.p, dotp,
+                  geo && P->has_kappa ? P->kappa32.p : nullptr);
   } else if (precision == SF_PREC_F32_GBF16) {
     e = apply_f32_gbf16(P->dim, P->n, P->q, x, y, P->cells.p, P->Gbf.p,
                         P->dB32.p, P->dD32.p, P->ncell, P->blocks(), 0, s);
@@ -636,8 +654,7 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   Progress prog("cg");
   const int chunk = 8;
   // 3D fused-geometry apply: p.Ap from the apply (alpha_parts / resid / xdir)
-  const bool fused = P->dim == 3 && P->npart > 0 && P->w_part.p && P->fused_geom &&
-                     !P->has_kappa && P->X.p;
+  const bool fused = P->npart > 0 && P->w_part.p && fused_f64(P);
   while (!h.done && h.iters < maxit) {
     prog.tick(h.iters, bb > 0 ? sqrt(h.rr / bb) : sqrt(h.rr));
     const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
@@ -864,8 +881,8 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
   const int chunk = 8;
   Progress prog("mixed cg (fp64 recurrence)");
   // 3D FP32 fused-geometry operator: p.Ap from the apply
-  const bool fused = P->dim == 3 && P->npart > 0 && P->w_part.p && precision == SF_PREC_F32 &&
-                     P->fused_geom && !P->has_kappa && P->X32.p;
+  const bool fused = P->npart > 0 && P->w_part.p && precision == SF_PREC_F32 &&
+                     fused_f32(P);
   for (;;) {
     while (!h.done && h.iters < maxit) {
       prog.tick(h.iters, bnorm > 0 ? sqrt(h.rr) / bnorm : sqrt(h.rr));
@@ -1262,6 +1279,7 @@ int sf_plan_set_kappa(sf_plan* P, const double* kappa) {
   SF_HIP(P->kappa.alloc(n));
   if (n > 0) SF_HIP(hipMemcpy(P->kappa.p, kappa, n * 8, hipMemcpyHostToDevice));
   P->has_kappa = true;
+  P->k32_valid = false;
   if (P->has_geom) return finish_geometry(P);
   P->g32_valid = P->gbf_valid = P->gm_valid = false;
   return SF_OK;
@@ -1342,6 +1360,19 @@ int sf_adjoint(sf_plan* P, const double* u, const double* lam,
   hipStream_t s = (hipStream_t)stream;
   if (P->nfree == 0) {  // every dof Dirichlet: u = lam = 0, so dJ/dkappa = 0
     if (dJdkappa) SF_HIP(launch_fill<double>(dJdkappa, P->ncell * P->nqc, 0.0, s));
+    return SF_OK;
+  }
+  if (fused_f64(P)) {
+    // block-assembled kernel, geometry from the cell vertices
+    if (!dJdkappa) return dJdu ? do_apply_f64(P, dJdu, lam, s) : SF_OK;
+    if (dJdu) SF_HIP(launch_fill<double>(dJdu, P->nfree, 0.0, s));
+    hipError_t e = adjoint3d_fused_f64(P->n, P->q, u, lam, dJdkappa, dJdu, &P->kargs,
+                                       P->dB.p, P->dD.p, P->X.p, P->dPts.p, P->dW.p,
+                                       P->has_kappa ? P->kappa.p : nullptr, s);
+    if (e == hipErrorNotSupported)
+      return fail(SF_ERR_UNSUPPORTED, "no adjoint kernel for dim=%d n=%d q=%d", P->dim, P->n,
+                  P->q);
+    SF_HIP(e);
     return SF_OK;
   }
   if (dJdu) SF_HIP(launch_fill<double>(dJdu, P->nfree, 0.0, s));

@@ -206,13 +206,17 @@ __device__ __forceinline__ void apply_geom(const T (&j0)[3], const T (&j1)[3],
 // dx/dxi and dx/deta linearly in zeta between edge vectors it forms once,
 // and dx/dzeta is constant along its column.
 //
+// KAP (with GEO): a diffusion coefficient kap[cell][qp] (qp = l Q^2 + k Q
+// + i) scales the quadrature weight; it is read in the prologue, 8 (FP64)
+// or 4 (FP32) bytes per quadrature point instead of a kappa-scaled G.
+//
 // DOT (with GEO): wave w of workgroup b also writes dotp[b * NT / 64 + w],
 // its share of x.Ax = sum over blocks of x . (the block's contribution):
 // in the write-back each lane adds x_d times the assembled value of its
 // distinct dofs d (FP32: x kept in registers from the prologue; FP64: x
 // re-read, which costs fewer registers).  A CG step then gets p.Ap without
 // reading p and q again.
-template <int N, int Q, typename T, typename TG, bool GEO, bool DOT>
+template <int N, int Q, typename T, typename TG, bool GEO, bool DOT, bool KAP = false>
 __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
 __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
     k_apply3d(const T* __restrict__ x, T* __restrict__ y,
@@ -224,8 +228,9 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
               const TG* __restrict__ G, const T* __restrict__ tB,
               const T* __restrict__ tD, const T* __restrict__ Xv,
               const T* __restrict__ tP, const T* __restrict__ tW, int accumulate,
-              double* __restrict__ dotp) {
+              double* __restrict__ dotp, const T* __restrict__ kap) {
   using S = Apply3Shape<N, Q, T, GEO>;
+  static_assert(GEO || !KAP, "apply3d: kappa per point needs the fused geometry");
   static_assert(S::EB == Apply3Shape<N, Q, T>::EB && S::NT == Apply3Shape<N, Q, T>::NT,
                 "apply3d: one block structure for both forms");
   __shared__ T smem[S::RC + (GEO ? S::EB * 24 : 0)];
@@ -275,9 +280,17 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
   const int e3 = tid / S::QQ, r3 = tid - e3 * S::QQ;  // r3 = k*Q + i
   const bool act3 = own3 && e3 < ncb;
   T gq[GEO ? 1 : 6][GEO ? 1 : Q];
+  T kq[KAP ? Q : 1];
   if constexpr (GEO) {
     // the block's vertex coordinates into LDS (ncb * 24 consecutive values)
     if (tid < ncb * 24) regV[tid] = Xv[base * 24 + tid];
+    if constexpr (KAP) {
+      // kappa of column r3 of cell e3 at every z level l
+      const T* kp = kap + (base + (act3 ? e3 : 0)) * S::QQQ + (act3 ? r3 : 0);
+#pragma unroll
+      for (int l = 0; l < Q; ++l)
+        kq[l] = act3 ? __builtin_nontemporal_load(kp + l * S::QQ) : T(0);
+    }
   } else if constexpr (sizeof(TG) == 8) {
     // block-interleaved G: thread (e3, r3) is lane tid of every [c*Q + l] row
     const TG* g = G + base * 6 * S::QQQ + (act3 ? tid : 0);
@@ -423,7 +436,8 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
           j0[a] = fma(ze, dE[a], E0[a]);
           j1[a] = fma(ze, dF[a], F0[a]);
         }
-        apply_geom<T>(j0, j1, H, wik * tW[l], gx, gy, gz, tx[l], ty[l], tz[l]);
+        const T wl = KAP ? wik * tW[l] * kq[l] : wik * tW[l];
+        apply_geom<T>(j0, j1, H, wl, gx, gy, gz, tx[l], ty[l], tz[l]);
       } else {
         tx[l] = gq[0][l] * gx + gq[1][l] * gy + gq[2][l] * gz;
         ty[l] = gq[1][l] * gx + gq[3][l] * gy + gq[4][l] * gz;
@@ -702,6 +716,9 @@ struct ApplyArgs {
   const T* tW = nullptr;
   // 3D with Xv: shares of x.Ax, one per wave (nblk * NT / 64 values), or none
   double* dotp = nullptr;
+  // 3D with Xv: diffusion coefficient per quadrature point ([cell][qp]), or
+  // none (kappa = 1)
+  const T* kap = nullptr;
 };
 
 // Host-built block structure of the 3D apply (see k_apply3d).
@@ -719,19 +736,32 @@ hipError_t launch_apply3d_nq(const ApplyArgs<T, TG>& a, const BlockArgs& k,
                              hipStream_t s) {
   using S = Apply3Shape<N, Q, T>;
   if (k.nblk <= 0) return hipSuccess;
-  if (a.dotp && !a.Xv) return hipErrorInvalidValue;  // x.Ax: fused geometry only
-  if (a.Xv && a.dotp)
-    hipLaunchKernelGGL((k_apply3d<N, Q, T, TG, true, true>), dim3(k.nblk), dim3(S::NT), 0,
-                       s, a.x, a.y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell,
-                       k.nblk, a.G, a.tB, a.tD, a.Xv, a.tP, a.tW, a.accumulate, a.dotp);
-  else if (a.Xv)
-    hipLaunchKernelGGL((k_apply3d<N, Q, T, TG, true, false>), dim3(k.nblk), dim3(S::NT), 0,
-                       s, a.x, a.y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell,
-                       k.nblk, a.G, a.tB, a.tD, a.Xv, a.tP, a.tW, a.accumulate, nullptr);
-  else
-    hipLaunchKernelGGL((k_apply3d<N, Q, T, TG, false, false>), dim3(k.nblk), dim3(S::NT), 0,
-                       s, a.x, a.y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell,
-                       k.nblk, a.G, a.tB, a.tD, a.Xv, a.tP, a.tW, a.accumulate, nullptr);
+  // x.Ax and kappa per point: fused geometry only
+  if ((a.dotp || a.kap) && !a.Xv) return hipErrorInvalidValue;
+  if constexpr (sizeof(TG) == sizeof(T)) {  // the fused forms read no G
+#define SF_GEO_LAUNCH(DOT_, KAP_)                                                        \
+  hipLaunchKernelGGL((k_apply3d<N, Q, T, TG, true, DOT_, KAP_>), dim3(k.nblk), dim3(S::NT), \
+                     0, s, a.x, a.y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell, k.nblk, \
+                     a.G, a.tB, a.tD, a.Xv, a.tP, a.tW, a.accumulate, a.dotp, a.kap)
+    if (a.Xv) {
+      if (a.dotp && a.kap)
+        SF_GEO_LAUNCH(true, true);
+      else if (a.dotp)
+        SF_GEO_LAUNCH(true, false);
+      else if (a.kap)
+        SF_GEO_LAUNCH(false, true);
+      else
+        SF_GEO_LAUNCH(false, false);
+      return hipGetLastError();
+    }
+#undef SF_GEO_LAUNCH
+  } else {
+    if (a.Xv) return hipErrorInvalidValue;
+  }
+  hipLaunchKernelGGL((k_apply3d<N, Q, T, TG, false, false>), dim3(k.nblk), dim3(S::NT), 0,
+                     s, a.x, a.y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell,
+                     k.nblk, a.G, a.tB, a.tD, a.Xv, a.tP, a.tW, a.accumulate, nullptr,
+                     nullptr);
   return hipGetLastError();
 }
 

@@ -45,18 +45,20 @@ hipError_t launch_to_bf16(const double* a, int64_t n, bf16s* b, hipStream_t s);
 // tW: the geometric factor is recomputed in the kernel instead of read.
 // 3D with dotp (and Xv): shares of x.Ax, one per wave, into
 // dotp[0 .. k->nblk * NT / 64) (NT from apply_shape_f64).
+// 3D with kap (and Xv): kappa per quadrature point, [cell][qp], scales the
+// weights (G is then unused).
 hipError_t apply_f64(int dim, int n, int q, const double* x, double* y,
                      const int32_t* cells, const double* G, const double* tB,
                      const double* tD, int64_t ncell, const BlockArgs* k,
                      int accumulate, hipStream_t s, const double* Xv = nullptr,
                      const double* tP = nullptr, const double* tW = nullptr,
-                     double* dotp = nullptr);
+                     double* dotp = nullptr, const double* kap = nullptr);
 hipError_t apply_f32(int dim, int n, int q, const float* x, float* y,
                      const int32_t* cells, const float* G, const float* tB,
                      const float* tD, int64_t ncell, const BlockArgs* k,
                      int accumulate, hipStream_t s, const float* Xv = nullptr,
                      const float* tP = nullptr, const float* tW = nullptr,
-                     double* dotp = nullptr);
+                     double* dotp = nullptr, const float* kap = nullptr);
 hipError_t apply_f32_gbf16(int dim, int n, int q, const float* x, float* y,
                            const int32_t* cells, const bf16s* G,
                            const float* tB, const float* tD, int64_t ncell,
@@ -82,6 +84,16 @@ hipError_t adjoint_f64(int dim, int n, int q, const double* u,
                        const double* tD, int64_t ncell,
                        const int32_t* cell_block, const int64_t* bcell,
                        hipStream_t s);
+
+// 3D, block-assembled, geometry recomputed from the cell vertices Xv (24 per
+// cell) with the 1D points / weights tP, tW: dJdk[cell][qp] (required) and,
+// if ybar, the dofs of A(kappa) lam (exclusive dofs stored, shared ones
+// added: the caller zeroes ybar).  kappa may be null (kappa = 1).
+hipError_t adjoint3d_fused_f64(int n, int q, const double* u, const double* lam,
+                               double* dJdk, double* ybar, const BlockArgs* k,
+                               const double* tB, const double* tD, const double* Xv,
+                               const double* tP, const double* tW, const double* kappa,
+                               hipStream_t s);
 
 // ---- BLAS-1 for the solvers (sf_blas.hip) ----
 struct CGScalars {

@@ -151,6 +151,43 @@ def test_adjoint_vs_oracle_and_fd(pkg, oracle_mod, cuda, D, m, p, q):
     A.free()
 
 
+def test_adjoint_every_fused_kernel(pkg, oracle_mod, cuda, monkeypatch):
+    """Every 3D (n, q) of the block-assembled fused-geometry adjoint (with
+    and without kappa) against the oracle's kappa gradient and A(kappa) lam
+    (1e-12), and against the per-cell stored-G kernel of a plan built with
+    SFHIP_FUSED_GEOMETRY=0."""
+    worst = 0.0
+    for n, q in NQ:
+        p = n - 1
+        m = 3 if n <= 5 else 2
+        model = pkg.CartesianModel(3, m, perturb=0.2, seed=1)
+        space = pkg.FESpace(model, p)
+        mp = pkg.SumFactorizationMap.with_points(3, p, q)
+        X = model.cell_vertices()
+        u = pkg.uniform(0, space.n_free, -1, 1)
+        lam = pkg.uniform(1, space.n_free, -1, 1)
+        ud, ld = cuda.from_numpy(u).cuda(), cuda.from_numpy(lam).cuda()
+        kb_ref = oracle_mod.kappa_grad(3, n, q, space.cell_dofs, X, u, lam, mp.tables).reshape(-1)
+        for with_kappa in (True, False):
+            say("fused adjoint case p=%d q=%d m=%d kappa=%d" % (p, q, m, with_kappa))
+            kappa = np.exp(pkg.normal(3, space.n_cells * q ** 3)) if with_kappa else None
+            ub_ref = oracle_mod.apply(3, n, q, space.cell_dofs, X, lam, mp.tables, kappa=kappa)
+            A = pkg.LazyMatrix(mp, space, kappa=kappa)
+            kb, ub = pkg.adjoint.pullback(A, ud, ld)
+            A.free()
+            monkeypatch.setenv("SFHIP_FUSED_GEOMETRY", "0")
+            As = pkg.LazyMatrix(mp, space, kappa=kappa)
+            monkeypatch.delenv("SFHIP_FUSED_GEOMETRY")
+            kbs, ubs = pkg.adjoint.pullback(As, ud, ld)
+            As.free()
+            cuda.cuda.synchronize()
+            kb, ub, kbs, ubs = (t.cpu().numpy() for t in (kb, ub, kbs, ubs))
+            errs = (rel_l2(kb, kb_ref), rel_l2(ub, ub_ref), rel_l2(kb, kbs), rel_l2(ub, ubs))
+            worst = max(worst, *errs)
+            assert max(errs) < 1e-12, (p, q, with_kappa, errs)
+    say("worst rel. l2 %.2e" % worst)
+
+
 def test_adjoint_outputs_optional(pkg, cuda):
     say("adjoint optional outputs")
     model, space, mp, A = make(pkg, 3, 3, 3, 5)
@@ -252,6 +289,85 @@ def test_cg_fused_step_matches_unfused(pkg, oracle_mod, cuda, monkeypatch):
                                 x2.cpu().numpy(), mp.tables)
     assert np.linalg.norm(r2) / np.linalg.norm(b2h) < 2e-10
     A.free()
+
+
+def test_kappa_fused_geometry_every_kernel(pkg, oracle_mod, cuda, monkeypatch):
+    """Every 3D (n, q) of the fused-geometry apply with kappa per quadrature
+    point (log-normal, as in the differentiation workload): FP64 against the
+    oracle (1e-12) and against the stored kappa-scaled G kernel of a plan
+    built with SFHIP_FUSED_GEOMETRY=0; FP32 against the oracle (1e-5)."""
+    worst64 = worst32 = 0.0
+    for n, q in NQ:
+        p = n - 1
+        m = 3 if n <= 5 else 2
+        say("kappa fused case p=%d q=%d m=%d" % (p, q, m))
+        model = pkg.CartesianModel(3, m, perturb=0.2, seed=1)
+        space = pkg.FESpace(model, p)
+        mp = pkg.SumFactorizationMap.with_points(3, p, q)
+        kappa = np.exp(pkg.normal(3, space.n_cells * q ** 3))
+        A = pkg.LazyMatrix(mp, space, kappa=kappa)
+        x = pkg.uniform(0, space.n_free, -1, 1)
+        y_ref = oracle_mod.apply(3, n, q, space.cell_dofs, model.cell_vertices(), x, mp.tables,
+                                 kappa=kappa)
+        xd = cuda.from_numpy(x).cuda()
+        yd = cuda.full_like(xd, float("nan"))
+        A.mul(yd, xd)
+        x32 = xd.float()
+        y32 = cuda.full_like(x32, float("nan"))
+        A.mul_reduced(y32, x32, "f32")
+        cuda.cuda.synchronize()
+        e64 = rel_l2(yd.cpu().numpy(), y_ref)
+        e32 = rel_l2(y32.cpu().numpy().astype(np.float64), y_ref)
+        A.free()
+        monkeypatch.setenv("SFHIP_FUSED_GEOMETRY", "0")
+        As = pkg.LazyMatrix(mp, space, kappa=kappa)
+        monkeypatch.delenv("SFHIP_FUSED_GEOMETRY")
+        ys = cuda.full_like(xd, float("nan"))
+        As.mul(ys, xd)
+        cuda.cuda.synchronize()
+        es = rel_l2(yd.cpu().numpy(), ys.cpu().numpy())
+        As.free()
+        worst64, worst32 = max(worst64, e64, es), max(worst32, e32)
+        assert e64 < 1e-12 and es < 1e-12, (p, q, e64, es)
+        assert e32 < 1e-5, (p, q, e32)
+    say("worst rel. l2: FP64 %.2e, FP32 %.2e" % (worst64, worst32))
+
+
+def test_kappa_cg_fused_matches_unfused(pkg, oracle_mod, cuda, monkeypatch):
+    """CG with kappa: the fused step (fused-geometry apply reading kappa,
+    p.Ap from the apply) against the unfused step on the stored
+    kappa-scaled G; FP64 and mixed (FP32 operator, FP64 recurrence)."""
+    say("kappa cg fused vs unfused")
+    model = pkg.CartesianModel(3, 5, perturb=0.2, seed=1)
+    space = pkg.FESpace(model, 4)
+    mp = pkg.SumFactorizationMap.with_points(3, 4, 6)
+    kappa = np.exp(pkg.normal(3, space.n_cells * 216))
+    b = cuda.from_numpy(pkg.normal(5, space.n_free)).cuda()
+    res = {}
+    for fused in (True, False):
+        if not fused:
+            monkeypatch.setenv("SFHIP_FUSED_GEOMETRY", "0")
+        A = pkg.LazyMatrix(mp, space, kappa=kappa)
+        if not fused:
+            monkeypatch.delenv("SFHIP_FUSED_GEOMETRY")
+        x = cuda.zeros_like(b)
+        st = pkg.cg_(x, A, b, rtol=1e-10, maxiter=20000)
+        xm = cuda.zeros_like(b)
+        sm = pkg.cg_mixed_(xm, A, b, rtol=1e-10, inner="f32", delta=0.1, maxiter=20000)
+        res[fused] = (st, sm, x.cpu().numpy(), xm.cpu().numpy())
+        A.free()
+    bh = b.cpu().numpy()
+    for fused in (True, False):
+        st, sm, x, xm = res[fused]
+        say("fused" if fused else "unfused", st, sm)
+        assert st["converged"] == 1 and sm["converged"] == 1
+        for xx in (x, xm):
+            r = bh - oracle_mod.apply(3, 5, 6, space.cell_dofs, model.cell_vertices(), xx,
+                                      mp.tables, kappa=kappa)
+            assert np.linalg.norm(r) / np.linalg.norm(bh) < 2e-10
+    it_f, it_u = res[True][0]["iterations"], res[False][0]["iterations"]
+    assert abs(it_f - it_u) <= max(3, 0.01 * it_u)
+    assert rel_l2(res[True][2], res[False][2]) < 1e-8
 
 
 @pytest.mark.parametrize("inner", ["f32", "f32_gbf16", "mfma_bf16x3"])
