@@ -415,9 +415,16 @@ def mode_q9(args, torch, pkg):
     x = torch.from_numpy(xh).cuda()
     y = torch.empty_like(x)
     wall, kern = time_apply(torch, A, x, y, args.steps, args.warmup)
-    B64 = alg_bytes(nf, E, n, q, D)
+    # fused-geometry apply reading kappa per quadrature point: bytes of the
+    # vectors, cell dofs, vertices and kappa; FLOPs of the reference's model
+    B64 = alg_bytes_fused(nf, E, n, D) + 8 * nq * E
+    F = pkg.count_manual_flops_poisson_matrix_free(E, D, (q,) * D, (n,) * D,
+                                                   count_only_fma_ops=False,
+                                                   structured_mesh_opt=False)
     res["f64_apply"] = {"gdofs": nf / wall / 1e9, "kernel_ms": kern * 1e3,
-                        "hbm_frac": B64 / kern / 1e9 / HBM_PEAK_GBS}
+                        "alg_bytes": B64, "alg_flops": F,
+                        "hbm_frac": B64 / kern / 1e9 / HBM_PEAK_GBS,
+                        "fp64_frac": F / kern / 1e12 / FP64_PEAK_TFLOPS}
     # the FP64 apply at this (n, q) with kappa is checked against the oracle
     # by the GPU tests; the reduced forms are measured against it here
     y64 = y.clone()

@@ -160,7 +160,9 @@ int sf_plan_set_geometry(sf_plan* plan, const double* G);
 int sf_plan_get_geometry(const sf_plan* plan, double* G);
 
 /* Variable diffusion coefficient kappa at quadrature points, host array
- * [cell][q^D] (positive).  Afterwards every apply uses kappa * G.
+ * [cell][q^D] (positive).  Afterwards every apply uses kappa * G (3D plans
+ * built from vertices read kappa per point and recompute G; the others,
+ * and the reduced-precision G forms, store kappa * G).
  * No reference counterpart (the reference's operator has kappa = 1). */
 int sf_plan_set_kappa(sf_plan* plan, const double* kappa);
 
@@ -201,8 +203,10 @@ int sf_apply_f32(sf_plan* plan, float* y, const float* x, int precision,
 /* Pullback of y = A(kappa) u for a cotangent lam (device pointers, FP64):
  *   dJdkappa[cell][qp] = grad(lam)_q . (G_q grad(u)_q)      (kappa-free G),
  *   dJdu               = A(kappa)^T lam = A(kappa) lam       (overwritten).
- * Gradients at quadrature points are recomputed, never stored.  Either
- * output may be NULL.  No reference counterpart (new kernel). */
+ * Gradients at quadrature points are recomputed, never stored; 3D plans
+ * built from vertices also recompute the geometric factor (one
+ * block-assembled kernel).  Either output may be NULL.  No reference
+ * counterpart (new kernel). */
 int sf_adjoint(sf_plan* plan, const double* u, const double* lam,
                double* dJdkappa, double* dJdu, void* stream);
 
