@@ -136,6 +136,17 @@ def mode_apply(args, torch, pkg):
                      "alg_bytes": Balg, "alg_flops": Falg,
                      "achieved_GBs": Balg / kern / 1e9},
     }
+    # host vectors (sf_apply_host: x staged in, A x, y staged out over PCIe);
+    # a side figure, never `value`
+    yh = np.empty_like(xh)
+    A.mul(yh, xh)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        A.mul(yh, xh)
+    th = (time.perf_counter() - t0) / 5
+    out["host_vector_apply"] = {"gdofs": space.n_free / th / 1e9, "ms": th * 1e3,
+                                "note": "sf_apply_host on pageable numpy vectors: "
+                                        "PCIe-inclusive"}
     # the stored-G form of the same apply (G read from HBM, HBM-bound)
     os.environ["SFHIP_FUSED_GEOMETRY"] = "0"
     model_s, space_s, mp_s, A_s = build_problem(pkg, D, m, p, q)
@@ -291,6 +302,19 @@ def mode_cg(args, torch, pkg):
     b = torch.from_numpy(pkg.normal(2, space.n_free)).cuda()
     res = {"metric": "CG to 1e-10: FP64 vs mixed-precision refinement, 3D 32^3 Q7 q=9",
            "n_free": space.n_free, "n_gpus": 1}
+    # the reference's CG benchmark (profiling/benchmark.jl:16-25): min and
+    # max over 10 runs of cg!(x, A, b, maxiter = 10) after one warm-up
+    times = []
+    for i in range(11):
+        x10 = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pkg.cg_(x10, A, b, rtol=1e-10, maxiter=10)
+        torch.cuda.synchronize()
+        if i:
+            times.append(time.perf_counter() - t0)
+    res["cg_maxiter10_ms"] = {"min": min(times) * 1e3, "max": max(times) * 1e3}
+    log("cg maxiter=10", res["cg_maxiter10_ms"])
     x64 = torch.zeros_like(b)
     st = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=args.maxit)
     res["f64"] = st
