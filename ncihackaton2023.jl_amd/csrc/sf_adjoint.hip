@@ -522,7 +522,9 @@ __attribute__((amdgpu_waves_per_eu(2)))
   const bool own3 = tid < S::EB * S::QQ;
   const int e3 = tid / S::QQ, r3 = tid - e3 * S::QQ;  // r3 = k*Q + i
   const bool act3 = own3 && e3 < ncb;
-  if (tid < ncb * 24) regV[tid] = Xv[base * 24 + tid];
+  // ncb * 24 values: more than NT when a block holds more than NT / 24
+  // cells (up to 64 at the lowest orders)
+  for (int t = tid; t < ncb * 24; t += S::NT) regV[t] = Xv[base * 24 + t];
   double kq[KAP ? Q : 1];
   if constexpr (KAP) {
     const double* kp = kap + (base + (act3 ? e3 : 0)) * S::QQQ + (act3 ? r3 : 0);

@@ -283,7 +283,9 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
   T kq[KAP ? Q : 1];
   if constexpr (GEO) {
     // the block's vertex coordinates into LDS (ncb * 24 consecutive values)
-    if (tid < ncb * 24) regV[tid] = Xv[base * 24 + tid];
+    // ncb * 24 values: more than NT when a block holds more than NT / 24
+    // cells (up to 64 at the lowest orders)
+    for (int t = tid; t < ncb * 24; t += S::NT) regV[t] = Xv[base * 24 + t];
     if constexpr (KAP) {
       // kappa of column r3 of cell e3 at every z level l
       const T* kp = kap + (base + (act3 ? e3 : 0)) * S::QQQ + (act3 ? r3 : 0);

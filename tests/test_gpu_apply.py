@@ -54,6 +54,49 @@ def test_every_instantiated_kernel_vs_oracle(pkg, oracle_mod, cuda, D):
     say("worst rel. l2 over %d kernels: %.2e" % (len(NQ), worst))
 
 
+def test_full_blocks_every_kernel(pkg, oracle_mod, cuda):
+    """3D meshes whose rows hold exactly one full cell block (m = cells per
+    block, up to 64 at the lowest orders): every (n, q) of the fused-geometry
+    FP64 and FP32 applies and of the block adjoint, with kappa, against the
+    oracle.  Blocks of more than NT / 24 cells need several vertex loads per
+    thread; the small meshes above never fill them."""
+    worst = 0.0
+    for n, q in NQ:
+        p = n - 1
+        _, _, _, A0 = make(pkg, 3, 2, p, q)
+        eb = A0.info()["cells_per_block"]
+        A0.free()
+        m = max(eb, 2)
+        say("full blocks p=%d q=%d m=%d" % (p, q, m))
+        model = pkg.CartesianModel(3, m, perturb=0.2, seed=1)
+        space = pkg.FESpace(model, p)
+        mp = pkg.SumFactorizationMap.with_points(3, p, q)
+        kappa = np.exp(pkg.normal(3, space.n_cells * q ** 3))
+        A = pkg.LazyMatrix(mp, space, kappa=kappa)
+        if m == eb:  # one full block per mesh row
+            assert A.info()["n_blocks"] == m * m
+        X = model.cell_vertices()
+        x = pkg.uniform(0, space.n_free, -1, 1)
+        lam = pkg.uniform(1, space.n_free, -1, 1)
+        y_ref = oracle_mod.apply(3, n, q, space.cell_dofs, X, x, mp.tables, kappa=kappa)
+        xd = cuda.from_numpy(x).cuda()
+        yd = cuda.full_like(xd, float("nan"))
+        A.mul(yd, xd)
+        y32 = cuda.full_like(xd.float(), float("nan"))
+        A.mul_reduced(y32, xd.float(), "f32")
+        kb, ub = pkg.adjoint.pullback(A, xd, cuda.from_numpy(lam).cuda())
+        cuda.cuda.synchronize()
+        kb_ref = oracle_mod.kappa_grad(3, n, q, space.cell_dofs, X, x, lam, mp.tables).reshape(-1)
+        ub_ref = oracle_mod.apply(3, n, q, space.cell_dofs, X, lam, mp.tables, kappa=kappa)
+        e = (rel_l2(yd.cpu().numpy(), y_ref), rel_l2(kb.cpu().numpy(), kb_ref),
+             rel_l2(ub.cpu().numpy(), ub_ref))
+        e32 = rel_l2(y32.cpu().numpy().astype(np.float64), y_ref)
+        worst = max(worst, *e)
+        assert max(e) < TOL and e32 < 1e-5, (p, q, m, e, e32)
+        A.free()
+    say("worst rel. l2 %.2e" % worst)
+
+
 def test_config1_2d_q4_vs_oracle_and_assembled(pkg, oracle_mod, brute, cuda):
     """2D unit square, 64x64 Cartesian quads, Q4, 6 Gauss points, u ~ U[-1,1]
     seed 0: against the oracle and the assembled CSR SpMV."""
