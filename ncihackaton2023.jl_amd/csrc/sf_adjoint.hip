@@ -498,7 +498,7 @@ __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
   for (int k = 0; k < S::KU; ++k) {
     const int i = tid + k * S::NT;
-    dv[k] = i < nu ? dofs[i] : -1;
+    dv[k] = i < nu ? __builtin_nontemporal_load(dofs + i) : -1;
   }
 #pragma unroll
   for (int k = 0; k < S::KU; ++k) {
@@ -514,7 +514,7 @@ __attribute__((amdgpu_waves_per_eu(2)))
   if (act1) {
     const uint16_t* lp = pidx + ((base + e1) * S::NN + r1) * N;
 #pragma unroll
-    for (int j = 0; j < N; ++j) li[j] = lp[j];
+    for (int j = 0; j < N; ++j) li[j] = __builtin_nontemporal_load(lp + j);
   } else {
 #pragma unroll
     for (int j = 0; j < N; ++j) li[j] = kNoDof;
@@ -524,7 +524,8 @@ __attribute__((amdgpu_waves_per_eu(2)))
   const bool act3 = own3 && e3 < ncb;
   // ncb * 24 values: more than NT when a block holds more than NT / 24
   // cells (up to 64 at the lowest orders)
-  for (int t = tid; t < ncb * 24; t += S::NT) regV[t] = Xv[base * 24 + t];
+  for (int t = tid; t < ncb * 24; t += S::NT)
+    regV[t] = __builtin_nontemporal_load(Xv + base * 24 + t);
   double kq[KAP ? Q : 1];
   if constexpr (KAP) {
     const double* kp = kap + (base + (act3 ? e3 : 0)) * S::QQQ + (act3 ? r3 : 0);

@@ -251,13 +251,17 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
   // ---- prologue: every global read of the block, issued at once ---------
   // In the order the stages need them (dof ids, x values, local indices,
   // G), so waiting for the first ones leaves the G stream in flight while
-  // the gather and the first two contraction stages run.
+  // the gather and the first two contraction stages run.  The block
+  // metadata (dof ids, local indices) is read once per apply: non-temporal
+  // loads keep it from displacing x lines that neighbouring blocks gather
+  // again (measured 1-3% faster at Q3-Q6; the vertex loads below the same,
+  // 1-2.5% at Q2-Q4).
   int32_t dv[S::KU];
   T xv[S::KU];
 #pragma unroll
   for (int k = 0; k < S::KU; ++k) {
     const int i = tid + k * S::NT;
-    dv[k] = i < nu ? dofs[i] : -1;
+    dv[k] = i < nu ? __builtin_nontemporal_load(dofs + i) : -1;
   }
 #pragma unroll
   for (int k = 0; k < S::KU; ++k) xv[k] = x[dv[k] >= 0 ? dv[k] : 0];
@@ -270,7 +274,7 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
   if (act1) {
     const uint16_t* lp = pidx + ((base + e1) * S::NN + r1) * N;
 #pragma unroll
-    for (int j = 0; j < N; ++j) li[j] = lp[j];
+    for (int j = 0; j < N; ++j) li[j] = __builtin_nontemporal_load(lp + j);
   } else {
 #pragma unroll
     for (int j = 0; j < N; ++j) li[j] = kNoDof;
@@ -285,7 +289,8 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
     // the block's vertex coordinates into LDS (ncb * 24 consecutive values)
     // ncb * 24 values: more than NT when a block holds more than NT / 24
     // cells (up to 64 at the lowest orders)
-    for (int t = tid; t < ncb * 24; t += S::NT) regV[t] = Xv[base * 24 + t];
+    for (int t = tid; t < ncb * 24; t += S::NT)
+      regV[t] = __builtin_nontemporal_load(Xv + base * 24 + t);
     if constexpr (KAP) {
       // kappa of column r3 of cell e3 at every z level l
       const T* kp = kap + (base + (act3 ? e3 : 0)) * S::QQQ + (act3 ? r3 : 0);
