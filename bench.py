@@ -350,6 +350,50 @@ def mode_cg(args, torch, pkg):
     return res
 
 
+def fd_check(torch, A, kappa, u, d, kb, ub, ndir, h=1e-6):
+    """Central finite differences of J = 1/2 |A(kappa) u - d|^2 along ndir
+    random unit directions in kappa and in u (h = 1e-6, FP64), against the
+    adjoint's <dJ/dkappa, dk> and <dJ/du, du>.  J is quadratic in each, so
+    the only error is rounding; J(+) - J(-) is formed as
+    1/2 (r+ - r-) . (r+ + r-) to keep the subtraction at the residual level.
+    Error measure per direction: |fd - an| / (|an| + 1e-3 |grad|)."""
+    gen = torch.Generator(device="cuda")
+    kd = torch.from_numpy(kappa).cuda()
+    y = torch.empty_like(u)
+
+    def resid(v):
+        A.mul(y, v)
+        return y - d
+
+    def central(rp, rm):
+        return 0.5 * float(((rp - rm) * (rp + rm)).sum()) / (2 * h)
+
+    errs_k, errs_u = [], []
+    for s in range(ndir):
+        gen.manual_seed(1000 + s)
+        dk = torch.randn(kd.shape, generator=gen, device="cuda", dtype=torch.float64)
+        dk /= torch.linalg.norm(dk)
+        A.set_kappa((kd + h * dk).cpu().numpy())
+        rp = resid(u)
+        A.set_kappa((kd - h * dk).cpu().numpy())
+        rm = resid(u)
+        an = float((kb * dk).sum())
+        errs_k.append(abs(central(rp, rm) - an) / (abs(an) + 1e-3 * float(torch.linalg.norm(kb))))
+    A.set_kappa(kappa)
+    for s in range(ndir):
+        gen.manual_seed(2000 + s)
+        du = torch.randn(u.shape, generator=gen, device="cuda", dtype=torch.float64)
+        du /= torch.linalg.norm(du)
+        rp = resid(u + h * du)
+        rm = resid(u - h * du)
+        an = float((ub * du).sum())
+        errs_u.append(abs(central(rp, rm) - an) / (abs(an) + 1e-3 * float(torch.linalg.norm(ub))))
+    return {"directions": ndir, "h": h, "max_err_dkappa": max(errs_k),
+            "max_err_du": max(errs_u), "median_err_dkappa": sorted(errs_k)[ndir // 2],
+            "median_err_du": sorted(errs_u)[ndir // 2],
+            "directions_from": "torch.randn on the device, seeds 1000+s (kappa), 2000+s (u)"}
+
+
 def mode_adjoint(args, torch, pkg):
     """Differentiation workload: 3D 48^3 perturbed hexes, Q5, 7 Gauss
     pts/dir, log-normal kappa per quadrature point.  The pullback of
@@ -407,6 +451,10 @@ def mode_adjoint(args, torch, pkg):
                      "adjoint_over_apply": t / kern}
         outs[form] = (y.clone(), kb.clone(), ub.clone())
         log(form, res[form])
+        if form == "fused" and args.fd_dirs > 0:
+            log("central finite differences along %d directions ..." % args.fd_dirs)
+            res["fd_check"] = fd_check(torch, A, kappa, u, d, kb, ub, args.fd_dirs)
+            log("fd", res["fd_check"])
         A.free()
     for i, name in enumerate(("Au", "dJdkappa", "dJdu")):
         a, b = outs["fused"][i], outs["stored_g"][i]
@@ -534,6 +582,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--maxit", type=int, default=300000)
     ap.add_argument("--max-outer", type=int, default=50)
+    ap.add_argument("--fd-dirs", type=int, default=64,
+                    help="adjoint mode: central-difference directions in kappa and in u")
     ap.add_argument("--q9-maxit", type=int, default=40000,
                     help="q9 mode: iteration cap of both CG solves")
     ap.add_argument("--inner", default="f32:1e-2,f32_gbf16:1e-2,mfma_bf16x3:1e-2,mfma_bf16:1e-2",
