@@ -97,6 +97,31 @@ def test_full_blocks_every_kernel(pkg, oracle_mod, cuda):
     say("worst rel. l2 %.2e" % worst)
 
 
+@pytest.mark.parametrize("p,q,m,perturb", [(1, 3, 4, 0.0), (1, 3, 4, 0.2), (3, 5, 3, 0.2)])
+def test_demo_step_by_step_on_gpu(pkg, oracle_mod, cuda, p, q, m, perturb):
+    """test/Demo_StepByStep.jl's case (2D, Q1, 4 x 4 cells, 3 Gauss points,
+    x = ones) and two perturbed variants: the HIP apply against the
+    explicit-loop restatement of the demo; on the demo's own case also the
+    textbook Q1 stencil ((8 - free neighbours) / 3 per free dof)."""
+    from oracle import stepbystep
+    say("step-by-step demo p=%d q=%d m=%d perturb=%g" % (p, q, m, perturb))
+    model, space, mp, A = make(pkg, 2, m, p, q, perturb=perturb)
+    x = np.ones(space.n_free) if perturb == 0.0 else pkg.uniform(0, space.n_free, -1, 1)
+    y = gpu_apply(cuda, A, x)
+    tab = [np.asarray(t, dtype=np.float64).tolist() for t in mp.tables]
+    y_sbs = np.array(stepbystep.apply_2d(space.cell_dofs, model.cell_vertices(), x, tab))
+    assert rel_l2(y, y_sbs) < TOL
+    if perturb == 0.0 and p == 1:
+        F = m - 1
+        k = np.zeros((F, F))
+        for dj in (-1, 0, 1):
+            for di in (-1, 0, 1):
+                if di or dj:
+                    k[max(0, -dj):F - max(0, dj), max(0, -di):F - max(0, di)] += 1
+        assert np.abs(y - (8 - k.reshape(-1)) / 3.0).max() < 1e-14
+    A.free()
+
+
 def test_config1_2d_q4_vs_oracle_and_assembled(pkg, oracle_mod, brute, cuda):
     """2D unit square, 64x64 Cartesian quads, Q4, 6 Gauss points, u ~ U[-1,1]
     seed 0: against the oracle and the assembled CSR SpMV."""

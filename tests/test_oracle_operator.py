@@ -223,3 +223,48 @@ def test_flop_model_contraction_count(pkg):
         assert all_flops == 4 * fma + D * q ** D
     # CompareFlops.jl's case: D=2, Q1 (SB=2), 3 points, 16 cells
     assert pkg.count_manual_flops_poisson_matrix_free(16, 2, (3, 3), (2, 2)) == 16 * 120
+
+
+def _q1_demo_known_answer(m):
+    """A 1 on the free dofs of Q1, m x m unit-square cells, homogeneous
+    Dirichlet: the Q1 stencil is 8/3 at the centre and -1/3 at each of the 8
+    neighbours, so a free dof with k free neighbours gets (8 - k) / 3."""
+    F = m - 1
+    y = np.empty(F * F)
+    for j in range(F):
+        for i in range(F):
+            k = sum(1 for dj in (-1, 0, 1) for di in (-1, 0, 1)
+                    if (di or dj) and 0 <= i + di < F and 0 <= j + dj < F)
+            y[i + F * j] = (8 - k) / 3.0
+    return y
+
+
+@pytest.mark.parametrize("p,q,m,perturb", [(1, 3, 4, 0.0), (1, 3, 4, 0.2), (3, 5, 3, 0.2)])
+def test_step_by_step_demo_restatement(pkg, oracle_mod, brute, p, q, m, perturb):
+    """test/Demo_StepByStep.jl's case (2D, Q1, 4 x 4 Cartesian cells,
+    degree 4 -> 3 Gauss points, x = ones) and two perturbed variants: the
+    explicit-loop restatement of the demo equals the C oracle and the
+    brute-force assembled matrix; on the demo's own case it also equals the
+    textbook Q1 stencil.  Swapping the value and derivative tables (a
+    transposed operand) breaks the agreement."""
+    from oracle import stepbystep
+    model = pkg.CartesianModel(2, m, perturb=perturb, seed=1)
+    space = pkg.FESpace(model, p)
+    mp = pkg.SumFactorizationMap.with_points(2, p, q)
+    x = np.ones(space.n_free) if perturb == 0.0 else pkg.uniform(0, space.n_free, -1, 1)
+    X = model.cell_vertices()
+    tab = [np.asarray(t, dtype=np.float64).tolist() for t in mp.tables]
+    y = np.array(stepbystep.apply_2d(space.cell_dofs, X, x, tab))
+    y_c = oracle_mod.apply(2, p + 1, q, space.cell_dofs, X, x, mp.tables)
+    Bb, Db, pb, wb = brute.tables(p, q)
+    G, _ = brute.geometry_numpy(2, X, pb, wb)
+    A = brute.assemble_csr(space.cell_dofs, brute.element_matrices(2, Bb, Db, G), space.n_free)
+    assert rel_l2(y, y_c) < 1e-13
+    assert rel_l2(y, A @ x) < 1e-13
+    if perturb == 0.0 and p == 1:
+        assert np.abs(y - _q1_demo_known_answer(m)).max() < 1e-14
+    else:
+        # (on the symmetric demo case with x = 1 the swap goes unnoticed)
+        swapped = [tab[1], tab[0], tab[2], tab[3]]
+        y_bad = np.array(stepbystep.apply_2d(space.cell_dofs, X, x, swapped))
+        assert rel_l2(y_bad, y_c) > 1e-3
