@@ -188,6 +188,34 @@ def test_adjoint_every_fused_kernel(pkg, oracle_mod, cuda, monkeypatch):
     say("worst rel. l2 %.2e" % worst)
 
 
+def test_adjoint_homogeneity_identity_config4_size(pkg, cuda):
+    """Size-independent properties of the pullback at the differentiation
+    workload's full size (48^3 Q5, q = 7, log-normal kappa): A(kappa) is
+    linear in kappa, so sum_q kappa_q dJdk_q = lam . A(kappa) u for the
+    cotangent lam (Euler's identity for a degree-1 homogeneous function);
+    and dJ/du = A(kappa) lam is the kappa apply itself."""
+    say("adjoint homogeneity 48^3 Q5")
+    model = pkg.CartesianModel(3, 48, perturb=0.2, seed=1)
+    space = pkg.FESpace(model, 5)
+    mp = pkg.SumFactorizationMap.with_points(3, 5, 7)
+    kappa = np.exp(pkg.normal(3, space.n_cells * 343))
+    A = pkg.LazyMatrix(mp, space, kappa=kappa)
+    u = cuda.from_numpy(pkg.uniform(0, space.n_free, -1, 1)).cuda()
+    lam = cuda.from_numpy(pkg.uniform(4, space.n_free, -1, 1)).cuda()
+    kb, ub = pkg.adjoint.pullback(A, u, lam)
+    Au = cuda.empty_like(u)
+    A.mul(Au, u)
+    Al = cuda.empty_like(u)
+    A.mul(Al, lam)
+    cuda.cuda.synchronize()
+    lhs = float((kb * cuda.from_numpy(kappa).cuda()).sum())
+    rhs = float((lam * Au).sum())
+    say("sum kappa dJdk = %.15e, lam.A u = %.15e" % (lhs, rhs))
+    assert abs(lhs - rhs) < 1e-11 * float(kb.abs().mul(cuda.from_numpy(kappa).cuda()).sum())
+    assert rel_l2(ub.cpu().numpy(), Al.cpu().numpy()) < 1e-14
+    A.free()
+
+
 def test_adjoint_outputs_optional(pkg, cuda):
     say("adjoint optional outputs")
     model, space, mp, A = make(pkg, 3, 3, 3, 5)
