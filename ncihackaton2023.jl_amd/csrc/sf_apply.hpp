@@ -86,11 +86,14 @@ struct Apply3Shape {
   static constexpr int EB = EB_T < EB_L ? EB_T : EB_L;
   static constexpr int NT = round_up(EB * MAXT, 64);
   // minimum waves per SIMD the compiler must fit the FP64 kernel into.  At
-  // N = 7 and (N = 8, Q <= 9) it would use 170-190 VGPRs (2 waves per SIMD)
-  // while the LDS allows 3 workgroups per CU; held to 168 (a few spills at
-  // N = 8) the third wave wins (measured: Q6 -13%, Q7 -6%).  Elsewhere the
-  // compiler's own choice was as fast or faster.
-  static constexpr int WPE = (sizeof(T) == 8 && (N == 7 || (N == 8 && Q <= 9))) ? 3 : 1;
+  // N = 7, (N = 8, Q <= 9) and (N = 9, Q <= 10) it would use 170-206 VGPRs
+  // (2 waves per SIMD) while the LDS allows 3 workgroups per CU; held to 168
+  // (a few spills from N = 8) the third wave wins (measured: Q6 -13%, Q7
+  // -6%, Q8 -12%).  At N = 10 the spills cost more than the wave gains
+  // (Q9 with q = 11: +14%); elsewhere the compiler's own choice was as fast
+  // or faster.
+  static constexpr int WPE =
+      (sizeof(T) == 8 && (N == 7 || (N == 8 && Q <= 9) || (N == 9 && Q <= 10))) ? 3 : 1;
   // the apply kernel's LDS strides (SA1 / SB1 / QP above size EB and serve
   // the adjoint kernel's dense layout).  The FP64 table assumes the shared
   // region from N = 5 on; the stored-G kernel at N = 5 (split regions)
