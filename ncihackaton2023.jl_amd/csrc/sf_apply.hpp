@@ -90,9 +90,11 @@ struct Apply3Shape {
   // (2 waves per SIMD) while the LDS allows 3 workgroups per CU; held to 168
   // (a few spills from N = 8) the third wave wins (measured: Q6 -13%, Q7
   // -6%, Q8 -12%).  At N = 10 the spills cost more than the wave gains
-  // (Q9 with q = 11: +14%); elsewhere the compiler's own choice was as fast
-  // or faster.
+  // (Q9 with q = 11: +14%).  The FP32 kernel at N = 6, Q <= 7 fits five
+  // waves without spills (Q5 -7%; at N = 7, 8 the same cap was +1% / +14%).
+  // Elsewhere the compiler's own choice was as fast or faster.
   static constexpr int WPE =
+      (sizeof(T) == 4 && N == 6 && Q <= 7) ? 5 :
       (sizeof(T) == 8 && (N == 7 || (N == 8 && Q <= 9) || (N == 9 && Q <= 10))) ? 3 : 1;
   // the apply kernel's LDS strides (SA1 / SB1 / QP above size EB and serve
   // the adjoint kernel's dense layout).  The FP64 table assumes the shared
