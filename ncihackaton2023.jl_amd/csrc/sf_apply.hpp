@@ -86,16 +86,19 @@ struct Apply3Shape {
   static constexpr int EB = EB_T < EB_L ? EB_T : EB_L;
   static constexpr int NT = round_up(EB * MAXT, 64);
   // minimum waves per SIMD the compiler must fit the FP64 kernel into.  At
-  // N = 7, (N = 8, Q <= 9) and (N = 9, Q <= 10) it would use 170-206 VGPRs
-  // (2 waves per SIMD) while the LDS allows 3 workgroups per CU; held to 168
-  // (a few spills from N = 8) the third wave wins (measured: Q6 -13%, Q7
-  // -6%, Q8 -12%).  At N = 10 the spills cost more than the wave gains
+  // N = 7 and, fused geometry only, (N = 8, Q <= 9) and (N = 9, Q <= 10) it
+  // would use 170-206 VGPRs (2 waves per SIMD) while the LDS allows 3
+  // workgroups per CU; held to 168 (a few spills from N = 8) the third wave
+  // wins (measured: Q6 -13%, Q7 -6%, Q8 -12%).  The stored-G kernel spills
+  // more: with the cap it is 2% faster at Q6 but 14% slower at Q7 and 26%
+  // slower at Q8.  At N = 10 the spills cost more than the wave gains
   // (Q9 with q = 11: +14%).  The FP32 kernel at N = 6, Q <= 7 fits five
   // waves without spills (Q5 -7%; at N = 7, 8 the same cap was +1% / +14%).
   // Elsewhere the compiler's own choice was as fast or faster.
   static constexpr int WPE =
       (sizeof(T) == 4 && N == 6 && Q <= 7) ? 5 :
-      (sizeof(T) == 8 && (N == 7 || (N == 8 && Q <= 9) || (N == 9 && Q <= 10))) ? 3 : 1;
+      (sizeof(T) == 8 && (N == 7 || (GEO && ((N == 8 && Q <= 9) || (N == 9 && Q <= 10))))) ? 3
+                                                                                            : 1;
   // the apply kernel's LDS strides (SA1 / SB1 / QP above size EB and serve
   // the adjoint kernel's dense layout).  The FP64 table assumes the shared
   // region from N = 5 on; the stored-G kernel at N = 5 (split regions)
@@ -223,7 +226,7 @@ __device__ __forceinline__ void apply_geom(const T (&j0)[3], const T (&j1)[3],
 // reading p and q again.
 template <int N, int Q, typename T, typename TG, bool GEO, bool DOT, bool KAP = false>
 __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
-__attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T>::WPE)))
+__attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
     k_apply3d(const T* __restrict__ x, T* __restrict__ y,
               const uint16_t* __restrict__ pidx,
               const int32_t* __restrict__ bdofs,
