@@ -589,9 +589,9 @@ def main():
     ap.add_argument("--inner", default="f32:1e-2,f32_gbf16:1e-2,mfma_bf16x3:1e-2,mfma_bf16:1e-2",
                     help="cg mode: restarted refinement runs, comma list of "
                          "inner_precision:inner_rtol")
-    ap.add_argument("--mixed", default="f32:0.1:f64,f32:0.1,f32_gbf16:0.1:f64,mfma_bf16x3:0.1",
+    ap.add_argument("--mixed", default="f32:0.1:f64p32,f32:0.1:f64,f32:0.1,f32_gbf16:0.1:f64,mfma_bf16x3:0.1:f64",
                     help="cg mode: reliable-update mixed CG runs, comma list of "
-                         "inner_precision:delta[:recurrence f32|f64]")
+                         "inner_precision:delta[:recurrence f32|f64|f64p32]")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); the apply "

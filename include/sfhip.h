@@ -233,7 +233,9 @@ int sf_cg_ir(sf_plan* plan, const double* b, double* x, double rtol,
  * vectors (dot products accumulated in FP64), x (FP64) absorbing the FP32
  * partial solution at each update; fp64_recurrence = 1: FP64 vectors and
  * updates, only the operator in reduced precision (an FP32 copy of the
- * direction goes into it).  Whenever the recurred residual has fallen by
+ * direction goes into it); fp64_recurrence = 2: as 1, but the search
+ * direction is held only in FP32 (3D plans with the fused-geometry FP32
+ * operator; elsewhere it runs as 1).  Whenever the recurred residual has fallen by
  * the factor delta (0 < delta < 1) since the last update, the residual is
  * replaced by the FP64 b - A x; the search direction is kept, so no Krylov
  * information is lost (unlike sf_cg_ir's restarts).  Stops when the FP64

@@ -848,10 +848,12 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
 // FP64 b - A x whenever it has dropped by delta (search direction kept,
 // beta' = ||r_true||^2 / ||r_{k-1}||^2).  Fewer iterations than the FP32
 // recurrence (which loses orthogonality faster), FP64 vector traffic.
+// p32only (3D fused FP32 operator): the direction is held only in FP32 (x
+// and r stay FP64), so the direction update moves 36 bytes per dof, not 48.
 int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
                        double rtol, int maxit, double delta, hipStream_t s,
                        int* iters_out, int* updates_out, int* nonfinite_out,
-                       double* rel_out) {
+                       double* rel_out, bool p32only = false) {
   Reducer red{P->red_part.p, P->red_counter.p};
   double* r = P->w_r.p;
   double* p = P->w_p.p;
@@ -883,6 +885,7 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
   // 3D FP32 fused-geometry operator: p.Ap from the apply
   const bool fused = P->npart > 0 && P->w_part.p && precision == SF_PREC_F32 &&
                      fused_f32(P);
+  if (!fused) p32only = false;
   for (;;) {
     while (!h.done && h.iters < maxit) {
       prog.tick(h.iters, bnorm > 0 ? sqrt(h.rr) / bnorm : sqrt(h.rr));
@@ -893,7 +896,10 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
           if (rc) return rc;
           SF_HIP(launch_cg_alpha_parts(P->w_part.p, P->npart, red, P->scal.p, s));
           SF_HIP(launch_cg_resid_mixed(r, q32, P->nfree, red, P->scal.p, s));
-          SF_HIP(launch_cg_xdir_mixed(x, p, p32, r, q32, P->nfree, P->scal.p, s));
+          if (p32only)
+            SF_HIP(launch_cg_xdir_p32(x, p32, r, q32, P->nfree, P->scal.p, s));
+          else
+            SF_HIP(launch_cg_xdir_mixed(x, p, p32, r, q32, P->nfree, P->scal.p, s));
           continue;
         }
         rc = do_apply_f32(P, q32, p32, precision, s, true);
@@ -922,7 +928,10 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
     }
     if (rn_new <= target || rn_new == 0.0 || h.iters >= maxit) return SF_OK;
     const double c = (h.rr_new > 0.0) ? rn_new * rn_new * h.beta / h.rr_new : 0.0;
-    SF_HIP(launch_dir_coef_mixed(p, p32, r, q32, c, P->nfree, s));
+    if (p32only)
+      SF_HIP(launch_dir_coef_p32(p32, r, q32, c, P->nfree, s));
+    else
+      SF_HIP(launch_dir_coef_mixed(p, p32, r, q32, c, P->nfree, s));
     const double t = threshold(rn_new);
     h.rr = rn_new * rn_new;
     h.stop_rr = t * t;
@@ -1441,6 +1450,8 @@ int sf_cg_mixed(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   if ((!b || !x) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
   if (!(rtol >= 0.0) || maxit < 0 || !(delta > 0.0 && delta < 1.0))
     return fail(SF_ERR_ARG, "bad rtol / maxit / delta");
+  if (fp64_recurrence < 0 || fp64_recurrence > 2)
+    return fail(SF_ERR_ARG, "fp64_recurrence must be 0, 1 or 2 (got %d)", fp64_recurrence);
   if (!is_reduced(inner_precision))
     return fail(SF_ERR_UNSUPPORTED, "inner precision %d not available", inner_precision);
   DeviceGuard g(P->device);
@@ -1455,7 +1466,7 @@ int sf_cg_mixed(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   double rel = 0.0;
   rc = fp64_recurrence
            ? cg_reliable_f64rec(P, inner_precision, b, x, rtol, maxit, delta, s, &iters,
-                                &updates, &nonfinite, &rel)
+                                &updates, &nonfinite, &rel, fp64_recurrence == 2)
            : cg_reliable(P, inner_precision, b, x, rtol, maxit, delta, s, &iters,
                          &updates, &nonfinite, &rel);
   if (rc) return rc;

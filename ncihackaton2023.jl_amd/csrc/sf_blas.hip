@@ -293,6 +293,43 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The same with the search direction held only in FP32 (x and r FP64):
+// x += alpha p32, p32 = r + beta p32, q32 = 0.  36 bytes per dof instead of
+// 48.
+__global__ void __launch_bounds__(256)
+    k_cg_xdir_p32(double* __restrict__ x, float* __restrict__ p32,
+                  const double* __restrict__ r, float* __restrict__ q32, int64_t n,
+                  const CGScalars* __restrict__ sc) {
+  if (!sc->xpend) return;
+  const double a = sc->alpha;
+  if (sc->done) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+      __builtin_nontemporal_store(fma(a, (double)p32[i], x[i]), x + i);
+    return;
+  }
+  const double b = sc->beta;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double pi = (double)p32[i];
+    __builtin_nontemporal_store(fma(a, pi, x[i]), x + i);
+    __builtin_nontemporal_store((float)fma(b, pi, r[i]), p32 + i);
+    __builtin_nontemporal_store(0.f, q32 + i);
+  }
+}
+
+// p32 = r + c p32, q32 = 0: the FP32-direction form of the direction update
+// after a residual replacement.
+__global__ void __launch_bounds__(256)
+    k_dir_coef_p32(float* __restrict__ p32, const double* __restrict__ r,
+                   float* __restrict__ q32, double c, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    p32[i] = (float)fma(c, (double)p32[i], r[i]);
+    q32[i] = 0.f;
+  }
+}
+
 template <typename V>
 __global__ void __launch_bounds__(256)
     k_cg_dir(V* __restrict__ p, const V* __restrict__ r, V* __restrict__ q,
@@ -543,6 +580,22 @@ hipError_t launch_cg_dir_mixed(double* p, float* p32, const double* r, float* q3
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_cg_dir_mixed, dim3(stream_grid(n)), dim3(256), 0, s, p, p32, r, q32,
                      n, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_xdir_p32(double* x, float* p32, const double* r, float* q32,
+                              int64_t n, const CGScalars* sc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cg_xdir_p32, dim3(stream_grid(n)), dim3(256), 0, s, x, p32, r, q32,
+                     n, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_dir_coef_p32(float* p32, const double* r, float* q32, double c,
+                               int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dir_coef_p32, dim3(stream_grid(n)), dim3(256), 0, s, p32, r, q32, c,
+                     n);
   return hipGetLastError();
 }
 

@@ -178,6 +178,12 @@ hipError_t launch_cg_resid_mixed(double* r, const float* q, int64_t n, Reducer r
                                  CGScalars* sc, hipStream_t s);
 hipError_t launch_cg_xdir_mixed(double* x, double* p, float* p32, const double* r,
                                 float* q32, int64_t n, CGScalars* sc, hipStream_t s);
+// ... with the search direction only in FP32: x += alpha p32,
+// p32 = r + beta p32, q32 = 0; and the host-coefficient form p32 = r + c p32
+hipError_t launch_cg_xdir_p32(double* x, float* p32, const double* r, float* q32,
+                              int64_t n, const CGScalars* sc, hipStream_t s);
+hipError_t launch_dir_coef_p32(float* p32, const double* r, float* q32, double c,
+                               int64_t n, hipStream_t s);
 // FP64 CG vectors with an FP32 operator (q = A32 p32):
 //   alpha_mixed : pq = p . q32, alpha = rr / pq
 //   update_mixed: x += alpha p, r -= alpha q32, rr_new = r.r, beta, done

@@ -53,20 +53,22 @@ def cg_ir_(x, A, b, rtol=1e-10, inner="f32", inner_rtol=1e-3, inner_maxiter=None
 
 
 def cg_mixed_(x, A, b, rtol=1e-10, inner="f32", delta=0.1, maxiter=None,
-              recurrence="f64", raise_on_fail=False):
+              recurrence="f64p32", raise_on_fail=False):
     """Mixed-precision CG with reliable updates: one CG recurrence on the
     reduced-precision operator (recurrence 'f32': FP32 vectors; 'f64': FP64
-    vectors, only the operator reduced), residual replaced by the FP64
+    vectors, only the operator reduced; 'f64p32': FP64 solution and residual,
+    the search direction only in FP32), residual replaced by the FP64
     b - A x whenever it has fallen by `delta` (search direction kept)."""
     _vec_ok(A, x)
     _vec_ok(A, b)
     st = _lib.SolveStats()
     prec = _lib.PRECISIONS[inner] if isinstance(inner, str) else int(inner)
     maxiter = 10 * A.n_free if maxiter is None else int(maxiter)
-    if recurrence not in ("f32", "f64"):
-        raise ValueError("recurrence must be 'f32' or 'f64'")
+    modes = {"f32": 0, "f64": 1, "f64p32": 2}
+    if recurrence not in modes:
+        raise ValueError("recurrence must be one of %s" % sorted(modes))
     rc = A._L.sf_cg_mixed(A.plan, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
-                          float(rtol), maxiter, prec, float(delta), int(recurrence == "f64"),
+                          float(rtol), maxiter, prec, float(delta), modes[recurrence],
                           ctypes.byref(st), _torch_stream(x))
     if rc != _lib.SF_OK and (raise_on_fail or rc != _lib.SF_ERR_NOT_CONVERGED):
         _lib.check(rc)

@@ -449,21 +449,22 @@ def test_mixed_cg_reliable_updates(pkg, oracle_mod, cuda, inner):
     A.free()
 
 
-@pytest.mark.parametrize("inner", ["f32", "f32_gbf16"])
-def test_mixed_cg_fp64_recurrence(pkg, oracle_mod, cuda, inner):
-    """FP64 Krylov recurrence with only the operator reduced: reaches the
-    FP64 tolerance in the FP64 residual, agrees with FP64 CG, and needs
-    about as many iterations as FP64 CG (the FP32 recurrence needs
-    noticeably more)."""
-    say("mixed CG, fp64 recurrence, inner=%s" % inner)
+@pytest.mark.parametrize("inner,rec", [("f32", "f64"), ("f32_gbf16", "f64"),
+                                       ("f32", "f64p32")])
+def test_mixed_cg_fp64_recurrence(pkg, oracle_mod, cuda, inner, rec):
+    """FP64 Krylov recurrence with only the operator reduced ('f64p32': the
+    search direction held in FP32 as well): reaches the FP64 tolerance in
+    the FP64 residual, agrees with FP64 CG, and needs about as many
+    iterations as FP64 CG (the FP32 recurrence needs noticeably more)."""
+    say("mixed CG, %s recurrence, inner=%s" % (rec, inner))
     model, space, mp, A = make(pkg, 3, 6, 5, 7)
     b = cuda.from_numpy(pkg.normal(3, space.n_free)).cuda()
     x64 = cuda.zeros_like(b)
     s64 = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=20000)
     assert s64["converged"] == 1
     x = cuda.zeros_like(b)
-    st = pkg.cg_mixed_(x, A, b, rtol=1e-10, inner=inner, delta=0.1, recurrence="f64")
-    say("f64", s64, "mixed/f64 recurrence", st)
+    st = pkg.cg_mixed_(x, A, b, rtol=1e-10, inner=inner, delta=0.1, recurrence=rec)
+    say("f64", s64, "mixed/%s recurrence" % rec, st)
     assert st["converged"] == 1 and st["rel_residual"] <= 1e-10
     assert st["outer_iterations"] >= 5
     assert st["iterations"] < 1.25 * s64["iterations"]
@@ -474,7 +475,7 @@ def test_mixed_cg_fp64_recurrence(pkg, oracle_mod, cuda, inner):
     assert rel_l2(xh, x64.cpu().numpy()) < 1e-7
     # nonzero initial guess: continues from it
     x2 = 0.5 * x64.clone()
-    st3 = pkg.cg_mixed_(x2, A, b, rtol=1e-10, inner=inner, recurrence="f64")
+    st3 = pkg.cg_mixed_(x2, A, b, rtol=1e-10, inner=inner, recurrence=rec)
     assert st3["converged"] == 1
     assert rel_l2(x2.cpu().numpy(), x64.cpu().numpy()) < 1e-7
     with pytest.raises(ValueError):
