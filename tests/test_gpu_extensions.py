@@ -480,4 +480,8 @@ def test_mixed_cg_fp64_recurrence(pkg, oracle_mod, cuda, inner, rec):
     assert rel_l2(x2.cpu().numpy(), x64.cpu().numpy()) < 1e-7
     with pytest.raises(ValueError):
         pkg.cg_mixed_(x, A, b, recurrence="f16")
+    import ctypes
+    rc_bad = A._L.sf_cg_mixed(A.plan, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                              1e-10, 10, pkg._lib.PRECISIONS["f32"], 0.1, 3, None, None)
+    assert rc_bad == pkg._lib.SF_ERR_ARG
     A.free()
