@@ -316,8 +316,9 @@ def mode_cg(args, torch, pkg):
     res["cg_maxiter10_ms"] = {"min": min(times) * 1e3, "max": max(times) * 1e3}
     log("cg maxiter=10", res["cg_maxiter10_ms"])
     x64 = torch.zeros_like(b)
-    st = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=args.maxit)
+    st = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=args.maxit, Pl=args.pl)
     res["f64"] = st
+    res["preconditioner"] = args.pl
     log("f64", st)
     for spec in [t for t in args.mixed.split(",") if t]:
         parts = spec.split(":")
@@ -325,7 +326,7 @@ def mode_cg(args, torch, pkg):
         rec = parts[2] if len(parts) > 2 else "f32"
         x = torch.zeros_like(b)
         st = pkg.cg_mixed_(x, A, b, rtol=1e-10, inner=inner, delta=float(delta),
-                           maxiter=args.maxit, recurrence=rec)
+                           maxiter=args.maxit, recurrence=rec, Pl=args.pl)
         st["err_vs_f64"] = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
         st["speedup_vs_f64"] = res["f64"]["seconds"] / st["seconds"]
         res["mixed_%s_delta%s_rec%s" % (inner, delta, rec)] = st
@@ -336,7 +337,7 @@ def mode_cg(args, torch, pkg):
         x = torch.zeros_like(b)
         try:
             st = pkg.cg_ir_(x, A, b, rtol=1e-10, inner=inner, inner_rtol=irt,
-                            inner_maxiter=args.maxit, max_outer=args.max_outer)
+                            inner_maxiter=args.maxit, max_outer=args.max_outer, Pl=args.pl)
         except pkg.SFError as e:  # e.g. refinement stalls with bf16 operands
             res["%s_inner%g" % (inner, irt)] = {"error": str(e)}
             log(inner, irt, "failed:", e)
@@ -521,16 +522,25 @@ def mode_q9(args, torch, pkg):
     # log-normal kappa multiplies the condition number by about
     # max(kappa) / min(kappa): both solves run under the same iteration cap
     # and report the residual they reach
-    st = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=args.q9_maxit)
+    st = pkg.cg_(x64, A, b, rtol=1e-10, maxiter=args.q9_maxit, Pl=args.pl)
     res["cg_f64"] = st
+    res["preconditioner"] = args.pl
     log("cg f64", st)
-    for inner in ("f32",):
+    for rec in [t for t in args.q9_mixed.split(",") if t]:
         xm = torch.zeros_like(b)
-        st = pkg.cg_mixed_(xm, A, b, rtol=1e-10, inner=inner, delta=0.1, maxiter=args.q9_maxit)
+        st = pkg.cg_mixed_(xm, A, b, rtol=1e-10, inner="f32", delta=0.1, maxiter=args.q9_maxit,
+                           recurrence=rec, Pl=args.pl)
         st["err_vs_f64"] = float(torch.linalg.norm(xm - x64) / torch.linalg.norm(x64))
         st["speedup_vs_f64"] = res["cg_f64"]["seconds"] / st["seconds"]
-        res["cg_mixed_" + inner] = st
-        log("cg mixed", inner, st)
+        res["cg_mixed_f32_rec" + rec] = st
+        log("cg mixed f32", rec, st)
+    xi = torch.zeros_like(b)
+    st = pkg.cg_ir_(xi, A, b, rtol=1e-10, inner="f32", inner_rtol=1e-2,
+                    inner_maxiter=args.q9_maxit, max_outer=args.max_outer, Pl=args.pl)
+    st["err_vs_f64"] = float(torch.linalg.norm(xi - x64) / torch.linalg.norm(x64))
+    st["speedup_vs_f64"] = res["cg_f64"]["seconds"] / st["seconds"]
+    res["cg_ir_f32"] = st
+    log("cg refinement f32", st)
     # adjoint of J = 1/2 |A(kappa) u - d|^2
     d = torch.from_numpy(pkg.uniform(4, nf, -1.0, 1.0)).cuda()
     A.mul(y, x)
@@ -585,7 +595,12 @@ def main():
     ap.add_argument("--fd-dirs", type=int, default=64,
                     help="adjoint mode: central-difference directions in kappa and in u")
     ap.add_argument("--q9-maxit", type=int, default=40000,
-                    help="q9 mode: iteration cap of both CG solves")
+                    help="q9 mode: iteration cap of the CG solves")
+    ap.add_argument("--q9-mixed", default="f64p32,f64,f32",
+                    help="q9 mode: mixed-CG recurrences to run (comma list)")
+    ap.add_argument("--pl", default=None, choices=[None, "jacobi", "gll"],
+                    help="cg / q9 modes: preconditioner of every solve (None: the "
+                         "reference's unpreconditioned CG)")
     ap.add_argument("--inner", default="f32:1e-2,f32_gbf16:1e-2,mfma_bf16x3:1e-2,mfma_bf16:1e-2",
                     help="cg mode: restarted refinement runs, comma list of "
                          "inner_precision:inner_rtol")

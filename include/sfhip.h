@@ -60,6 +60,18 @@ enum sf_precision {
                             tile): about 16 significant bits                     */
 };
 
+/* Preconditioner of the device-resident solvers (sf_cg, sf_cg_mixed,
+ * sf_cg_ir), selected per plan with sf_plan_set_preconditioner. */
+enum sf_preconditioner {
+  SF_PC_NONE = 0,   /* unpreconditioned CG, as the reference's cg! call       */
+  SF_PC_JACOBI = 1, /* z = diag(A)^{-1} r: the matrix-free diagonal of A(kappa),
+                       inverted once in FP32 and used by every solver form     */
+  SF_PC_GLL = 2     /* 3D: Jacobi in the Gauss-Lobatto nodal basis of the same
+                       space, z = T diag(T^T A T)^{-1} T^T r with T the
+                       tensor-product interpolation from the Gauss-Lobatto to
+                       the equispaced nodes (two block-assembled passes)      */
+};
+
 typedef struct sf_plan sf_plan; /* opaque */
 
 typedef struct sf_plan_info {
@@ -211,6 +223,28 @@ int sf_adjoint(sf_plan* plan, const double* u, const double* lam,
                double* dJdkappa, double* dJdu, void* stream);
 
 /* ---- solvers (device pointers, FP64 b and x) --------------------------- */
+
+/* diag = the diagonal of the assembled operator A(kappa) on the free dofs
+ * (FP64, device pointer, overwritten, stream ordered), formed matrix-free
+ * from the sum-factorised element-matrix diagonals.  The diag(A) a caller
+ * of IterativeSolvers.cg!(x, A, b; Pl = ...) would take from the assembled
+ * matrix (profiling/drivers.jl:7-26); the reference's lazy operator has no
+ * counterpart. */
+int sf_diagonal(sf_plan* plan, double* diag, void* stream);
+
+/* Preconditioner of sf_cg, sf_cg_mixed and sf_cg_ir on this plan
+ * (SF_PC_NONE or SF_PC_JACOBI): the Pl keyword of IterativeSolvers.cg!,
+ * which the reference's compute_solution leaves at the identity
+ * (profiling/drivers.jl:85-95).  The Jacobi vector is built at the next
+ * solve and again after any change of the operator (tables, cells,
+ * geometry, kappa). */
+int sf_plan_set_preconditioner(sf_plan* plan, int kind);
+
+/* z = M^{-1} r for the plan's preconditioner (FP64 device vectors, z
+ * overwritten, stream ordered): r itself without one, diag(A)^{-1} r for
+ * Jacobi, T diag(T^T A T)^{-1} T^T r for SF_PC_GLL.  The ldiv!(z, Pl, r)
+ * that IterativeSolvers.cg! applies to its Pl in every iteration. */
+int sf_precond_apply(sf_plan* plan, double* z, const double* r, void* stream);
 
 /* Unpreconditioned CG on the FP64 operator, x used as the initial guess,
  * stop when ||r|| <= rtol ||b||.  Replaces IterativeSolvers.cg!(x, A, b)

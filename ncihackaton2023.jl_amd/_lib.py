@@ -34,6 +34,12 @@ SF_PREC_MFMA_BF16 = 3
 SF_PREC_MFMA_BF16X3 = 4
 SF_PREC_MFMA_BF16X2 = 5
 
+SF_PC_NONE = 0
+SF_PC_JACOBI = 1
+SF_PC_GLL = 2
+PRECONDITIONERS = {None: SF_PC_NONE, "none": SF_PC_NONE, "jacobi": SF_PC_JACOBI,
+                   "gll": SF_PC_GLL}
+
 PRECISIONS = {"f64": SF_PREC_F64, "f32": SF_PREC_F32, "f32_gbf16": SF_PREC_F32_GBF16,
               "mfma_bf16": SF_PREC_MFMA_BF16, "mfma_bf16x2": SF_PREC_MFMA_BF16X2,
               "mfma_bf16x3": SF_PREC_MFMA_BF16X3}
@@ -92,6 +98,9 @@ SIGNATURES = [
     ("sf_apply_host", _i, [_vp, _vp, _vp]),
     ("sf_apply_f32", _i, [_vp, _vp, _vp, _i, _vp]),
     ("sf_adjoint", _i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("sf_diagonal", _i, [_vp, _vp, _vp]),
+    ("sf_plan_set_preconditioner", _i, [_vp, _i]),
+    ("sf_precond_apply", _i, [_vp, _vp, _vp, _vp]),
     ("sf_cg", _i, [_vp, _vp, _vp, _d, _i, ctypes.POINTER(SolveStats), _vp]),
     ("sf_cg_ir", _i, [_vp, _vp, _vp, _d, _i, _i, _d, _i, ctypes.POINTER(SolveStats), _vp]),
     ("sf_cg_mixed", _i, [_vp, _vp, _vp, _d, _i, _i, _d, _i, ctypes.POINTER(SolveStats), _vp]),

@@ -9,7 +9,10 @@ Julia-side binding is an rrule for ``apply(A, kappa, u)``): it returns
     ubar               = A(kappa)^T ybar = A(kappa) ybar,
 
 both from one fused device kernel (sf_adjoint) that recomputes the
-quadrature-point gradients instead of storing them.
+quadrature-point gradients instead of storing them.  The pullback holds the
+kappa of its forward call: if the plan's kappa has changed since (another
+apply, a taped pass over several kappas), it is set back before the adjoint
+runs, so ubar is always A(kappa of the forward) ybar.
 """
 import ctypes
 
@@ -41,9 +44,14 @@ def pullback(A, u, ybar, want_kappa=True, want_u=True):
 
 def rrule(A, kappa, u):
     """(y, pullback) for y = A(kappa) u."""
-    y = apply(A, kappa, u)
+    import numpy as np
+    k = np.array(kappa, dtype=np.float64, copy=True).reshape(-1)
+    y = apply(A, k, u)
+    version = A.kappa_version
 
     def _pullback(ybar):
+        if A.kappa_version != version:
+            A.set_kappa(k)
         return pullback(A, u, ybar)
 
     return y, _pullback

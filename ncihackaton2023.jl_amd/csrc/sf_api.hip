@@ -148,6 +148,51 @@ int build_tables(int p, int q, double* B, double* Dm, double* pts,
   return SF_OK;
 }
 
+// Gauss-Lobatto-Legendre points of degree p on [0, 1] in the reference's
+// node order [0, 1, interior ascending]: the roots of (1 - x^2) P_p'(x) by
+// the Newton iteration x <- x - (x P_p - P_{p-1}) / ((p + 1) P_p) from the
+// Chebyshev-Lobatto points cos(pi k / p).
+std::vector<double> lobatto_nodes(int p) {
+  std::vector<double> x(p + 1);
+  const double pi = 3.14159265358979323846;
+  for (int k = 0; k <= p; ++k) {
+    double z = cos(pi * k / p);
+    for (int it = 0; it < 100; ++it) {
+      double pn, pm;
+      legendre_pair(p, z, &pn, &pm);
+      const double dz = (z * pn - pm) / ((p + 1) * pn);
+      z -= dz;
+      if (fabs(dz) <= 1e-16) break;
+    }
+    x[k] = 0.5 * (z + 1.0);
+  }
+  std::sort(x.begin(), x.end());
+  std::vector<double> t;
+  t.push_back(0.0);
+  t.push_back(1.0);
+  for (int k = 1; k < p; ++k) t.push_back(x[k]);
+  return t;
+}
+
+// T1[a][b] = l_b(t_a): the GLL Lagrange polynomials (nodes s) at the
+// equispaced nodes t, both in the reference's order.  Rows of the two
+// vertex nodes are exact unit rows.
+std::vector<double> gll_to_equispaced(int p) {
+  const int n = p + 1;
+  const std::vector<double> s = lobatto_nodes(p), t = lagrange_nodes(p);
+  std::vector<double> T(n * n);
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) {
+      double v = 1.0;
+      for (int m = 0; m < n; ++m)
+        if (m != b) v *= (t[a] - s[m]) / (s[b] - s[m]);
+      T[a * n + b] = v;
+    }
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < n; ++b) T[a * n + b] = (a == b) ? 1.0 : 0.0;
+  return T;
+}
+
 // ---------------------------------------------------------------------------
 // device buffer helper
 
@@ -230,6 +275,18 @@ struct sf_plan {
   DevBuf<unsigned> red_counter;
   DevBuf<CGScalars> scal;
   DevBuf<double> dscal;  // dot results
+  // preconditioner of the solvers (sf_plan_set_preconditioner): the kind,
+  // and the FP32 inverse diagonal, rebuilt when the operator changes
+  int pc_kind = SF_PC_NONE;
+  bool pc_valid = false;
+  DevBuf<float> dinv;  // Jacobi: 1 / diag(A); GLL: 1 / diag(T^T A T)
+  // GLL preconditioner: the n x n table and its transpose, 1 / (cells per
+  // dof), and the z / scratch vectors of the solvers
+  DevBuf<double> dT1, dT1t;
+  DevBuf<float> minv;
+  DevBuf<double> w_z, w_w;
+  DevBuf<float> w_z32, w_w32;
+  int max_nu = 0;  // most distinct dofs of one block
   hipStream_t own_stream = nullptr;
 
   const double* active_G() const { return has_kappa ? Gk.p : G.p; }
@@ -248,7 +305,9 @@ struct sf_plan {
     w_r.release(); w_p.release(); w_q.release(); w_t.release();
     w_r32.release(); w_s32.release(); w_d32.release(); w_p32.release(); w_q32.release();
     red_part.release(); w_part.release(); red_counter.release(); scal.release();
-    dscal.release();
+    dscal.release(); dinv.release();
+    dT1.release(); dT1t.release(); minv.release();
+    w_z.release(); w_w.release(); w_z32.release(); w_w32.release();
     if (own_stream) (void)hipStreamDestroy(own_stream);
     own_stream = nullptr;
   }
@@ -263,7 +322,8 @@ struct sf_plan {
            xs.bytes() + ys.bytes() + w_r.bytes() + w_p.bytes() + w_q.bytes() +
            w_t.bytes() + w_r32.bytes() + w_s32.bytes() + w_d32.bytes() + w_p32.bytes() +
            w_q32.bytes() + red_part.bytes() + w_part.bytes() + red_counter.bytes() +
-           scal.bytes() + dscal.bytes();
+           scal.bytes() + dscal.bytes() + dinv.bytes() + dT1.bytes() + dT1t.bytes() +
+           minv.bytes() + w_z.bytes() + w_w.bytes() + w_z32.bytes() + w_w32.bytes();
   }
 };
 
@@ -569,6 +629,9 @@ int build_blocks(sf_plan* P, const std::vector<int32_t>& h) {
   if (E > 0) SF_HIP(hipMemcpy(P->cell_block.p, cblk.data(), E * 4, hipMemcpyHostToDevice));
   P->eb = eb;
   P->nblk = (int)nb;
+  P->max_nu = 0;
+  for (int64_t b = 0; b < nb; ++b)
+    P->max_nu = std::max<int>(P->max_nu, (int)(off[b + 1] - off[b]));
   P->npart = nb * (nt / 64);
   P->kargs = BlockArgs{P->pidx.p, P->bdofs.p, P->boff.p, P->bexcl.p, P->bcell.p, (int)nb};
   return SF_OK;
@@ -580,6 +643,14 @@ int ensure_solver_ws(sf_plan* P, bool mixed) {
   SF_HIP(P->w_p.alloc(P->nfree));
   SF_HIP(P->w_q.alloc(P->nfree));
   SF_HIP(P->w_t.alloc(P->nfree));
+  if (P->pc_kind == SF_PC_GLL) {
+    SF_HIP(P->w_z.alloc(P->nfree));
+    SF_HIP(P->w_w.alloc(P->nfree));
+    if (mixed) {
+      SF_HIP(P->w_z32.alloc(P->nfree));
+      SF_HIP(P->w_w32.alloc(P->nfree));
+    }
+  }
   if (mixed) {
     SF_HIP(P->w_r32.alloc(P->nfree));
     SF_HIP(P->w_s32.alloc(P->nfree));
@@ -588,12 +659,107 @@ int ensure_solver_ws(sf_plan* P, bool mixed) {
     SF_HIP(P->w_q32.alloc(P->nfree));
   }
   if (!P->red_part.p) {
-    SF_HIP(P->red_part.alloc(kPartSlots));
+    SF_HIP(P->red_part.alloc(2 * kPartSlots));
     SF_HIP(P->red_counter.alloc(1));
     SF_HIP(hipMemset(P->red_counter.p, 0, sizeof(unsigned)));
     SF_HIP(P->scal.alloc(1));
     SF_HIP(P->dscal.alloc(4));
   }
+  return SF_OK;
+}
+
+// diag = diag(A) on the free dofs (FP64, overwritten) on stream s.
+int compute_diagonal(sf_plan* P, double* diag, hipStream_t s) {
+  if (P->nfree == 0) return SF_OK;
+  SF_HIP(launch_fill<double>(diag, P->nfree, 0.0, s));
+  hipError_t e = launch_diagonal(P->dim, P->n, P->q, P->active_G(), P->dB.p, P->dD.p,
+                                 P->cells.p, P->cell_block.p, P->bcell.p, P->ncell, diag, s);
+  if (e == hipErrorNotSupported)
+    return fail(SF_ERR_UNSUPPORTED, "no diagonal kernel for n=%d q=%d", P->n, P->q);
+  SF_HIP(e);
+  return SF_OK;
+}
+
+// The solvers' preconditioner, rebuilt after the operator changed.  Jacobi:
+// *dinv = the FP32 inverse diagonal; GLL: *gll = true (the passes read the
+// plan's tables); none: both unset.
+int ensure_pc(sf_plan* P, hipStream_t s, const float** dinv, bool* gll = nullptr) {
+  *dinv = nullptr;
+  if (gll) *gll = false;
+  if (P->pc_kind == SF_PC_NONE || P->nfree == 0) return SF_OK;
+  if (P->pc_kind == SF_PC_GLL && !gll)
+    return fail(SF_ERR_UNSUPPORTED, "the Gauss-Lobatto preconditioner is not available here");
+  if (!P->pc_valid) {
+    DevBuf<double> d, tb, td;
+    SF_HIP(d.alloc(P->nfree));
+    SF_HIP(P->dinv.alloc(P->nfree));
+    int rc = SF_OK;
+    if (P->pc_kind == SF_PC_GLL) {
+      // T1, and the GLL basis at the quadrature points: B T1, D T1 (q x n)
+      const int n = P->n, q = P->q;
+      const std::vector<double> T = gll_to_equispaced(P->order);
+      std::vector<double> Tt(n * n), bg(2 * q * n), dg(2 * q * n);
+      for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) Tt[b * n + a] = T[a * n + b];
+      for (int i = 0; i < q; ++i)
+        for (int b = 0; b < n; ++b) {
+          double vb = 0.0, vd = 0.0;
+          for (int a = 0; a < n; ++a) {
+            vb += P->hB[i * n + a] * T[a * n + b];
+            vd += P->hD[i * n + a] * T[a * n + b];
+          }
+          bg[i * n + b] = vb;
+          dg[i * n + b] = vd;
+        }
+      SF_HIP(P->dT1.alloc(n * n));
+      SF_HIP(P->dT1t.alloc(n * n));
+      SF_HIP(tb.alloc(q * n));
+      SF_HIP(td.alloc(q * n));
+      SF_HIP(P->minv.alloc(P->nfree));
+      SF_HIP(hipMemcpy(P->dT1.p, T.data(), n * n * 8, hipMemcpyHostToDevice));
+      SF_HIP(hipMemcpy(P->dT1t.p, Tt.data(), n * n * 8, hipMemcpyHostToDevice));
+      SF_HIP(hipMemcpy(tb.p, bg.data(), q * n * 8, hipMemcpyHostToDevice));
+      SF_HIP(hipMemcpy(td.p, dg.data(), q * n * 8, hipMemcpyHostToDevice));
+      SF_HIP(hipMemsetAsync(P->minv.p, 0, P->nfree * sizeof(float), s));
+      SF_HIP(launch_mult_inverse(P->cells.p, P->ncell * P->nloc, P->minv.p, P->nfree, s));
+      SF_HIP(launch_fill<double>(d.p, P->nfree, 0.0, s));
+      hipError_t e = launch_diagonal(P->dim, n, q, P->active_G(), tb.p, td.p, P->cells.p,
+                                     P->cell_block.p, P->bcell.p, P->ncell, d.p, s);
+      if (e == hipErrorNotSupported)
+        rc = fail(SF_ERR_UNSUPPORTED, "no diagonal kernel for n=%d q=%d", n, q);
+      else if (e != hipSuccess)
+        rc = fail(SF_ERR_HIP, "diagonal kernel: %s", hipGetErrorString(e));
+    } else {
+      rc = compute_diagonal(P, d.p, s);
+    }
+    if (rc) return rc;
+    hipError_t e = hipMemsetAsync(P->flag.p, 0, sizeof(int), s);
+    if (e == hipSuccess) e = launch_inv_diag(d.p, P->dinv.p, P->nfree, P->flag.p, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    SF_HIP(e);
+    int bad = 0;
+    rc = read_flag(P, &bad);
+    if (rc) return rc;
+    if (bad) return fail(SF_ERR_NONFINITE, "preconditioner diagonal not positive and finite");
+    P->pc_valid = true;
+  }
+  if (P->pc_kind == SF_PC_GLL) {
+    *gll = true;
+    return SF_OK;
+  }
+  *dinv = P->dinv.p;
+  return SF_OK;
+}
+
+// z = M^{-1} r by the GLL passes (w: scratch), skipped once sc->done.
+template <typename V>
+int pc_gll(sf_plan* P, const V* r, V* z, V* w, const CGScalars* sc, hipStream_t s) {
+  hipError_t e = launch_pc_gll<V, V>(P->n, r, z, w, P->nfree, P->minv.p, P->dinv.p, P->dT1.p,
+                                     P->dT1t.p, P->kargs, P->eb, P->max_nu, sc, s);
+  if (e == hipErrorNotSupported)
+    return fail(SF_ERR_UNSUPPORTED, "no Gauss-Lobatto preconditioner kernel for n=%d "
+                "(or its blocks exceed the workgroup's LDS)", P->n);
+  SF_HIP(e);
   return SF_OK;
 }
 
@@ -638,7 +804,7 @@ struct Progress {
 // iterations are queued between host checks of the device scalars; once
 // converged the remaining queued updates are no-ops.
 int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
-           hipStream_t s, int* iters_out, int* nonfinite_out) {
+           hipStream_t s, int* iters_out, int* nonfinite_out, const float* dinv, bool gll) {
   Reducer red{P->red_part.p, P->red_counter.p};
   double bb = 0.0;
   int rc = dot_host(P, b, b, &bb, s);
@@ -647,31 +813,60 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   rc = do_apply_f64(P, P->w_t.p, x, s);
   if (rc) return rc;
   SF_HIP(launch_cg_start<double>(b, P->w_t.p, P->w_r.p, P->w_p.p, P->w_q.p,
-                                 P->nfree, red, P->scal.p, stop_rr, s));
+                                 P->nfree, red, P->scal.p, stop_rr, s, dinv));
+  double* const z = P->w_z.p;
+  if (gll) {  // p = z = M^{-1} r, rr = r.z
+    rc = pc_gll<double>(P, P->w_r.p, z, P->w_w.p, P->scal.p, s);
+    if (rc) return rc;
+    SF_HIP((launch_cg_set_rz<double, double>(P->w_r.p, z, P->nfree, red, P->scal.p, s)));
+    SF_HIP(hipMemcpyAsync(P->w_p.p, z, P->nfree * 8, hipMemcpyDeviceToDevice, s));
+  }
   CGScalars h{};
   rc = read_scalars(P, &h, s);
   if (rc) return rc;
-  Progress prog("cg");
+  Progress prog(gll ? "cg (gll)" : dinv ? "cg (jacobi)" : "cg");
   const int chunk = 8;
   // 3D fused-geometry apply: p.Ap from the apply (alpha_parts / resid / xdir)
   const bool fused = P->npart > 0 && P->w_part.p && fused_f64(P);
   while (!h.done && h.iters < maxit) {
-    prog.tick(h.iters, bb > 0 ? sqrt(h.rr / bb) : sqrt(h.rr));
+    prog.tick(h.iters, bb > 0 ? sqrt(h.res2 / bb) : sqrt(h.res2));
     const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
     for (int k = 0; k < todo; ++k) {
       if (fused) {
         rc = do_apply_f64(P, P->w_q.p, P->w_p.p, s, 0, true, P->w_part.p);
         if (rc) return rc;
         SF_HIP(launch_cg_alpha_parts(P->w_part.p, P->npart, red, P->scal.p, s));
-        SF_HIP(launch_cg_resid<double>(P->w_r.p, P->w_q.p, P->nfree, red, P->scal.p, s));
-        SF_HIP(launch_cg_xdir<double>(x, P->w_p.p, P->w_r.p, P->w_q.p, P->nfree, P->scal.p, s));
+        if (gll) {
+          SF_HIP((launch_cg_resid_shares<double, double>(P->w_r.p, P->w_q.p, P->nfree, red,
+                                                         P->scal.p, s)));
+          rc = pc_gll<double>(P, P->w_r.p, z, P->w_w.p, P->scal.p, s);
+          if (rc) return rc;
+          SF_HIP((launch_rz_shares<double, double>(P->w_r.p, z, P->nfree, red, P->scal.p, s)));
+          SF_HIP(launch_cg_finish(P->nfree, red, P->scal.p, s));
+          SF_HIP(launch_cg_xdir<double>(x, P->w_p.p, z, P->w_q.p, P->nfree, P->scal.p, s));
+          continue;
+        }
+        SF_HIP(launch_cg_resid<double>(P->w_r.p, P->w_q.p, P->nfree, red, P->scal.p, s, dinv));
+        SF_HIP(launch_cg_xdir<double>(x, P->w_p.p, P->w_r.p, P->w_q.p, P->nfree, P->scal.p, s,
+                                      dinv));
         continue;
       }
       rc = do_apply_f64(P, P->w_q.p, P->w_p.p, s, 0, true);
       if (rc) return rc;
       SF_HIP(launch_cg_alpha<double>(P->w_p.p, P->w_q.p, P->nfree, red, P->scal.p, s));
-      SF_HIP(launch_cg_update<double>(x, P->w_r.p, P->w_p.p, P->w_q.p, P->nfree, red, P->scal.p, s));
-      SF_HIP(launch_cg_dir<double>(P->w_p.p, P->w_r.p, P->w_q.p, P->nfree, P->scal.p, s));
+      if (gll) {
+        SF_HIP((launch_cg_update_shares<double, double>(x, P->w_r.p, P->w_p.p, P->w_q.p,
+                                                        P->nfree, red, P->scal.p, s)));
+        rc = pc_gll<double>(P, P->w_r.p, z, P->w_w.p, P->scal.p, s);
+        if (rc) return rc;
+        SF_HIP((launch_rz_shares<double, double>(P->w_r.p, z, P->nfree, red, P->scal.p, s)));
+        SF_HIP(launch_cg_finish(P->nfree, red, P->scal.p, s));
+        SF_HIP(launch_cg_dir<double>(P->w_p.p, z, P->w_q.p, P->nfree, P->scal.p, s));
+        continue;
+      }
+      SF_HIP(launch_cg_update<double>(x, P->w_r.p, P->w_p.p, P->w_q.p, P->nfree, red, P->scal.p,
+                                      s, dinv));
+      SF_HIP(launch_cg_dir<double>(P->w_p.p, P->w_r.p, P->w_q.p, P->nfree, P->scal.p, s, dinv));
     }
     rc = read_scalars(P, &h, s);
     if (rc) return rc;
@@ -685,7 +880,8 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
 // b32 = w_r32 of unit norm, d = 0 initially, to ||r|| <= inner_rtol or
 // maxit iterations.
 int cg_f32_inner(sf_plan* P, int precision, double inner_rtol, int maxit,
-                 hipStream_t s, int* iters_out, int* nonfinite_out) {
+                 hipStream_t s, int* iters_out, int* nonfinite_out, const float* dinv,
+                 bool gll) {
   Reducer red{P->red_part.p, P->red_counter.p};
   const float* b32 = P->w_r32.p;
   float* r = P->w_s32.p;
@@ -694,21 +890,37 @@ int cg_f32_inner(sf_plan* P, int precision, double inner_rtol, int maxit,
   float* q = P->w_q32.p;
   SF_HIP(launch_fill<float>(d, P->nfree, 0.0f, s));
   SF_HIP(launch_cg_start<float>(b32, d, r, p, q, P->nfree, red, P->scal.p,
-                                inner_rtol * inner_rtol, s));
+                                inner_rtol * inner_rtol, s, dinv));
+  float* const z = P->w_z32.p;
+  if (gll) {
+    int rc0 = pc_gll<float>(P, r, z, P->w_w32.p, P->scal.p, s);
+    if (rc0) return rc0;
+    SF_HIP((launch_cg_set_rz<float, float>(r, z, P->nfree, red, P->scal.p, s)));
+    SF_HIP(hipMemcpyAsync(p, z, P->nfree * 4, hipMemcpyDeviceToDevice, s));
+  }
   CGScalars h{};
   int rc = read_scalars(P, &h, s);
   if (rc) return rc;
   Progress prog("refinement inner cg");
   const int chunk = 8;
   while (!h.done && h.iters < maxit) {
-    prog.tick(h.iters, sqrt(h.rr));
+    prog.tick(h.iters, sqrt(h.res2));
     const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
     for (int k = 0; k < todo; ++k) {
       rc = do_apply_f32(P, q, p, precision, s, true);
       if (rc) return rc;
       SF_HIP(launch_cg_alpha<float>(p, q, P->nfree, red, P->scal.p, s));
-      SF_HIP(launch_cg_update<float>(d, r, p, q, P->nfree, red, P->scal.p, s));
-      SF_HIP(launch_cg_dir<float>(p, r, q, P->nfree, P->scal.p, s));
+      if (gll) {
+        SF_HIP((launch_cg_update_shares<float, float>(d, r, p, q, P->nfree, red, P->scal.p, s)));
+        rc = pc_gll<float>(P, r, z, P->w_w32.p, P->scal.p, s);
+        if (rc) return rc;
+        SF_HIP((launch_rz_shares<float, float>(r, z, P->nfree, red, P->scal.p, s)));
+        SF_HIP(launch_cg_finish(P->nfree, red, P->scal.p, s));
+        SF_HIP(launch_cg_dir<float>(p, z, q, P->nfree, P->scal.p, s));
+        continue;
+      }
+      SF_HIP(launch_cg_update<float>(d, r, p, q, P->nfree, red, P->scal.p, s, dinv));
+      SF_HIP(launch_cg_dir<float>(p, r, q, P->nfree, P->scal.p, s, dinv));
     }
     rc = read_scalars(P, &h, s);
     if (rc) return rc;
@@ -759,7 +971,7 @@ int true_residual(sf_plan* P, const double* b, const double* x, double* rn,
 int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
                 double rtol, int maxit, double delta, hipStream_t s,
                 int* iters_out, int* updates_out, int* nonfinite_out,
-                double* rel_out) {
+                double* rel_out, const float* dinv, bool gll) {
   Reducer red{P->red_part.p, P->red_counter.p};
   float* r = P->w_s32.p;
   float* xl = P->w_d32.p;
@@ -785,7 +997,14 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
   SF_HIP(launch_fill<float>(xl, P->nfree, 0.0f, s));
   const double t0 = threshold(rn);
   SF_HIP(launch_cg_start<float>(P->w_r32.p, xl, r, p, q, P->nfree, red, P->scal.p,
-                                t0 * t0, s));
+                                t0 * t0, s, dinv));
+  float* const z = P->w_z32.p;
+  if (gll) {
+    rc = pc_gll<float>(P, r, z, P->w_w32.p, P->scal.p, s);
+    if (rc) return rc;
+    SF_HIP((launch_cg_set_rz<float, float>(r, z, P->nfree, red, P->scal.p, s)));
+    SF_HIP(hipMemcpyAsync(p, z, P->nfree * 4, hipMemcpyDeviceToDevice, s));
+  }
   CGScalars h{};
   rc = read_scalars(P, &h, s);
   if (rc) return rc;
@@ -793,14 +1012,24 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
   Progress prog("mixed cg");
   for (;;) {
     while (!h.done && h.iters < maxit) {
-      prog.tick(h.iters, (bnorm > 0 ? rn / bnorm : rn) * sqrt(h.rr));
+      prog.tick(h.iters, (bnorm > 0 ? rn / bnorm : rn) * sqrt(h.res2));
       const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
       for (int k = 0; k < todo; ++k) {
         rc = do_apply_f32(P, q, p, precision, s, true);
         if (rc) return rc;
         SF_HIP(launch_cg_alpha<float>(p, q, P->nfree, red, P->scal.p, s));
-        SF_HIP(launch_cg_update<float>(xl, r, p, q, P->nfree, red, P->scal.p, s));
-        SF_HIP(launch_cg_dir<float>(p, r, q, P->nfree, P->scal.p, s));
+        if (gll) {
+          SF_HIP((launch_cg_update_shares<float, float>(xl, r, p, q, P->nfree, red, P->scal.p,
+                                                        s)));
+          rc = pc_gll<float>(P, r, z, P->w_w32.p, P->scal.p, s);
+          if (rc) return rc;
+          SF_HIP((launch_rz_shares<float, float>(r, z, P->nfree, red, P->scal.p, s)));
+          SF_HIP(launch_cg_finish(P->nfree, red, P->scal.p, s));
+          SF_HIP(launch_cg_dir<float>(p, z, q, P->nfree, P->scal.p, s));
+          continue;
+        }
+        SF_HIP(launch_cg_update<float>(xl, r, p, q, P->nfree, red, P->scal.p, s, dinv));
+        SF_HIP(launch_cg_dir<float>(p, r, q, P->nfree, P->scal.p, s, dinv));
       }
       rc = read_scalars(P, &h, s);
       if (rc) return rc;
@@ -823,17 +1052,32 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
     }
     if (rn_new <= target || rn_new == 0.0 || h.iters >= maxit) return SF_OK;
     // continue the recurrence in units of rn_new; the queued pieces after
-    // `done` were no-ops, so p is the last direction and q is stale
-    const double c = (h.rr_new > 0.0) ? (rn_new / rn) * h.beta / h.rr_new : 0.0;
+    // `done` were no-ops, so p is the last direction and q is stale.  In
+    // the new units p <- z + c p with c = (rn_new / rn) rz / rz_{k-1}
+    // (rz = r.z of the replaced residual in the new units, 1 without a
+    // preconditioner; rz_{k-1} = rr_new / beta in the old units).
     SF_HIP(launch_scale_to_f32(P->w_r.p, 1.0 / rn_new, r, P->nfree, s));
-    SF_HIP(launch_dir_coef(p, r, q, (float)c, P->nfree, s));
-    SF_HIP(launch_fill<float>(xl, P->nfree, 0.0f, s));
-    double rr = 0.0;
+    double rr = 0.0, rz = 0.0;
     rc = dot_host_f32(P, r, r, &rr, s);
     if (rc) return rc;
+    rz = rr;
+    if (dinv) {
+      SF_HIP(launch_wdot<float>(r, dinv, P->nfree, red, P->dscal.p, s));
+      SF_HIP(hipMemcpyAsync(&rz, P->dscal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+      SF_HIP(hipStreamSynchronize(s));
+    } else if (gll) {  // z of the replaced residual (the device scalars say done)
+      rc = pc_gll<float>(P, r, z, P->w_w32.p, nullptr, s);
+      if (rc) return rc;
+      rc = dot_host_f32(P, r, z, &rz, s);
+      if (rc) return rc;
+    }
+    const double c = (h.rr_new > 0.0) ? (rn_new / rn) * rz * h.beta / h.rr_new : 0.0;
+    SF_HIP(launch_dir_coef(p, gll ? z : r, q, (float)c, P->nfree, s, dinv));
+    SF_HIP(launch_fill<float>(xl, P->nfree, 0.0f, s));
     rn = rn_new;
     const double t = threshold(rn);
-    h.rr = rr;
+    h.res2 = rr;
+    h.rr = rz;
     h.stop_rr = t * t;
     h.done = 0;
     rc = write_scalars(P, &h, s);
@@ -853,7 +1097,7 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
 int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
                        double rtol, int maxit, double delta, hipStream_t s,
                        int* iters_out, int* updates_out, int* nonfinite_out,
-                       double* rel_out, bool p32only = false) {
+                       double* rel_out, bool p32only, const float* dinv, bool gll) {
   Reducer red{P->red_part.p, P->red_counter.p};
   double* r = P->w_r.p;
   double* p = P->w_p.p;
@@ -874,7 +1118,14 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
   auto threshold = [&](double rnow) { return (delta * rnow > target) ? delta * rnow : target; };
   const double t0 = threshold(rn);
   SF_HIP(launch_cg_start<double>(b, P->w_t.p, r, p, P->w_q.p, P->nfree, red, P->scal.p,
-                                 t0 * t0, s));
+                                 t0 * t0, s, dinv));
+  double* const z = P->w_z.p;
+  if (gll) {
+    rc = pc_gll<double>(P, r, z, P->w_w.p, P->scal.p, s);
+    if (rc) return rc;
+    SF_HIP((launch_cg_set_rz<double, double>(r, z, P->nfree, red, P->scal.p, s)));
+    SF_HIP(hipMemcpyAsync(p, z, P->nfree * 8, hipMemcpyDeviceToDevice, s));
+  }
   SF_HIP(launch_to_f32(p, P->nfree, p32, s));
   SF_HIP(launch_fill<float>(q32, P->nfree, 0.0f, s));
   CGScalars h{};
@@ -888,25 +1139,47 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
   if (!fused) p32only = false;
   for (;;) {
     while (!h.done && h.iters < maxit) {
-      prog.tick(h.iters, bnorm > 0 ? sqrt(h.rr) / bnorm : sqrt(h.rr));
+      prog.tick(h.iters, bnorm > 0 ? sqrt(h.res2) / bnorm : sqrt(h.res2));
       const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
       for (int k = 0; k < todo; ++k) {
         if (fused) {
           rc = do_apply_f32(P, q32, p32, precision, s, true, P->w_part.p);
           if (rc) return rc;
           SF_HIP(launch_cg_alpha_parts(P->w_part.p, P->npart, red, P->scal.p, s));
-          SF_HIP(launch_cg_resid_mixed(r, q32, P->nfree, red, P->scal.p, s));
+          if (gll) {
+            SF_HIP((launch_cg_resid_shares<double, float>(r, q32, P->nfree, red, P->scal.p, s)));
+            rc = pc_gll<double>(P, r, z, P->w_w.p, P->scal.p, s);
+            if (rc) return rc;
+            SF_HIP((launch_rz_shares<double, double>(r, z, P->nfree, red, P->scal.p, s)));
+            SF_HIP(launch_cg_finish(P->nfree, red, P->scal.p, s));
+            if (p32only)
+              SF_HIP(launch_cg_xdir_p32(x, p32, z, q32, P->nfree, P->scal.p, s));
+            else
+              SF_HIP(launch_cg_xdir_mixed(x, p, p32, z, q32, P->nfree, P->scal.p, s));
+            continue;
+          }
+          SF_HIP(launch_cg_resid_mixed(r, q32, P->nfree, red, P->scal.p, s, dinv));
           if (p32only)
-            SF_HIP(launch_cg_xdir_p32(x, p32, r, q32, P->nfree, P->scal.p, s));
+            SF_HIP(launch_cg_xdir_p32(x, p32, r, q32, P->nfree, P->scal.p, s, dinv));
           else
-            SF_HIP(launch_cg_xdir_mixed(x, p, p32, r, q32, P->nfree, P->scal.p, s));
+            SF_HIP(launch_cg_xdir_mixed(x, p, p32, r, q32, P->nfree, P->scal.p, s, dinv));
           continue;
         }
         rc = do_apply_f32(P, q32, p32, precision, s, true);
         if (rc) return rc;
         SF_HIP(launch_cg_alpha_mixed(p, q32, P->nfree, red, P->scal.p, s));
-        SF_HIP(launch_cg_update_mixed(x, r, p, q32, P->nfree, red, P->scal.p, s));
-        SF_HIP(launch_cg_dir_mixed(p, p32, r, q32, P->nfree, P->scal.p, s));
+        if (gll) {
+          SF_HIP((launch_cg_update_shares<double, float>(x, r, p, q32, P->nfree, red, P->scal.p,
+                                                         s)));
+          rc = pc_gll<double>(P, r, z, P->w_w.p, P->scal.p, s);
+          if (rc) return rc;
+          SF_HIP((launch_rz_shares<double, double>(r, z, P->nfree, red, P->scal.p, s)));
+          SF_HIP(launch_cg_finish(P->nfree, red, P->scal.p, s));
+          SF_HIP(launch_cg_dir_mixed(p, p32, z, q32, P->nfree, P->scal.p, s));
+          continue;
+        }
+        SF_HIP(launch_cg_update_mixed(x, r, p, q32, P->nfree, red, P->scal.p, s, dinv));
+        SF_HIP(launch_cg_dir_mixed(p, p32, r, q32, P->nfree, P->scal.p, s, dinv));
       }
       rc = read_scalars(P, &h, s);
       if (rc) return rc;
@@ -927,13 +1200,27 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
       return SF_OK;
     }
     if (rn_new <= target || rn_new == 0.0 || h.iters >= maxit) return SF_OK;
-    const double c = (h.rr_new > 0.0) ? rn_new * rn_new * h.beta / h.rr_new : 0.0;
+    // p <- z + c p, c = r.z / rz_{k-1} (rz_{k-1} = rr_new / beta)
+    double rz = rn_new * rn_new;
+    if (dinv) {
+      SF_HIP(launch_wdot<double>(r, dinv, P->nfree, red, P->dscal.p, s));
+      SF_HIP(hipMemcpyAsync(&rz, P->dscal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+      SF_HIP(hipStreamSynchronize(s));
+    } else if (gll) {  // z of the replaced residual (the device scalars say done)
+      rc = pc_gll<double>(P, r, z, P->w_w.p, nullptr, s);
+      if (rc) return rc;
+      rc = dot_host(P, r, z, &rz, s);
+      if (rc) return rc;
+    }
+    const double c = (h.rr_new > 0.0) ? rz * h.beta / h.rr_new : 0.0;
+    const double* zr = gll ? z : r;
     if (p32only)
-      SF_HIP(launch_dir_coef_p32(p32, r, q32, c, P->nfree, s));
+      SF_HIP(launch_dir_coef_p32(p32, zr, q32, c, P->nfree, s, dinv));
     else
-      SF_HIP(launch_dir_coef_mixed(p, p32, r, q32, c, P->nfree, s));
+      SF_HIP(launch_dir_coef_mixed(p, p32, zr, q32, c, P->nfree, s, dinv));
     const double t = threshold(rn_new);
-    h.rr = rn_new * rn_new;
+    h.res2 = rn_new * rn_new;
+    h.rr = rz;
     h.stop_rr = t * t;
     h.done = 0;
     h.xpend = 0;
@@ -948,6 +1235,8 @@ double now_s() {
 }
 
 }  // namespace
+
+static int finish_geometry(sf_plan* P);
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -1087,10 +1376,34 @@ int sf_plan_set_tables(sf_plan* P, const double* B, const double* Dm,
   const int64_t tn = (int64_t)P->q * P->n;
   for (int64_t i = 0; i < tn; ++i)
     if (!isfinite(B[i]) || !isfinite(Dm[i])) return fail(SF_ERR_ARG, "non-finite table entry");
+  if (weights) {
+    for (int i = 0; i < P->q; ++i)
+      if (!(weights[i] > 0.0) || !isfinite(weights[i]))
+        return fail(SF_ERR_ARG, "quadrature weight %d is not positive and finite", i);
+    // the geometric factor holds the weights: with the vertices at hand it
+    // is recomputed below, a caller-supplied G cannot be
+    if (P->has_geom && !P->X.p)
+      return fail(SF_ERR_NOT_READY,
+                  "new quadrature weights after a geometry the plan cannot recompute: "
+                  "set the tables before the geometry");
+  }
+  SF_HIP(hipDeviceSynchronize());  // kernels in flight may still read the tables
   P->hB.assign(B, B + tn);
   P->hD.assign(Dm, Dm + tn);
   if (weights) P->hW.assign(weights, weights + P->q);
-  return upload_tables(P);
+  P->pc_valid = false;
+  int rc = upload_tables(P);
+  if (rc || !weights || !P->has_geom) return rc;
+  // G, kappa * G and every reduced-precision copy derive from the weights
+  P->X32.release();
+  P->Pts32.release();
+  P->W32.release();
+  rc = clear_flag(P);
+  if (rc) return rc;
+  SF_HIP(launch_geometry(P->dim, P->X.p, P->dPts.p, P->dW.p, P->q, P->ncell, P->cell_block.p,
+                         P->bcell.p, P->G.p, P->flag.p, nullptr));
+  SF_HIP(hipDeviceSynchronize());
+  return finish_geometry(P);
   SF_CATCH
 }
 
@@ -1117,6 +1430,7 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
   }
   std::vector<int32_t> h((size_t)(P->ncell * nl));
   const int64_t nf = P->nfree;
+  SF_HIP(hipDeviceSynchronize());  // kernels in flight may still read the plan
   for (int64_t c = 0; c < P->ncell; ++c) {
     const int32_t* src = cell_dofs + c * nl;
     int32_t* dst = h.data() + c * nl;
@@ -1134,6 +1448,7 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
       dst[perm[k]] = (int32_t)v;
     }
   }
+  P->pc_valid = false;
   SF_HIP(P->cells.alloc(P->ncell * nl));
   if (!h.empty())
     SF_HIP(hipMemcpy(P->cells.p, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -1155,6 +1470,7 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
 
 static int finish_geometry(sf_plan* P) {
   P->g32_valid = P->gbf_valid = P->gm_valid = false;
+  P->pc_valid = false;
   if (P->has_kappa) {
     SF_HIP(P->Gk.alloc(P->ncell * P->ng * P->nqc));
     if (P->dim == 3)
@@ -1181,6 +1497,7 @@ int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
   if (!vertices && P->ncell > 0) return fail(SF_ERR_ARG, "null vertices");
   if (need_cells_for_geometry(P)) return SF_ERR_NOT_READY;
   DeviceGuard g(P->device);
+  SF_HIP(hipDeviceSynchronize());  // kernels in flight may still read the plan
   const int64_t nv = P->ncell * (P->dim == 3 ? 24 : 8);
   DevBuf<double>& X = P->X;
   P->fused_geom = false;
@@ -1229,6 +1546,7 @@ int sf_plan_set_geometry(sf_plan* P, const double* G) {
   const int64_t n = P->ncell * P->ng * P->nqc;
   for (int64_t i = 0; i < n; ++i)
     if (!isfinite(G[i])) return fail(SF_ERR_ARG, "non-finite G entry at %lld", (long long)i);
+  SF_HIP(hipDeviceSynchronize());  // kernels in flight may still read the plan
   SF_HIP(P->G.alloc(n));
   if (n > 0) {
     if (P->dim == 3) {
@@ -1285,10 +1603,14 @@ int sf_plan_set_kappa(sf_plan* P, const double* kappa) {
   for (int64_t i = 0; i < n; ++i)
     if (!(kappa[i] > 0.0) || !isfinite(kappa[i]))
       return fail(SF_ERR_ARG, "kappa[%lld] = %g is not positive and finite", (long long)i, kappa[i]);
+  // an apply, adjoint or solver still running on another stream reads
+  // kappa, kappa * G and their copies: let it finish before they change
+  SF_HIP(hipDeviceSynchronize());
   SF_HIP(P->kappa.alloc(n));
   if (n > 0) SF_HIP(hipMemcpy(P->kappa.p, kappa, n * 8, hipMemcpyHostToDevice));
   P->has_kappa = true;
   P->k32_valid = false;
+  P->pc_valid = false;
   if (P->has_geom) return finish_geometry(P);
   P->g32_valid = P->gbf_valid = P->gm_valid = false;
   return SF_OK;
@@ -1395,6 +1717,58 @@ int sf_adjoint(sf_plan* P, const double* u, const double* lam,
   SF_CATCH
 }
 
+int sf_diagonal(sf_plan* P, double* diag, void* stream) {
+  SF_TRY
+  int rc = check_ready(P);
+  if (rc) return rc;
+  if (!diag && P->nfree > 0) return fail(SF_ERR_ARG, "null diag");
+  DeviceGuard g(P->device);
+  return compute_diagonal(P, diag, (hipStream_t)stream);
+  SF_CATCH
+}
+
+int sf_plan_set_preconditioner(sf_plan* P, int kind) {
+  SF_TRY
+  if (check_plan(P)) return SF_ERR_ARG;
+  if (kind != SF_PC_NONE && kind != SF_PC_JACOBI && kind != SF_PC_GLL)
+    return fail(SF_ERR_ARG, "unknown preconditioner %d", kind);
+  if (kind == SF_PC_GLL && P->dim != 3)
+    return fail(SF_ERR_UNSUPPORTED, "the Gauss-Lobatto preconditioner is 3D only");
+  if (kind != P->pc_kind) P->pc_valid = false;
+  P->pc_kind = kind;
+  if (kind == SF_PC_NONE) {
+    DeviceGuard g(P->device);
+    P->dinv.release();
+  }
+  return SF_OK;
+  SF_CATCH
+}
+
+int sf_precond_apply(sf_plan* P, double* z, const double* r, void* stream) {
+  SF_TRY
+  int rc = check_ready(P);
+  if (rc) return rc;
+  if ((!z || !r) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
+  DeviceGuard g(P->device);
+  hipStream_t s = (hipStream_t)stream;
+  if (P->nfree == 0) return SF_OK;
+  const float* dinv = nullptr;
+  bool gll = false;
+  rc = ensure_pc(P, s, &dinv, &gll);
+  if (rc) return rc;
+  if (gll) {
+    SF_HIP(P->w_w.alloc(P->nfree));
+    return pc_gll<double>(P, r, z, P->w_w.p, nullptr, s);
+  }
+  if (dinv) {
+    SF_HIP(launch_scale_vec(r, dinv, z, P->nfree, s));
+    return SF_OK;
+  }
+  SF_HIP(hipMemcpyAsync(z, r, P->nfree * 8, hipMemcpyDeviceToDevice, s));
+  return SF_OK;
+  SF_CATCH
+}
+
 int sf_cg(sf_plan* P, const double* b, double* x, double rtol, int maxit,
           sf_solve_stats* stats, void* stream) {
   SF_TRY
@@ -1405,6 +1779,10 @@ int sf_cg(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   DeviceGuard g(P->device);
   hipStream_t s = (hipStream_t)stream;
   rc = ensure_solver_ws(P, false);
+  if (rc) return rc;
+  const float* dinv = nullptr;
+  bool gll = false;
+  rc = ensure_pc(P, s, &dinv, &gll);
   if (rc) return rc;
   SF_HIP(hipStreamSynchronize(s));
   const double t0 = now_s();
@@ -1417,7 +1795,7 @@ int sf_cg(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   // recomputes r = b - A x).
   for (;;) {
     int it = 0;
-    rc = cg_f64(P, b, x, rtol, maxit - iters, s, &it, &nonfinite);
+    rc = cg_f64(P, b, x, rtol, maxit - iters, s, &it, &nonfinite, dinv, gll);
     if (rc) return rc;
     iters += it;
     double rn = 0.0;
@@ -1460,15 +1838,19 @@ int sf_cg_mixed(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   if (rc) return rc;
   rc = ensure_solver_ws(P, true);
   if (rc) return rc;
+  const float* dinv = nullptr;
+  bool gll = false;
+  rc = ensure_pc(P, s, &dinv, &gll);
+  if (rc) return rc;
   SF_HIP(hipStreamSynchronize(s));
   const double t0 = now_s();
   int iters = 0, updates = 0, nonfinite = 0;
   double rel = 0.0;
   rc = fp64_recurrence
            ? cg_reliable_f64rec(P, inner_precision, b, x, rtol, maxit, delta, s, &iters,
-                                &updates, &nonfinite, &rel, fp64_recurrence == 2)
+                                &updates, &nonfinite, &rel, fp64_recurrence == 2, dinv, gll)
            : cg_reliable(P, inner_precision, b, x, rtol, maxit, delta, s, &iters,
-                         &updates, &nonfinite, &rel);
+                         &updates, &nonfinite, &rel, dinv, gll);
   if (rc) return rc;
   SF_HIP(hipStreamSynchronize(s));
   const double t1 = now_s();
@@ -1503,6 +1885,10 @@ int sf_cg_ir(sf_plan* P, const double* b, double* x, double rtol,
   if (rc) return rc;
   rc = ensure_solver_ws(P, true);
   if (rc) return rc;
+  const float* dinv = nullptr;
+  bool gll = false;
+  rc = ensure_pc(P, s, &dinv, &gll);
+  if (rc) return rc;
   SF_HIP(hipStreamSynchronize(s));
   const double t0 = now_s();
   double bb = 0.0;
@@ -1526,7 +1912,7 @@ int sf_cg_ir(sf_plan* P, const double* b, double* x, double rtol,
     // inner solve on the unit-norm residual in reduced precision
     SF_HIP(launch_scale_to_f32(P->w_r.p, 1.0 / rn, P->w_r32.p, P->nfree, s));
     int it = 0, nf = 0;
-    rc = cg_f32_inner(P, inner_precision, inner_rtol, inner_maxit, s, &it, &nf);
+    rc = cg_f32_inner(P, inner_precision, inner_rtol, inner_maxit, s, &it, &nf, dinv, gll);
     if (rc) return rc;
     inner_total += it;
     ++outer;

@@ -9,6 +9,7 @@ namespace sf {
 
 struct bf16s;
 struct BlockArgs;  // host-built block structure of the 3D apply (sf_apply.hpp)
+struct CGScalars;  // device scalars of the CG solvers (below)
 
 // ---- geometry (sf_geom.hip) ----
 // 3D writes G in the block-interleaved layout (cell_block, bcell); 2D in
@@ -95,14 +96,45 @@ hipError_t adjoint3d_fused_f64(int n, int q, const double* u, const double* lam,
                                const double* tP, const double* tW, const double* kappa,
                                hipStream_t s);
 
+// ---- diagonal of the operator (sf_diag.hip) ----
+// diag += the element-matrix diagonals of every cell (the caller zeroes
+// diag): 3D reads G in the block-interleaved layout (cell_block, bcell),
+// 2D in the caller's [cell][entry][qp]; cells are the plan's 0-based
+// tensor-ordered ids (negative = Dirichlet).
+hipError_t launch_diagonal(int dim, int n, int q, const double* G, const double* tB,
+                           const double* tD, const int32_t* cells,
+                           const int32_t* cell_block, const int64_t* bcell,
+                           int64_t ncell, double* diag, hipStream_t s);
+// inv = 1 / d (FP32); *bad |= 1 for a non-positive or non-finite d
+hipError_t launch_inv_diag(const double* d, float* inv, int64_t n, int* bad, hipStream_t s);
+
+// ---- Gauss-Lobatto-basis preconditioner (sf_precond.hip, 3D) ----
+// minv = 1 / (number of cells holding each free dof); minv zeroed by the
+// caller; cells: n_cells * n^3 plan ids
+hipError_t launch_mult_inverse(const int32_t* cells, int64_t ncell_dofs, float* minv,
+                               int64_t nfree, hipStream_t s);
+// z = T (dginv . T^T (minv . r)) over the block structure k (eb cells per
+// block at most, max_nu distinct dofs); w: scratch vector of nfree values;
+// T1 / T1t: the n x n GLL-to-equispaced table and its transpose; sc (may
+// be null): nothing runs once sc->done is set
+template <typename TR, typename TZ>
+hipError_t launch_pc_gll(int n, const TR* r, TZ* z, TZ* w, int64_t nfree, const float* minv,
+                         const float* dginv, const double* T1, const double* T1t,
+                         const BlockArgs& k, int eb, int max_nu, const CGScalars* sc,
+                         hipStream_t s);
+// z = dinv . r (the Jacobi preconditioner as a stand-alone product)
+hipError_t launch_scale_vec(const double* r, const float* dinv, double* z, int64_t n,
+                            hipStream_t s);
+
 // ---- BLAS-1 for the solvers (sf_blas.hip) ----
 struct CGScalars {
-  double rr;       // current r.r
+  double rr;       // current r.z (r.r without a preconditioner)
   double pq;       // p.Ap
   double alpha;
   double beta;
   double rr_new;
-  double stop_rr;  // stop when rr <= stop_rr
+  double stop_rr;  // stop when res2 <= stop_rr
+  double res2;     // current r.r (the stop test and progress reports)
   int done;        // 1 once rr <= stop_rr (later updates are no-ops)
   int iters;       // iterations taken while not done
   int nonfinite;   // a NaN / Inf was met
@@ -131,27 +163,35 @@ hipError_t launch_zero_list(V* y, const int32_t* idx, int64_t n, hipStream_t s);
 template <typename V>
 hipError_t launch_dot(const V* a, const V* b, int64_t n, Reducer red,
                       double* out, hipStream_t s);
+// out = a . (w a), w FP32 (r.z of the Jacobi preconditioner)
+template <typename V>
+hipError_t launch_wdot(const V* a, const float* w, int64_t n, Reducer red,
+                       double* out, hipStream_t s);
 
-// CG pieces on device scalars sc:
-//   cg_start : r = b - Ax, p = r, q = 0, rr = r.r, done = rr <= stop_rr
+// CG pieces on device scalars sc.  dinv (may be null): the Jacobi
+// preconditioner, z = dinv r (FP32 inverse diagonal); null: z = r.
+//   cg_start : r = b - Ax, p = z, q = 0, res2 = r.r, rr = r.z,
+//              done = res2 <= stop_rr
 //   cg_alpha : pq = p.q, alpha = rr / pq
-//   cg_update: x += alpha p, r -= alpha q, rr_new = r.r, beta = rr_new / rr,
-//              rr = rr_new, iters++, done = rr <= stop_rr
-//   cg_dir   : p = r + beta p, q = 0 (ready for the next q += A p)
-// Once done is set every later piece returns at once.
+//   cg_update: x += alpha p, r -= alpha q, res2 = r.r, rr_new = r.z,
+//              beta = rr_new / rr, rr = rr_new, iters++, done = res2 <= stop_rr
+//   cg_dir   : p = z + beta p, q = 0 (ready for the next q += A p)
+// Once done is set every later piece returns at once.  The reductions use
+// 2 * kStreamBlocks slots of red.part.
 template <typename V>
 hipError_t launch_cg_start(const V* b, const V* Ax, V* r, V* p, V* q,
                            int64_t n, Reducer red, CGScalars* sc,
-                           double stop_rr, hipStream_t s);
+                           double stop_rr, hipStream_t s, const float* dinv = nullptr);
 template <typename V>
 hipError_t launch_cg_alpha(const V* p, const V* q, int64_t n, Reducer red,
                            CGScalars* sc, hipStream_t s);
 template <typename V>
 hipError_t launch_cg_update(V* x, V* r, const V* p, const V* q, int64_t n,
-                            Reducer red, CGScalars* sc, hipStream_t s);
+                            Reducer red, CGScalars* sc, hipStream_t s,
+                            const float* dinv = nullptr);
 template <typename V>
 hipError_t launch_cg_dir(V* p, const V* r, V* q, int64_t n, CGScalars* sc,
-                         hipStream_t s);
+                         hipStream_t s, const float* dinv = nullptr);
 
 // IR helpers (FP64 outer, FP32 inner)
 // r = b - Ax (FP64)
@@ -162,44 +202,66 @@ hipError_t launch_scale_to_f32(const double* r, double inv_scale, float* r32,
                                int64_t n, hipStream_t s);
 // Fused CG step (3D; the apply writes per-wave shares of p.Ap):
 //   alpha_parts: alpha = rr / sum(part[0 .. np)), clears xpend
-//   resid      : r -= alpha q, rr_new = r.r, beta, done; sets xpend
-//   xdir       : if xpend: x += alpha p; unless done: p = r + beta p, q = 0
+//   resid      : r -= alpha q, res2 = r.r, rr_new = r.z, beta, done; sets xpend
+//   xdir       : if xpend: x += alpha p; unless done: p = z + beta p, q = 0
 hipError_t launch_cg_alpha_parts(const double* part, int64_t np, Reducer red,
                                  CGScalars* sc, hipStream_t s);
 template <typename V>
 hipError_t launch_cg_resid(V* r, const V* q, int64_t n, Reducer red, CGScalars* sc,
-                           hipStream_t s);
+                           hipStream_t s, const float* dinv = nullptr);
 template <typename V>
 hipError_t launch_cg_xdir(V* x, V* p, const V* r, V* q, int64_t n, CGScalars* sc,
-                          hipStream_t s);
+                          hipStream_t s, const float* dinv = nullptr);
 // ... with FP64 vectors and an FP32 operator (q32 = A32 p32; xdir also
 // refreshes p32 = (float)p)
 hipError_t launch_cg_resid_mixed(double* r, const float* q, int64_t n, Reducer red,
-                                 CGScalars* sc, hipStream_t s);
+                                 CGScalars* sc, hipStream_t s, const float* dinv = nullptr);
 hipError_t launch_cg_xdir_mixed(double* x, double* p, float* p32, const double* r,
-                                float* q32, int64_t n, CGScalars* sc, hipStream_t s);
+                                float* q32, int64_t n, CGScalars* sc, hipStream_t s,
+                                const float* dinv = nullptr);
 // ... with the search direction only in FP32: x += alpha p32,
-// p32 = r + beta p32, q32 = 0; and the host-coefficient form p32 = r + c p32
+// p32 = z + beta p32, q32 = 0; and the host-coefficient form p32 = z + c p32
 hipError_t launch_cg_xdir_p32(double* x, float* p32, const double* r, float* q32,
-                              int64_t n, const CGScalars* sc, hipStream_t s);
+                              int64_t n, const CGScalars* sc, hipStream_t s,
+                              const float* dinv = nullptr);
 hipError_t launch_dir_coef_p32(float* p32, const double* r, float* q32, double c,
-                               int64_t n, hipStream_t s);
+                               int64_t n, hipStream_t s, const float* dinv = nullptr);
 // FP64 CG vectors with an FP32 operator (q = A32 p32):
 //   alpha_mixed : pq = p . q32, alpha = rr / pq
-//   update_mixed: x += alpha p, r -= alpha q32, rr_new = r.r, beta, done
-//   dir_mixed   : p = r + beta p, p32 = (float)p, q32 = 0
+//   update_mixed: x += alpha p, r -= alpha q32, res2, rr_new = r.z, beta, done
+//   dir_mixed   : p = z + beta p, p32 = (float)p, q32 = 0
 //   dir_coef_mixed: the same with the coefficient c given by the host
 hipError_t launch_cg_alpha_mixed(const double* p, const float* q, int64_t n,
                                  Reducer red, CGScalars* sc, hipStream_t s);
 hipError_t launch_cg_update_mixed(double* x, double* r, const double* p, const float* q,
-                                  int64_t n, Reducer red, CGScalars* sc, hipStream_t s);
+                                  int64_t n, Reducer red, CGScalars* sc, hipStream_t s,
+                                  const float* dinv = nullptr);
 hipError_t launch_cg_dir_mixed(double* p, float* p32, const double* r, float* q32,
-                               int64_t n, CGScalars* sc, hipStream_t s);
+                               int64_t n, CGScalars* sc, hipStream_t s,
+                               const float* dinv = nullptr);
 hipError_t launch_dir_coef_mixed(double* p, float* p32, const double* r, float* q32,
-                                 double c, int64_t n, hipStream_t s);
-// p = r + c p, q = 0 (FP32)
+                                 double c, int64_t n, hipStream_t s,
+                                 const float* dinv = nullptr);
+// Pieces for a preconditioner that runs as separate passes (the GLL one):
+// residual / update without the finish, r.z shares into the second half of
+// red.part, the finish (res2, rr_new = r.z, beta, the stop test), and at
+// the start rr = rr_new = r.z.
+template <typename V, typename W>
+hipError_t launch_cg_resid_shares(V* r, const W* q, int64_t n, Reducer red, CGScalars* sc,
+                                  hipStream_t s);
+template <typename V, typename W>
+hipError_t launch_cg_update_shares(V* x, V* r, const V* p, const W* q, int64_t n, Reducer red,
+                                   CGScalars* sc, hipStream_t s);
+template <typename V, typename Z>
+hipError_t launch_rz_shares(const V* r, const Z* z, int64_t n, Reducer red, CGScalars* sc,
+                            hipStream_t s);
+hipError_t launch_cg_finish(int64_t n, Reducer red, CGScalars* sc, hipStream_t s);
+template <typename V, typename Z>
+hipError_t launch_cg_set_rz(const V* r, const Z* z, int64_t n, Reducer red, CGScalars* sc,
+                            hipStream_t s);
+// p = z + c p, q = 0 (FP32)
 hipError_t launch_dir_coef(float* p, const float* r, float* q, float c,
-                           int64_t n, hipStream_t s);
+                           int64_t n, hipStream_t s, const float* dinv = nullptr);
 // x += scale * d32
 hipError_t launch_axpy_f32_to_f64(double* x, const float* d32, double scale,
                                   int64_t n, hipStream_t s);

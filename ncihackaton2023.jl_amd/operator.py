@@ -51,6 +51,7 @@ class LazyMatrix:
                                     self.n_cells, self.n_free, 0))
         self._plan = h
         self._L = L
+        self.kappa_version = 0  # bumped by every set_kappa
         cells = np.ascontiguousarray(space.cell_dofs, dtype=np.int32)
         _lib.check(L.sf_plan_set_cells(h, cells.ctypes.data, 0, None))
         if geometry is None:
@@ -90,6 +91,7 @@ class LazyMatrix:
         if k.size != self.n_cells * self.m.nquad ** self.D:
             raise ValueError("kappa must have n_cells * q^D entries")
         _lib.check(self._L.sf_plan_set_kappa(self.plan, k.ctypes.data))
+        self.kappa_version += 1
 
     def geometry(self):
         """The FP64 geometric factor G[cell][entry][qp] held on the device."""
@@ -97,6 +99,27 @@ class LazyMatrix:
         G = np.empty((self.n_cells, ng, self.m.nquad ** self.D))
         _lib.check(self._L.sf_plan_get_geometry(self.plan, G.ctypes.data))
         return G
+
+    def diagonal(self, out=None):
+        """diag(A) on the free dofs (FP64 torch CUDA tensor), matrix-free."""
+        import torch
+        if out is None:
+            out = torch.empty(self.n_free, dtype=torch.float64, device="cuda")
+        self._check_dev(out, out, "float64")
+        _lib.check(self._L.sf_diagonal(self.plan, ctypes.c_void_p(out.data_ptr()),
+                                       _torch_stream(out)))
+        return out
+
+    def precond(self, z, r, Pl):
+        """z = M^{-1} r for the preconditioner Pl (None, 'jacobi', 'gll'),
+        FP64 torch CUDA tensors: the ldiv!(z, Pl, r) of a CG iteration."""
+        self._check_dev(r, z, "float64")
+        if Pl not in _lib.PRECONDITIONERS:
+            raise ValueError("unknown preconditioner %r" % (Pl,))
+        _lib.check(self._L.sf_plan_set_preconditioner(self.plan, _lib.PRECONDITIONERS[Pl]))
+        _lib.check(self._L.sf_precond_apply(self.plan, ctypes.c_void_p(z.data_ptr()),
+                                            ctypes.c_void_p(r.data_ptr()), _torch_stream(r)))
+        return z
 
     def prepare(self, precision):
         _lib.check(self._L.sf_plan_prepare(self.plan, _lib.PRECISIONS.get(precision, precision)))

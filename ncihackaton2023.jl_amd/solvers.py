@@ -16,17 +16,29 @@ from . import _lib
 from .operator import _torch_stream
 
 
+def _set_pc(A, Pl):
+    """The Pl keyword of IterativeSolvers.cg! as a plan setting: None /
+    'none' (the reference's unpreconditioned call), 'jacobi' (diag(A)) or
+    'gll' (3D: Jacobi in the Gauss-Lobatto nodal basis of the space)."""
+    if Pl not in _lib.PRECONDITIONERS:
+        raise ValueError("Pl must be one of %s" % sorted(k for k in _lib.PRECONDITIONERS if k))
+    _lib.check(A._L.sf_plan_set_preconditioner(A.plan, _lib.PRECONDITIONERS[Pl]))
+
+
 def _vec_ok(A, v):
     if not (type(v).__module__.startswith("torch") and v.is_cuda and str(v.dtype) == "torch.float64"
             and v.is_contiguous() and v.numel() == A.n_free):
         raise TypeError("solver vectors must be contiguous torch CUDA float64 of n_free entries")
 
 
-def cg_(x, A, b, rtol=1e-10, maxiter=None, raise_on_fail=False):
-    """Plain FP64 CG: stop when ||b - A x|| <= rtol ||b||.  Returns the
-    statistics dict (iterations, rel_residual, converged, seconds)."""
+def cg_(x, A, b, rtol=1e-10, maxiter=None, raise_on_fail=False, Pl=None):
+    """FP64 CG: stop when ||b - A x|| <= rtol ||b||.  Pl selects the
+    preconditioner (None: unpreconditioned, as the reference's driver).
+    Returns the statistics dict (iterations, rel_residual, converged,
+    seconds)."""
     _vec_ok(A, x)
     _vec_ok(A, b)
+    _set_pc(A, Pl)
     st = _lib.SolveStats()
     maxiter = A.n_free if maxiter is None else int(maxiter)
     rc = A._L.sf_cg(A.plan, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
@@ -37,10 +49,12 @@ def cg_(x, A, b, rtol=1e-10, maxiter=None, raise_on_fail=False):
 
 
 def cg_ir_(x, A, b, rtol=1e-10, inner="f32", inner_rtol=1e-3, inner_maxiter=None,
-           max_outer=50, raise_on_fail=False):
-    """Mixed-precision iterative refinement (FP64 outer, reduced inner)."""
+           max_outer=50, raise_on_fail=False, Pl=None):
+    """Mixed-precision iterative refinement (FP64 outer, reduced inner,
+    preconditioned inner CG when Pl is set)."""
     _vec_ok(A, x)
     _vec_ok(A, b)
+    _set_pc(A, Pl)
     st = _lib.SolveStats()
     prec = _lib.PRECISIONS[inner] if isinstance(inner, str) else int(inner)
     inner_maxiter = A.n_free if inner_maxiter is None else int(inner_maxiter)
@@ -53,14 +67,16 @@ def cg_ir_(x, A, b, rtol=1e-10, inner="f32", inner_rtol=1e-3, inner_maxiter=None
 
 
 def cg_mixed_(x, A, b, rtol=1e-10, inner="f32", delta=0.1, maxiter=None,
-              recurrence="f64p32", raise_on_fail=False):
+              recurrence="f64p32", raise_on_fail=False, Pl=None):
     """Mixed-precision CG with reliable updates: one CG recurrence on the
     reduced-precision operator (recurrence 'f32': FP32 vectors; 'f64': FP64
     vectors, only the operator reduced; 'f64p32': FP64 solution and residual,
     the search direction only in FP32), residual replaced by the FP64
-    b - A x whenever it has fallen by `delta` (search direction kept)."""
+    b - A x whenever it has fallen by `delta` (search direction kept).
+    Pl: the preconditioner, as in cg_."""
     _vec_ok(A, x)
     _vec_ok(A, b)
+    _set_pc(A, Pl)
     st = _lib.SolveStats()
     prec = _lib.PRECISIONS[inner] if isinstance(inner, str) else int(inner)
     maxiter = 10 * A.n_free if maxiter is None else int(maxiter)

@@ -113,6 +113,81 @@ def element_matrices(dim, B, Dm, G):
     return K
 
 
+def diagonal(dim, B, Dm, G, cells, nfree):
+    """diag(A) on the free dofs: the element-matrix diagonals
+    K_e[a, a] = sum_t Grad_t[:, a]^T G_t Grad_t[:, a] (dense reference
+    gradients, no factorisation), summed over the cells that hold dof a."""
+    Gr = np.stack(grad_matrices(dim, B, Dm))  # (D, nq, nl)
+    de = np.einsum("rta,ctrk,kta->ca", Gr, np.asarray(G), Gr)
+    cells = np.asarray(cells)
+    d = np.zeros(nfree)
+    m = cells >= 0
+    np.add.at(d, cells[m], de[m])
+    return d
+
+
+def gll_nodes(p):
+    """Gauss-Lobatto-Legendre points on [0, 1] in the reference's node order
+    [0, 1, interior ascending]: the interior ones are the roots of P_p'
+    (numpy's Legendre roots)."""
+    if p == 1:
+        return np.array([0.0, 1.0])
+    c = np.zeros(p + 1)
+    c[-1] = 1.0
+    inner = np.sort(npleg.legroots(npleg.legder(c)))
+    return np.array([0.0, 1.0] + list(0.5 * (inner + 1.0)))
+
+
+def lagrange_matrix(nodes, x):
+    """L[i, j] = l_j(x_i), the Lagrange polynomials of `nodes` at x."""
+    nodes = np.asarray(nodes)
+    x = np.asarray(x)
+    L = np.ones((x.size, nodes.size))
+    for j in range(nodes.size):
+        for m in range(nodes.size):
+            if m != j:
+                L[:, j] *= (x - nodes[m]) / (nodes[j] - nodes[m])
+    return L
+
+
+def gll_to_equispaced(p):
+    """T1[a, b] = l_b(t_a): Gauss-Lobatto Lagrange polynomials at the
+    equispaced nodes (both in the reference's order)."""
+    return lagrange_matrix(gll_nodes(p), nodes_1d(p))
+
+
+def global_interp(dim, cells, T1, nfree):
+    """The global (free x free) matrix of the per-cell tensor product of T1:
+    the entries every cell holding a pair of dofs agrees on."""
+    Te = T1
+    for _ in range(dim - 1):
+        Te = np.kron(T1, Te)
+    cells = np.asarray(cells)
+    rows, cols, vals = [], [], []
+    for c in cells:
+        a_idx = np.nonzero(c >= 0)[0]
+        blk = Te[np.ix_(a_idx, a_idx)]
+        nz = np.abs(blk) > 0
+        ia, ib = np.nonzero(nz)
+        rows.append(c[a_idx][ia])
+        cols.append(c[a_idx][ib])
+        vals.append(blk[ia, ib])
+    rows, cols, vals = (np.concatenate(t) for t in (rows, cols, vals))
+    T = sp.coo_matrix((vals, (rows, cols)), shape=(nfree, nfree)).tocsr()
+    cnt = sp.coo_matrix((np.ones_like(vals), (rows, cols)), shape=(nfree, nfree)).tocsr()
+    T.data /= cnt.data  # duplicates were summed; every cell gives the same value
+    return T
+
+
+def gll_preconditioner(dim, p, B, Dm, G, cells, nfree):
+    """(T, dg): z = T (T^T r / dg) with dg = diag(T^T A T), A assembled from
+    the brute-force element matrices."""
+    T = global_interp(dim, cells, gll_to_equispaced(p), nfree)
+    A = assemble_csr(cells, element_matrices(dim, B, Dm, G), nfree)
+    dg = (T.T @ A @ T).diagonal()
+    return T, dg
+
+
 def apply_dense(cells, K, x):
     """y = sum_e P_e^T K_e P_e x with Dirichlet ids (< 0) dropped."""
     cells = np.asarray(cells)

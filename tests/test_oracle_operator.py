@@ -268,3 +268,107 @@ def test_step_by_step_demo_restatement(pkg, oracle_mod, brute, p, q, m, perturb)
         swapped = [tab[1], tab[0], tab[2], tab[3]]
         y_bad = np.array(stepbystep.apply_2d(space.cell_dofs, X, x, swapped))
         assert rel_l2(y_bad, y_c) > 1e-3
+
+
+def test_flop_model_unstructured_hand_values(pkg):
+    """The metric-work branch of count_manual_flops_poisson_matrix_free
+    (src/Utils.jl:47-56, structured_mesh_opt = false) that the FP64 apply's
+    roofline uses, against values worked out by hand from the reference's
+    formula: in 3D the weights-and-Jacobian step is (D-1) D^D + D^2 = 63
+    FMA-only operations per quadrature point, 2*63 + 27 + 6 = 159 in all;
+    in 2D 8 and 2*8 + 4 + 4 = 24.  The contractions of config 2 (Q4,
+    q = 6) are 3 (6*125 + 36*25 + 216*5) = 8190 FMAs each way per cell."""
+    f = pkg.count_manual_flops_poisson_matrix_free
+    # config 2: 64^3 cells, q = 6, n = 5; 16380 + 159*216 + 16380 = 67104 per cell
+    assert f(64 ** 3, 3, (6,) * 3, (5,) * 3, count_only_fma_ops=False,
+             structured_mesh_opt=False) == 17_590_910_976
+    assert f(1, 3, (6,) * 3, (5,) * 3, count_only_fma_ops=True,
+             structured_mesh_opt=False) == 8190 + 63 * 216 + 8190
+    # structured: only the D multiplications by the weight per point count
+    assert f(1, 3, (6,) * 3, (5,) * 3, count_only_fma_ops=False,
+             structured_mesh_opt=True) == 16380 + 3 * 216 + 16380
+    # 2D Q4, q = 6: contractions 2 (6*25 + 36*5) = 660 FMAs each way
+    assert f(1, 2, (6, 6), (5, 5), count_only_fma_ops=False,
+             structured_mesh_opt=False) == 2 * 660 + 24 * 36 + 2 * 660
+    assert f(1, 2, (6, 6), (5, 5), count_only_fma_ops=True,
+             structured_mesh_opt=False) == 660 + 8 * 36 + 660
+
+
+@pytest.mark.parametrize("D,m,p,q", [(2, 4, 3, 5), (3, 2, 4, 6), (3, 2, 2, 3)])
+def test_brute_diagonal_pins(pkg, oracle_mod, brute, D, m, p, q):
+    """The brute-force diagonal (element-matrix diagonals summed over cells)
+    equals the diagonal of the assembled CSR matrix on a perturbed mesh with
+    a coefficient, e_i^T A e_i by the C oracle's product for a few dofs, and
+    the Kronecker closed form's diagonal on the uniform grid."""
+    model, space, mp = setup(pkg, D, m, p, q)
+    X = model.cell_vertices()
+    B, Dm, pts, w = brute.tables(p, q)
+    G, _ = brute.geometry_numpy(D, X, pts, w)
+    kappa = np.exp(pkg.normal(3, space.n_cells * q ** D)).reshape(space.n_cells, q ** D)
+    Gk = G * kappa[:, :, None, None]
+    d = brute.diagonal(D, B, Dm, Gk, space.cell_dofs, space.n_free)
+    A = brute.assemble_csr(space.cell_dofs, brute.element_matrices(D, B, Dm, Gk), space.n_free)
+    assert rel_l2(d, A.diagonal()) < 1e-14
+    for i in (0, space.n_free // 3, space.n_free - 1):
+        e = np.zeros(space.n_free)
+        e[i] = 1.0
+        Ae = oracle_mod.apply(D, p + 1, q, space.cell_dofs, X, e, mp.tables, kappa=kappa.reshape(-1))
+        assert abs(Ae[i] - d[i]) < 1e-13 * abs(d[i])
+    model0, space0, _ = setup(pkg, D, m, p, q, perturb=0.0)
+    G0, _ = brute.geometry_numpy(D, model0.cell_vertices(), pts, w)
+    d0 = brute.diagonal(D, B, Dm, G0, space0.cell_dofs, space0.n_free)
+    assert rel_l2(d0, brute.kron_operator(D, p, q, m).diagonal()) < 1e-13
+    # a planted mistake: the derivative table in place of the value table
+    # in every direction changes the diagonal
+    d_bad = brute.diagonal(D, Dm, Dm, Gk, space.cell_dofs, space.n_free)
+    assert rel_l2(d_bad, d) > 1e-2
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 5, 7, 9])
+def test_brute_gll_interpolation_pins(brute, p):
+    """The Gauss-Lobatto points and the GLL-to-equispaced table: endpoints
+    and symmetry of the points, the Lobatto rule on them integrates x^k
+    exactly for k <= 2p - 1 (which fixes the points), T1 reproduces every
+    polynomial of degree <= p, its vertex rows are unit rows, and for p <= 2
+    the two node sets coincide (T1 = I)."""
+    s = brute.gll_nodes(p)
+    assert s[0] == 0.0 and s[1] == 1.0
+    inner = np.sort(s[2:])
+    assert np.allclose(inner, 1.0 - inner[::-1], atol=1e-14)
+    # Lobatto weights from the Lagrange basis: w_j = int_0^1 l_j
+    xg, wg = brute.gauss01(p + 2)
+    wl = brute.lagrange_matrix(s, xg).T @ wg
+    for k in range(2 * p):
+        assert abs(wl @ s ** k - 1.0 / (k + 1)) < 1e-13, k
+    T1 = brute.gll_to_equispaced(p)
+    t = brute.nodes_1d(p)
+    for k in range(p + 1):
+        assert np.abs(T1 @ s ** k - t ** k).max() < 1e-12
+    assert np.array_equal(T1[:2, :2], np.eye(2)) and not T1[:2, 2:].any()
+    if p <= 2:
+        assert np.abs(T1 - np.eye(p + 1)).max() < 1e-15
+
+
+@pytest.mark.parametrize("m,p,q", [(2, 4, 6), (2, 3, 5)])
+def test_brute_gll_preconditioner_pins(pkg, brute, m, p, q):
+    """The global interpolation T of the brute-force GLL preconditioner is
+    the per-cell one on every cell (conforming), the diagonal of T^T A T is
+    the element-matrix diagonal with the tables B T1, D T1, and
+    M^{-1} = T diag^{-1} T^T is symmetric positive definite."""
+    model, space, mp = setup(pkg, 3, m, p, q)
+    B, Dm, pts, w = brute.tables(p, q)
+    G, _ = brute.geometry_numpy(3, model.cell_vertices(), pts, w)
+    T, dg = brute.gll_preconditioner(3, p, B, Dm, G, space.cell_dofs, space.n_free)
+    T1 = brute.gll_to_equispaced(p)
+    Te = np.kron(T1, np.kron(T1, T1))
+    Td = T.toarray()
+    for c in space.cell_dofs:
+        a = np.nonzero(c >= 0)[0]
+        assert np.abs(Td[np.ix_(c[a], c[a])] - Te[np.ix_(a, a)]).max() < 1e-15
+    d2 = brute.diagonal(3, B @ T1, Dm @ T1, G, space.cell_dofs, space.n_free)
+    assert rel_l2(d2, dg) < 1e-13
+    u = pkg.uniform(1, space.n_free, -1, 1)
+    v = pkg.uniform(2, space.n_free, -1, 1)
+    Mi = lambda r: T @ ((T.T @ r) / dg)
+    assert abs(u @ Mi(v) - v @ Mi(u)) < 1e-13 * abs(u @ Mi(v))
+    assert u @ Mi(u) > 0 and v @ Mi(v) > 0
