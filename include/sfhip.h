@@ -84,6 +84,10 @@ typedef struct sf_plan_info {
   int64_t device_bytes;    /* device memory the plan holds                   */
   int n_blocks;            /* 3D: cell blocks (<= cells_per_block cells each) */
   int launches_per_apply;  /* kernel launches of one apply (besides the fill) */
+  int cells_reordered;     /* 3D: 1 when the plan keeps its cells in an order
+                              built from the connectivity (the caller's order
+                              gave more blocks); per-cell arrays still use
+                              the caller's order at the boundary            */
 } sf_plan_info;
 
 typedef struct sf_solve_stats {
@@ -144,6 +148,11 @@ int sf_plan_set_tables(sf_plan* plan, const double* B, const double* Dm,
                        const double* weights);
 
 /* Cell-to-dof map, host array of n_cells * n^D int32.
+ * 3D: the apply works on blocks of consecutive cells that share dofs.  When
+ * the caller's cell order yields more blocks than an order built from the
+ * face connectivity (rows of face neighbours), the plan keeps its cells in
+ * the latter; every per-cell array at the boundary (vertices, G, kappa,
+ * dJ/dkappa) stays in the caller's order.
  * index_base = 1: Gridap convention (free ids 1..n_free, ids <= 0 are
  *                 Dirichlet), i.e. get_cell_dof_ids(U).data as used by
  *                 src/LazyMatrices.jl:47 and test/Demo_StepByStep.jl:57-69;

@@ -60,7 +60,7 @@ class PlanInfo(ctypes.Structure):
                 ("has_geometry", ctypes.c_int), ("has_kappa", ctypes.c_int),
                 ("threads_per_block", ctypes.c_int), ("cells_per_block", ctypes.c_int),
                 ("device_bytes", ctypes.c_int64), ("n_blocks", ctypes.c_int),
-                ("launches_per_apply", ctypes.c_int)]
+                ("launches_per_apply", ctypes.c_int), ("cells_reordered", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):

@@ -39,7 +39,8 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
                 const double* __restrict__ tB, const double* __restrict__ tD,
                 int64_t ncell, int nblk,
                 const int32_t* __restrict__ cell_block,
-                const int64_t* __restrict__ bcell) {
+                const int64_t* __restrict__ bcell,
+                const int32_t* __restrict__ corig) {
   using S = Adj3Shape<N, Q>;
   __shared__ double smem[S::EB * S::SE];
   const int blk = xcd_remap(blockIdx.x, nblk);
@@ -188,7 +189,8 @@ __global__ void __launch_bounds__((Adj3Shape<N, Q>::NT))
         const double hx = g0 * gux + g1 * guy + g2 * guz;
         const double hy = g1 * gux + g3 * guy + g4 * guz;
         const double hz = g2 * gux + g4 * guy + g5 * guz;
-        dJdk[ge * S::QQQ + l * S::QQ + r] = glx * hx + gly * hy + glz * hz;
+        dJdk[(corig ? (int64_t)corig[ge] : ge) * S::QQQ + l * S::QQ + r] =
+            glx * hx + gly * hy + glz * hz;
       }
       tx[l] = kq * (g0 * glx + g1 * gly + g2 * glz);
       ty[l] = kq * (g1 * glx + g3 * gly + g4 * glz);
@@ -474,7 +476,8 @@ __attribute__((amdgpu_waves_per_eu(2)))
                     const int64_t* __restrict__ bcell, int nblk,
                     const double* __restrict__ tB, const double* __restrict__ tD,
                     const double* __restrict__ Xv, const double* __restrict__ tP,
-                    const double* __restrict__ tW, const double* __restrict__ kap) {
+                    const double* __restrict__ tW, const double* __restrict__ kap,
+                    const int32_t* __restrict__ corig) {
   using S = AdjB3Shape<N, Q>;
   __shared__ double smem[S::RC + S::EB * 24];
   double* const regV = smem + S::RC;          // the block's cell vertices
@@ -679,7 +682,8 @@ __attribute__((amdgpu_waves_per_eu(2)))
       dF[a] = f1 - f0;
       H[a] = (1 - et) * h0 + et * h1;
     }
-    double* kout = dJdk + (base + (act3 ? e3 : 0)) * S::QQQ + r3;
+    const int64_t crow = base + (act3 ? e3 : 0);  // the plan's cell, the caller's row
+    double* kout = dJdk + (corig ? (int64_t)corig[crow] : crow) * S::QQQ + r3;
     TableRows<double, S::TK> t_(tB, tD);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
@@ -831,13 +835,14 @@ template <int N, int Q>
 hipError_t launch_adj3_blk(const double* u, const double* lam, double* dJdk,
                            double* ybar, const BlockArgs& k, const double* tB,
                            const double* tD, const double* Xv, const double* tP,
-                           const double* tW, const double* kap, hipStream_t s) {
+                           const double* tW, const double* kap, const int32_t* corig,
+                           hipStream_t s) {
   using S = AdjB3Shape<N, Q>;
   if (k.nblk <= 0) return hipSuccess;
 #define SF_ADJ_LAUNCH(KAP_, WY_)                                                     \
   hipLaunchKernelGGL((k_adjoint3d_blk<N, Q, KAP_, WY_>), dim3(k.nblk), dim3(S::NT), 0, \
                      s, u, lam, dJdk, ybar, k.pidx, k.bdofs, k.boff, k.bexcl,        \
-                     k.bcell, k.nblk, tB, tD, Xv, tP, tW, kap)
+                     k.bcell, k.nblk, tB, tD, Xv, tP, tW, kap, corig)
   if (kap && ybar)
     SF_ADJ_LAUNCH(true, true);
   else if (ybar)
@@ -852,12 +857,13 @@ hipError_t adjoint3d_fused_f64(int n, int q, const double* u, const double* lam,
                                double* dJdk, double* ybar, const BlockArgs* k,
                                const double* tB, const double* tD, const double* Xv,
                                const double* tP, const double* tW, const double* kappa,
-                               hipStream_t s) {
+                               const int32_t* corig, hipStream_t s) {
   if (!k || !dJdk || !Xv) return hipErrorInvalidValue;
   switch (n * 16 + q) {
 #define SF_CASE(N_, Q_)                                                            \
   case N_ * 16 + Q_:                                                               \
-    return launch_adj3_blk<N_, Q_>(u, lam, dJdk, ybar, *k, tB, tD, Xv, tP, tW, kappa, s);
+    return launch_adj3_blk<N_, Q_>(u, lam, dJdk, ybar, *k, tB, tD, Xv, tP, tW, kappa, corig, \
+                                   s);
     SF_FOR_EACH_NQ(SF_CASE)
 #undef SF_CASE
     default:
@@ -871,14 +877,14 @@ hipError_t launch_adj3(const double* u, const double* lam, double* dJdk,
                        const double* kappa, const double* tB,
                        const double* tD, int64_t ncell,
                        const int32_t* cell_block, const int64_t* bcell,
-                       hipStream_t s) {
+                       const int32_t* corig, hipStream_t s) {
   using S = Adj3Shape<N, Q>;
   const int64_t nb = (ncell + S::EB - 1) / S::EB;
   if (nb <= 0) return hipSuccess;
   if (nb > 0x7fffffff) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_adjoint3d<N, Q>), dim3((int)nb), dim3(S::NT), 0, s, u,
                      lam, dJdk, ybar, cells, G, kappa, tB, tD, ncell, (int)nb,
-                     cell_block, bcell);
+                     cell_block, bcell, corig);
   return hipGetLastError();
 }
 
@@ -902,13 +908,13 @@ hipError_t adjoint_f64(int dim, int n, int q, const double* u,
                        const double* kappa, const double* tB,
                        const double* tD, int64_t ncell,
                        const int32_t* cell_block, const int64_t* bcell,
-                       hipStream_t s) {
+                       const int32_t* corig, hipStream_t s) {
   if (dim == 3) {
     switch (n * 16 + q) {
 #define SF_CASE(N_, Q_)                                                   \
   case N_ * 16 + Q_:                                                      \
     return launch_adj3<N_, Q_>(u, lam, dJdk, ybar, cells, G, kappa, tB, tD, \
-                               ncell, cell_block, bcell, s);
+                               ncell, cell_block, bcell, corig, s);
       SF_FOR_EACH_NQ(SF_CASE)
 #undef SF_CASE
       default:

@@ -8,9 +8,14 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <climits>
+#include <deque>
+#include <map>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "sf_apply.hpp"
@@ -287,6 +292,10 @@ struct sf_plan {
   DevBuf<double> w_z, w_w;
   DevBuf<float> w_z32, w_w32;
   int max_nu = 0;  // most distinct dofs of one block
+  // 3D cell order: the plan's cell i is the caller's cell perm[i] (empty:
+  // the caller's order); corig is its device copy (for outputs per cell)
+  std::vector<int32_t> perm;
+  DevBuf<int32_t> corig;
   hipStream_t own_stream = nullptr;
 
   const double* active_G() const { return has_kappa ? Gk.p : G.p; }
@@ -306,7 +315,7 @@ struct sf_plan {
     w_r32.release(); w_s32.release(); w_d32.release(); w_p32.release(); w_q32.release();
     red_part.release(); w_part.release(); red_counter.release(); scal.release();
     dscal.release(); dinv.release();
-    dT1.release(); dT1t.release(); minv.release();
+    dT1.release(); dT1t.release(); minv.release(); corig.release();
     w_z.release(); w_w.release(); w_z32.release(); w_w32.release();
     if (own_stream) (void)hipStreamDestroy(own_stream);
     own_stream = nullptr;
@@ -323,7 +332,8 @@ struct sf_plan {
            w_t.bytes() + w_r32.bytes() + w_s32.bytes() + w_d32.bytes() + w_p32.bytes() +
            w_q32.bytes() + red_part.bytes() + w_part.bytes() + red_counter.bytes() +
            scal.bytes() + dscal.bytes() + dinv.bytes() + dT1.bytes() + dT1t.bytes() +
-           minv.bytes() + w_z.bytes() + w_w.bytes() + w_z32.bytes() + w_w32.bytes();
+           minv.bytes() + w_z.bytes() + w_w.bytes() + w_z32.bytes() + w_w32.bytes() +
+           corig.bytes();
   }
 };
 
@@ -526,43 +536,14 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
 //    atomics);
 //  * per cell dof, its index in the block's list (uint16, 0xffff =
 //    Dirichlet); per cell, its block.
+std::vector<int64_t> block_starts(const std::vector<int32_t>& h, int64_t E, int64_t nl, int eb);
+
 int build_blocks(sf_plan* P, const std::vector<int32_t>& h) {
   int nt = 0, eb = 0;
   apply_shape_f64(3, P->n, P->q, &nt, &eb);
   if (eb <= 0) return fail(SF_ERR_UNSUPPORTED, "no kernel for n=%d q=%d", P->n, P->q);
   const int64_t E = P->ncell, nl = P->nloc, nf = P->nfree;
-  // runs of consecutive cells that share a free dof with their predecessor
-  std::vector<int64_t> bcell;
-  {
-    std::vector<int32_t> prev, cur;
-    int64_t run0 = 0;
-    auto close_run = [&](int64_t r0, int64_t r1) {
-      const int64_t len = r1 - r0;
-      const int64_t nbk = (len + eb - 1) / eb;
-      for (int64_t k = 0; k < nbk; ++k) bcell.push_back(r0 + (k * len) / nbk);
-    };
-    for (int64_t c = 0; c < E; ++c) {
-      cur.clear();
-      for (int64_t i = c * nl; i < (c + 1) * nl; ++i)
-        if (h[i] >= 0) cur.push_back(h[i]);
-      std::sort(cur.begin(), cur.end());
-      if (c > 0) {
-        bool shared = false;
-        size_t a = 0, bb = 0;
-        while (a < prev.size() && bb < cur.size()) {
-          if (prev[a] == cur[bb]) { shared = true; break; }
-          if (prev[a] < cur[bb]) ++a; else ++bb;
-        }
-        if (!shared) {
-          close_run(run0, c);
-          run0 = c;
-        }
-      }
-      prev.swap(cur);
-    }
-    if (E > 0) close_run(run0, E);
-    bcell.push_back(E);
-  }
+  const std::vector<int64_t> bcell = block_starts(h, E, nl, eb);
   const int64_t nb = (int64_t)bcell.size() - 1;
   if (nb > 0x7fffffffLL) return fail(SF_ERR_ARG, "too many cell blocks");
   // distinct sorted dofs per block
@@ -635,6 +616,181 @@ int build_blocks(sf_plan* P, const std::vector<int32_t>& h) {
   P->npart = nb * (nt / 64);
   P->kargs = BlockArgs{P->pidx.p, P->bdofs.p, P->boff.p, P->bexcl.p, P->bcell.p, (int)nb};
   return SF_OK;
+}
+
+// Rows of `width` values per cell in the caller's cell order, in the
+// plan's order (the caller's array itself when the orders agree).
+template <typename T>
+const T* to_plan_order(const sf_plan* P, const T* src, int64_t width, std::vector<T>& tmp) {
+  if (P->perm.empty() || !src) return src;
+  tmp.resize((size_t)(P->ncell * width));
+  for (int64_t i = 0; i < P->ncell; ++i) {
+    const T* row = src + (int64_t)P->perm[i] * width;
+    std::copy(row, row + width, tmp.data() + i * width);
+  }
+  return tmp.data();
+}
+
+// Starts of the runs of consecutive cells (tensor-ordered 0-based dofs h,
+// cell c at rows order[c]) that share a free dof with their predecessor,
+// each run split into near-equal blocks of at most eb cells; the last entry
+// is E.  This is the apply's block structure for that cell order.
+std::vector<int64_t> block_starts(const std::vector<int32_t>& h, int64_t E, int64_t nl, int eb) {
+  std::vector<int64_t> bcell;
+  std::vector<int32_t> prev, cur;
+  int64_t run0 = 0;
+  auto close_run = [&](int64_t r0, int64_t r1) {
+    const int64_t len = r1 - r0;
+    const int64_t nbk = (len + eb - 1) / eb;
+    for (int64_t k = 0; k < nbk; ++k) bcell.push_back(r0 + (k * len) / nbk);
+  };
+  for (int64_t c = 0; c < E; ++c) {
+    cur.clear();
+    for (int64_t i = c * nl; i < (c + 1) * nl; ++i)
+      if (h[i] >= 0) cur.push_back(h[i]);
+    std::sort(cur.begin(), cur.end());
+    if (c > 0) {
+      bool shared = false;
+      size_t a = 0, bb = 0;
+      while (a < prev.size() && bb < cur.size()) {
+        if (prev[a] == cur[bb]) { shared = true; break; }
+        if (prev[a] < cur[bb]) ++a; else ++bb;
+      }
+      if (!shared) {
+        close_run(run0, c);
+        run0 = c;
+      }
+    }
+    prev.swap(cur);
+  }
+  if (E > 0) close_run(run0, E);
+  bcell.push_back(E);
+  return bcell;
+}
+
+// A cell order built from the connectivity alone (3D hexahedra, dofs
+// tensor-ordered, 1D node order [0, 1, interior]): face f = 2 d + s of a
+// cell is its layer of local index s (t = 0 or 1) along direction d; two
+// cells are neighbours across a face when they share its dofs.  Rows are
+// chains of face neighbours through opposite faces (a hex mesh's chords),
+// each from its low end; the next row is the one across the +y face of
+// the row's first cell while there is one, then across the +z faces.  On
+// a structured mesh numbered in any order this is the lexicographic
+// (z, y, x) order; on an unstructured hex mesh it is a breadth-first walk
+// over rows.  Returns order[new position] = caller's cell index.
+std::vector<int32_t> connectivity_order(const std::vector<int32_t>& h, int64_t E, int n) {
+  const int64_t nl = (int64_t)n * n * n;
+  // face keys: the smallest free dof among the face's interior nodes
+  // (local indices >= 2 in both in-face directions, p >= 2) -- held by the
+  // face's two cells only; at p = 1 the face's four vertices (at least
+  // three free: three vertices fix a quadrilateral face)
+  std::vector<int64_t> nb(E * 6, -1);
+  std::vector<int8_t> nbf(E * 6, -1);
+  auto local = [&](int d, int s, int i, int j) -> int64_t {
+    int idx[3];
+    idx[d] = s;
+    idx[d == 0 ? 1 : 0] = i;
+    idx[d == 2 ? 1 : 2] = j;
+    return ((int64_t)idx[2] * n + idx[1]) * n + idx[0];
+  };
+  if (n >= 3) {
+    std::unordered_map<int32_t, int64_t> first;  // key -> c * 6 + f
+    first.reserve((size_t)(E * 3));
+    for (int64_t c = 0; c < E; ++c)
+      for (int f = 0; f < 6; ++f) {
+        const int d = f / 2, sd = f % 2;
+        int32_t key = INT32_MAX;
+        for (int i = 2; i < n; ++i)
+          for (int j = 2; j < n; ++j) {
+            const int32_t id = h[c * nl + local(d, sd, i, j)];
+            if (id >= 0 && id < key) key = id;
+          }
+        if (key == INT32_MAX) continue;
+        auto it = first.find(key);
+        if (it == first.end()) {
+          first.emplace(key, c * 6 + f);
+        } else {
+          const int64_t o = it->second;
+          nb[c * 6 + f] = o / 6;
+          nbf[c * 6 + f] = (int8_t)(o % 6);
+          nb[o] = c;
+          nbf[o] = (int8_t)f;
+        }
+      }
+  } else {
+    std::map<std::array<int32_t, 4>, int64_t> first;
+    for (int64_t c = 0; c < E; ++c)
+      for (int f = 0; f < 6; ++f) {
+        const int d = f / 2, sd = f % 2;
+        std::array<int32_t, 4> key;
+        int nfree = 0;
+        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < 2; ++j) {
+            key[2 * i + j] = h[c * nl + local(d, sd, i, j)];
+            nfree += key[2 * i + j] >= 0;
+          }
+        if (nfree < 3) continue;
+        std::sort(key.begin(), key.end());
+        auto it = first.find(key);
+        if (it == first.end()) {
+          first.emplace(key, c * 6 + f);
+        } else {
+          const int64_t o = it->second;
+          nb[c * 6 + f] = o / 6;
+          nbf[c * 6 + f] = (int8_t)(o % 6);
+          nb[o] = c;
+          nbf[o] = (int8_t)f;
+        }
+      }
+  }
+  std::vector<char> seen(E, 0);
+  std::vector<int32_t> order;
+  order.reserve(E);
+  std::deque<int64_t> todo;
+  std::vector<int64_t> left, row;
+  // walk from c across face f, then through the face opposite each entry
+  auto walk = [&](int64_t c, int f, std::vector<int64_t>& out) {
+    out.clear();
+    for (;;) {
+      const int64_t nx = nb[c * 6 + f];
+      if (nx < 0 || seen[nx]) return;
+      const int fo = nbf[c * 6 + f];
+      seen[nx] = 1;
+      out.push_back(nx);
+      c = nx;
+      f = fo ^ 1;
+    }
+  };
+  // seeds: cells with no -x, -y, -z neighbour first (mesh corners)
+  std::vector<int64_t> seeds;
+  for (int64_t c = 0; c < E; ++c)
+    if (nb[c * 6 + 0] < 0 && nb[c * 6 + 2] < 0 && nb[c * 6 + 4] < 0) seeds.push_back(c);
+  for (int64_t c = 0; c < E; ++c) seeds.push_back(c);
+  for (int64_t seed : seeds) {
+    if (seen[seed]) continue;
+    todo.push_back(seed);
+    while (!todo.empty()) {
+      const int64_t c = todo.front();
+      todo.pop_front();
+      if (seen[c]) continue;
+      seen[c] = 1;
+      walk(c, 0, left);
+      row.assign(left.rbegin(), left.rend());
+      row.push_back(c);
+      walk(c, 1, left);
+      row.insert(row.end(), left.begin(), left.end());
+      for (int64_t rc : row) order.push_back((int32_t)rc);
+      for (int64_t rc : row)
+        for (int f = 2; f < 6; ++f) {
+          const int64_t nx = nb[rc * 6 + f];
+          if (nx >= 0 && !seen[nx]) todo.push_back(nx);
+        }
+      // the +y neighbour of the row's first cell continues the sheet
+      const int64_t ny = nb[row.front() * 6 + 3];
+      if (ny >= 0 && !seen[ny]) todo.push_front(ny);
+    }
+  }
+  return order;
 }
 
 int ensure_solver_ws(sf_plan* P, bool mixed) {
@@ -1238,6 +1394,52 @@ double now_s() {
 
 static int finish_geometry(sf_plan* P);
 
+namespace {
+
+// FP64 CG on the plan's workspace (already in place): the body shared by
+// sf_cg and sf_cg_noalloc.
+int cg_solve(sf_plan* P, const double* b, double* x, double rtol, int maxit,
+             sf_solve_stats* stats, hipStream_t s) {
+  const float* dinv = nullptr;
+  bool gll = false;
+  int rc = ensure_pc(P, s, &dinv, &gll);
+  if (rc) return rc;
+  SF_HIP(hipStreamSynchronize(s));
+  const double t0 = now_s();
+  int iters = 0, nonfinite = 0;
+  double bb = 0.0, rel = 0.0;
+  rc = dot_host(P, b, b, &bb, s);
+  if (rc) return rc;
+  // The recurred residual drifts from the true one; when the recurrence
+  // stops short of rtol in the true residual, restart from x (the restart
+  // recomputes r = b - A x).
+  for (;;) {
+    int it = 0;
+    rc = cg_f64(P, b, x, rtol, maxit - iters, s, &it, &nonfinite, dinv, gll);
+    if (rc) return rc;
+    iters += it;
+    double rn = 0.0;
+    rc = true_residual(P, b, x, &rn, s);
+    if (rc) return rc;
+    rel = bb > 0 ? rn / sqrt(bb) : rn;
+    if (nonfinite || !(rel > rtol) || iters >= maxit || it == 0) break;
+  }
+  const double t1 = now_s();
+  if (stats) {
+    stats->iterations = iters;
+    stats->outer_iterations = 0;
+    stats->rel_residual = rel;
+    stats->converged = (rel <= rtol) ? 1 : 0;
+    stats->seconds = t1 - t0;
+  }
+  if (nonfinite || !isfinite(rel)) return fail(SF_ERR_NONFINITE, "CG met a non-finite value");
+  if (rel > rtol && iters >= maxit)
+    return fail(SF_ERR_NOT_CONVERGED, "CG did not reach rtol %g in %d iterations (rel. residual %g)", rtol, maxit, rel);
+  return SF_OK;
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // C ABI
 
@@ -1363,6 +1565,7 @@ int sf_plan_get_info(const sf_plan* P, sf_plan_info* info) {
   info->device_bytes = P->device_bytes();
   info->n_blocks = P->nblk;
   info->launches_per_apply = 1;
+  info->cells_reordered = P->perm.empty() ? 0 : 1;
   return SF_OK;
   SF_CATCH
 }
@@ -1449,6 +1652,33 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
     }
   }
   P->pc_valid = false;
+  // 3D: blocks follow the cell order, so a caller order whose consecutive
+  // cells are not neighbours (any numbering but a row-wise one) would
+  // leave one cell per block.  The plan then keeps its cells in the order
+  // built from the connectivity when that gives fewer blocks, and maps
+  // per-cell inputs and outputs between the two orders.
+  P->perm.clear();
+  P->corig.release();
+  if (P->dim == 3 && P->ncell > 1) {
+    int nt = 0, eb = 0;
+    apply_shape_f64(3, P->n, P->q, &nt, &eb);
+    const int64_t E = P->ncell;
+    const int64_t nb_caller = (int64_t)block_starts(h, E, nl, eb).size() - 1;
+    if (eb > 0 && nb_caller > (E + eb - 1) / eb) {
+      std::vector<int32_t> ord = connectivity_order(h, E, P->n);
+      std::vector<int32_t> hp(h.size());
+      for (int64_t i = 0; i < E; ++i)
+        std::copy(h.begin() + (int64_t)ord[i] * nl, h.begin() + ((int64_t)ord[i] + 1) * nl,
+                  hp.begin() + i * nl);
+      const int64_t nb_conn = (int64_t)block_starts(hp, E, nl, eb).size() - 1;
+      if (nb_conn < nb_caller) {
+        h.swap(hp);
+        P->perm.swap(ord);
+        SF_HIP(P->corig.alloc(E));
+        SF_HIP(hipMemcpy(P->corig.p, P->perm.data(), E * sizeof(int32_t), hipMemcpyHostToDevice));
+      }
+    }
+  }
   SF_HIP(P->cells.alloc(P->ncell * nl));
   if (!h.empty())
     SF_HIP(hipMemcpy(P->cells.p, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -1506,7 +1736,9 @@ int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
   int rc = SF_OK;
   do {
     if (nv > 0) {
-      hipError_t e = hipMemcpy(X.p, vertices, nv * 8, hipMemcpyHostToDevice);
+      std::vector<double> tmp;
+      const double* Xp = to_plan_order(P, vertices, nv / P->ncell, tmp);
+      hipError_t e = hipMemcpy(X.p, Xp, nv * 8, hipMemcpyHostToDevice);
       if (e != hipSuccess) { rc = fail(SF_ERR_HIP, "vertex upload: %s", hipGetErrorString(e)); break; }
     }
     hipError_t e = P->G.alloc(P->ncell * P->ng * P->nqc);
@@ -1528,11 +1760,12 @@ int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
     P->has_geom = true;
     rc = finish_geometry(P);
   } while (0);
-  // 3D: keep the vertices; the FP64 apply recomputes G from them
-  // (SFHIP_FUSED_GEOMETRY=0 reads the stored G instead)
+  // 3D: the FP64 apply recomputes G from the vertices (SFHIP_FUSED_GEOMETRY=0
+  // reads the stored G instead).  Every form keeps them (24 or 8 values per
+  // cell): new quadrature weights (sf_plan_set_tables) recompute G from them.
   const char* fg = getenv("SFHIP_FUSED_GEOMETRY");
   P->fused_geom = rc == SF_OK && P->dim == 3 && !(fg && fg[0] == '0');
-  if (!P->fused_geom) X.release();
+  if (rc != SF_OK) X.release();
   return rc;
   SF_CATCH
 }
@@ -1552,7 +1785,9 @@ int sf_plan_set_geometry(sf_plan* P, const double* G) {
     if (P->dim == 3) {
       DevBuf<double> tmp;
       SF_HIP(tmp.alloc(n));
-      hipError_t e = hipMemcpy(tmp.p, G, n * 8, hipMemcpyHostToDevice);
+      std::vector<double> hrows;
+      const double* Gp = to_plan_order(P, G, P->ng * P->nqc, hrows);
+      hipError_t e = hipMemcpy(tmp.p, Gp, n * 8, hipMemcpyHostToDevice);
       if (e == hipSuccess)
         e = launch_g3_relayout(tmp.p, P->G.p, P->q, P->ncell, P->cell_block.p, P->bcell.p, 1, nullptr);
       if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -1583,9 +1818,18 @@ int sf_plan_get_geometry(const sf_plan* P, double* G) {
       SF_HIP(tmp.alloc(n));
       hipError_t e = launch_g3_relayout(P->G.p, tmp.p, P->q, P->ncell, P->cell_block.p,
                                         P->bcell.p, 0, nullptr);
-      if (e == hipSuccess) e = hipMemcpy(G, tmp.p, n * 8, hipMemcpyDeviceToHost);
+      std::vector<double> hrows;
+      if (!P->perm.empty()) hrows.resize((size_t)n);
+      double* dst = P->perm.empty() ? G : hrows.data();
+      if (e == hipSuccess) e = hipMemcpy(dst, tmp.p, n * 8, hipMemcpyDeviceToHost);
       tmp.release();
       SF_HIP(e);
+      if (!P->perm.empty()) {
+        const int64_t w = P->ng * P->nqc;
+        for (int64_t i = 0; i < P->ncell; ++i)
+          std::copy(hrows.begin() + i * w, hrows.begin() + (i + 1) * w,
+                    G + (int64_t)P->perm[i] * w);
+      }
     } else {
       SF_HIP(hipMemcpy(G, P->G.p, n * 8, hipMemcpyDeviceToHost));
     }
@@ -1607,7 +1851,9 @@ int sf_plan_set_kappa(sf_plan* P, const double* kappa) {
   // kappa, kappa * G and their copies: let it finish before they change
   SF_HIP(hipDeviceSynchronize());
   SF_HIP(P->kappa.alloc(n));
-  if (n > 0) SF_HIP(hipMemcpy(P->kappa.p, kappa, n * 8, hipMemcpyHostToDevice));
+  std::vector<double> krows;
+  const double* kp = to_plan_order(P, kappa, P->nqc, krows);
+  if (n > 0) SF_HIP(hipMemcpy(P->kappa.p, kp, n * 8, hipMemcpyHostToDevice));
   P->has_kappa = true;
   P->k32_valid = false;
   P->pc_valid = false;
@@ -1699,7 +1945,7 @@ int sf_adjoint(sf_plan* P, const double* u, const double* lam,
     if (dJdu) SF_HIP(launch_fill<double>(dJdu, P->nfree, 0.0, s));
     hipError_t e = adjoint3d_fused_f64(P->n, P->q, u, lam, dJdkappa, dJdu, &P->kargs,
                                        P->dB.p, P->dD.p, P->X.p, P->dPts.p, P->dW.p,
-                                       P->has_kappa ? P->kappa.p : nullptr, s);
+                                       P->has_kappa ? P->kappa.p : nullptr, P->corig.p, s);
     if (e == hipErrorNotSupported)
       return fail(SF_ERR_UNSUPPORTED, "no adjoint kernel for dim=%d n=%d q=%d", P->dim, P->n,
                   P->q);
@@ -1709,7 +1955,8 @@ int sf_adjoint(sf_plan* P, const double* u, const double* lam,
   if (dJdu) SF_HIP(launch_fill<double>(dJdu, P->nfree, 0.0, s));
   hipError_t e = adjoint_f64(P->dim, P->n, P->q, u, lam, dJdkappa, dJdu,
                              P->cells.p, P->G.p, P->has_kappa ? P->kappa.p : nullptr,
-                             P->dB.p, P->dD.p, P->ncell, P->cell_block.p, P->bcell.p, s);
+                             P->dB.p, P->dD.p, P->ncell, P->cell_block.p, P->bcell.p,
+                             P->corig.p, s);
   if (e == hipErrorNotSupported)
     return fail(SF_ERR_UNSUPPORTED, "no adjoint kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
   SF_HIP(e);
@@ -1780,42 +2027,7 @@ int sf_cg(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   hipStream_t s = (hipStream_t)stream;
   rc = ensure_solver_ws(P, false);
   if (rc) return rc;
-  const float* dinv = nullptr;
-  bool gll = false;
-  rc = ensure_pc(P, s, &dinv, &gll);
-  if (rc) return rc;
-  SF_HIP(hipStreamSynchronize(s));
-  const double t0 = now_s();
-  int iters = 0, nonfinite = 0;
-  double bb = 0.0, rel = 0.0;
-  rc = dot_host(P, b, b, &bb, s);
-  if (rc) return rc;
-  // The recurred residual drifts from the true one; when the recurrence
-  // stops short of rtol in the true residual, restart from x (the restart
-  // recomputes r = b - A x).
-  for (;;) {
-    int it = 0;
-    rc = cg_f64(P, b, x, rtol, maxit - iters, s, &it, &nonfinite, dinv, gll);
-    if (rc) return rc;
-    iters += it;
-    double rn = 0.0;
-    rc = true_residual(P, b, x, &rn, s);
-    if (rc) return rc;
-    rel = bb > 0 ? rn / sqrt(bb) : rn;
-    if (nonfinite || !(rel > rtol) || iters >= maxit || it == 0) break;
-  }
-  const double t1 = now_s();
-  if (stats) {
-    stats->iterations = iters;
-    stats->outer_iterations = 0;
-    stats->rel_residual = rel;
-    stats->converged = (rel <= rtol) ? 1 : 0;
-    stats->seconds = t1 - t0;
-  }
-  if (nonfinite || !isfinite(rel)) return fail(SF_ERR_NONFINITE, "CG met a non-finite value");
-  if (rel > rtol && iters >= maxit)
-    return fail(SF_ERR_NOT_CONVERGED, "CG did not reach rtol %g in %d iterations (rel. residual %g)", rtol, maxit, rel);
-  return SF_OK;
+  return cg_solve(P, b, x, rtol, maxit, stats, s);
   SF_CATCH
 }
 

@@ -891,5 +891,6 @@ template hipError_t launch_cg_set_rz<float, float>(const float*, const float*, i
 SF_INST(double)
 SF_INST(float)
 #undef SF_INST
+template hipError_t launch_fill<unsigned>(unsigned*, int64_t, unsigned, hipStream_t);
 
 }  // namespace sf

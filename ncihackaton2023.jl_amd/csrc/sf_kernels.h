@@ -78,13 +78,15 @@ hipError_t launch_g3_to_mfma(const double* G, uint32_t* out, int q,
 // ---- adjoint (sf_adjoint.hip) ----
 // dJdk[cell][qp] = grad(lam) . G grad(u);  ybar += A(kappa) lam
 // 3D reads G through the block-interleaved layout (cell_block, bcell).
+// corig (3D, may be null): the caller's index of each plan cell, the row
+// dJdk is written to.
 hipError_t adjoint_f64(int dim, int n, int q, const double* u,
                        const double* lam, double* dJdk, double* ybar,
                        const int32_t* cells, const double* G,
                        const double* kappa, const double* tB,
                        const double* tD, int64_t ncell,
                        const int32_t* cell_block, const int64_t* bcell,
-                       hipStream_t s);
+                       const int32_t* corig, hipStream_t s);
 
 // 3D, block-assembled, geometry recomputed from the cell vertices Xv (24 per
 // cell) with the 1D points / weights tP, tW: dJdk[cell][qp] (required) and,
@@ -94,7 +96,7 @@ hipError_t adjoint3d_fused_f64(int n, int q, const double* u, const double* lam,
                                double* dJdk, double* ybar, const BlockArgs* k,
                                const double* tB, const double* tD, const double* Xv,
                                const double* tP, const double* tW, const double* kappa,
-                               hipStream_t s);
+                               const int32_t* corig, hipStream_t s);
 
 // ---- diagonal of the operator (sf_diag.hip) ----
 // diag += the element-matrix diagonals of every cell (the caller zeroes

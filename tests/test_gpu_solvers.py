@@ -201,7 +201,7 @@ def test_jacobi_pcg(pkg, oracle_mod, cuda, monkeypatch, D, m, p, q, with_kappa):
     x0h = x0.cpu().numpy()
     runs = {}
     x = cuda.zeros_like(b)
-    runs["f64"] = (pkg.cg_(x, A, b, rtol=1e-10, maxiter=200000, Pl="jacobi"), x)
+    runs["cg"] = (pkg.cg_(x, A, b, rtol=1e-10, maxiter=200000, Pl="jacobi"), x)
     recs = ("f64p32", "f64", "f32") if D == 3 else ("f64", "f32")
     for rec in recs:
         xm = cuda.zeros_like(b)
@@ -214,7 +214,7 @@ def test_jacobi_pcg(pkg, oracle_mod, cuda, monkeypatch, D, m, p, q, with_kappa):
     if D == 3:
         Au = lazy(pkg, mp, space, kappa, monkeypatch, False)
         xu = cuda.zeros_like(b)
-        runs["f64_unfused"] = (pkg.cg_(xu, Au, b, rtol=1e-10, maxiter=200000, Pl="jacobi"), xu)
+        runs["cg_unfused"] = (pkg.cg_(xu, Au, b, rtol=1e-10, maxiter=200000, Pl="jacobi"), xu)
         Au.free()
     say("  plain", s0)
     for key, (st, xx) in runs.items():
@@ -225,7 +225,7 @@ def test_jacobi_pcg(pkg, oracle_mod, cuda, monkeypatch, D, m, p, q, with_kappa):
         assert rel < 2e-10, (key, rel)
         assert rel_l2(xh, x0h) < 1e-6, key
     if D == 3:
-        it_f, it_u = runs["f64"][0]["iterations"], runs["f64_unfused"][0]["iterations"]
+        it_f, it_u = runs["cg"][0]["iterations"], runs["cg_unfused"][0]["iterations"]
         assert abs(it_f - it_u) <= max(5, 0.02 * it_u), (it_f, it_u)
     # switching the preconditioner off again gives the plain iteration
     x2 = cuda.zeros_like(b)
@@ -387,4 +387,79 @@ def test_set_tables_weights_after_geometry(pkg, oracle_mod, cuda, monkeypatch):
     assert rc == pkg._lib.SF_ERR_NOT_READY
     # tables without weights are still accepted
     pkg._lib.check(Ag._L.sf_plan_set_tables(Ag.plan, B.ctypes.data, Dm.ctypes.data, None))
+    Ag.free()
+
+
+class _Reordered:
+    """A space / model pair with the cells listed in another order."""
+
+    def __init__(self, space, order):
+        self.D, self.order, self.n = space.D, space.order, space.n
+        self.n_free, self.n_cells = space.n_free, space.n_cells
+        self.cell_dofs = np.ascontiguousarray(space.cell_dofs[order])
+        verts = space.model.cell_vertices()[order]
+
+        class _M:
+            def cell_vertices(self_inner):
+                return verts
+
+        self.model = _M()
+
+
+def test_shuffled_cell_order(pkg, oracle_mod, cuda, monkeypatch):
+    """Cells listed in a random order (no two consecutive cells neighbours):
+    the plan orders them by connectivity, so the block count equals the
+    row-wise order's; A x, the kappa gradient (in the caller's cell order),
+    G read back, and the GLL-preconditioned CG all match the oracle on the
+    shuffled arrays; a row-wise order is kept as it is."""
+    say("shuffled cell order")
+    model, space, mp, _ = problem(pkg, 3, 6, 3, 5)
+    rng = np.random.default_rng(11)
+    order = rng.permutation(space.n_cells)
+    sh = _Reordered(space, order)
+    kappa = np.exp(pkg.normal(3, space.n_cells * 125))
+    # kappa per cell follows the caller's cell order
+    ks = np.ascontiguousarray(kappa.reshape(space.n_cells, -1)[order].reshape(-1))
+    Xs = sh.model.cell_vertices()
+    for fused in (True, False):
+        A0 = lazy(pkg, mp, space, kappa, monkeypatch, fused)
+        As = lazy(pkg, mp, sh, ks, monkeypatch, fused)
+        i0, i1 = A0.info(), As.info()
+        say("  fused=%d blocks: row-wise %d (reordered %d), shuffled %d (reordered %d)"
+            % (fused, i0["n_blocks"], i0["cells_reordered"], i1["n_blocks"],
+               i1["cells_reordered"]))
+        assert i0["cells_reordered"] == 0 and i1["cells_reordered"] == 1
+        assert i1["n_blocks"] == i0["n_blocks"]
+        xh = pkg.uniform(0, space.n_free, -1, 1)
+        lh = pkg.uniform(1, space.n_free, -1, 1)
+        y_ref = oracle_mod.apply(3, 4, 5, sh.cell_dofs, Xs, xh, mp.tables, kappa=ks)
+        x, lam = cuda.from_numpy(xh).cuda(), cuda.from_numpy(lh).cuda()
+        y = cuda.empty_like(x)
+        As.mul(y, x)
+        kb, ub = pkg.adjoint.pullback(As, x, lam)
+        cuda.cuda.synchronize()
+        assert rel_l2(y.cpu().numpy(), y_ref) < 1e-12
+        kb_ref = oracle_mod.kappa_grad(3, 4, 5, sh.cell_dofs, Xs, xh, lh, mp.tables).reshape(-1)
+        assert rel_l2(kb.cpu().numpy(), kb_ref) < 1e-12
+        ub_ref = oracle_mod.apply(3, 4, 5, sh.cell_dofs, Xs, lh, mp.tables, kappa=ks)
+        assert rel_l2(ub.cpu().numpy(), ub_ref) < 1e-12
+        G_ref = oracle_mod.geometry(3, 5, Xs, mp.tables)
+        assert rel_l2(As.geometry().reshape(-1), G_ref.reshape(-1)) < 1e-13
+        b = cuda.from_numpy(pkg.normal(2, space.n_free)).cuda()
+        x0, x1 = cuda.zeros_like(b), cuda.zeros_like(b)
+        s0 = pkg.cg_(x0, A0, b, rtol=1e-10, maxiter=20000, Pl="gll")
+        s1 = pkg.cg_(x1, As, b, rtol=1e-10, maxiter=20000, Pl="gll")
+        assert s1["converged"] == 1 and abs(s1["iterations"] - s0["iterations"]) <= 2
+        assert rel_l2(x1.cpu().numpy(), x0.cpu().numpy()) < 1e-8
+        A0.free()
+        As.free()
+    # a caller-supplied G in the shuffled order goes through the same map
+    Ag = pkg.LazyMatrix(mp, sh, geometry=oracle_mod.geometry(3, 5, Xs, mp.tables))
+    assert Ag.info()["cells_reordered"] == 1
+    xh = pkg.uniform(0, space.n_free, -1, 1)
+    y = cuda.empty(space.n_free, dtype=cuda.float64, device="cuda")
+    Ag.mul(y, cuda.from_numpy(xh).cuda())
+    cuda.cuda.synchronize()
+    y_ref = oracle_mod.apply(3, 4, 5, sh.cell_dofs, Xs, xh, mp.tables)
+    assert rel_l2(y.cpu().numpy(), y_ref) < 1e-12
     Ag.free()
