@@ -261,6 +261,25 @@ int sf_precond_apply(sf_plan* plan, double* z, const double* r, void* stream);
 int sf_cg(sf_plan* plan, const double* b, double* x, double rtol, int maxit,
           sf_solve_stats* stats, void* stream);
 
+/* Size of the device array behind sf_cg_noalloc, per GPU (128 MiB). */
+#define SF_NOALLOC_SCRATCH_BYTES (128u << 20)
+
+/* sf_cg without any device allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (serving loops, code
+ * under stream capture, allocation-free harnesses).  The CG workspace
+ * (4 vectors of n_free doubles, 6 with SF_PC_GLL; reduction shares; the
+ * scalars) lives in one fixed device array of SF_NOALLOC_SCRATCH_BYTES per
+ * GPU, which the library holds until the process ends: about 4M free dofs
+ * unpreconditioned or with Jacobi, 2.7M with SF_PC_GLL.  A plan whose
+ * workspace does not fit is refused (SF_ERR_ARG); so is a plan whose
+ * preconditioner is selected but not built yet (SF_ERR_NOT_READY: one
+ * sf_cg or sf_precond_apply call builds it).  Same arguments, results and
+ * errors as sf_cg otherwise.  Every call on a GPU shares that GPU's array
+ * and keeps nothing from the previous one: calls that overlap on one GPU
+ * (two host threads, two streams) race. */
+int sf_cg_noalloc(sf_plan* plan, const double* b, double* x, double rtol, int maxit,
+                  sf_solve_stats* stats, void* stream);
+
 /* Mixed-precision iterative refinement: FP64 outer residual r = b - A x,
  * FP64 norms and dot products, inner CG with the reduced-precision operator
  * `inner_precision` (FP32 vectors) to relative tolerance inner_rtol or

@@ -14,13 +14,13 @@ from .fe import (SumFactorizationMap, get_dimensional_parameters, get_dims, get_
                  gauss_points_for_degree)
 from .mesh import CartesianModel, FESpace, uniform, normal, splitmix64
 from .operator import LazyMatrix, mul_
-from .solvers import cg_, cg_ir_, cg_mixed_, compute_solution
+from .solvers import cg_, cg_ir_, cg_mixed_, cg_noalloc_, compute_solution
 from . import adjoint
 
 __all__ = ["SumFactorizationMap", "get_dimensional_parameters", "get_dims", "get_basis_sizes",
            "get_quads_sizes", "count_manual_flops_poisson_matrix_free", "tables_1d",
            "gauss_points_for_degree", "CartesianModel", "FESpace", "uniform", "normal",
-           "splitmix64", "LazyMatrix", "mul_", "cg_", "cg_ir_", "cg_mixed_", "compute_solution", "adjoint",
+           "splitmix64", "LazyMatrix", "mul_", "cg_", "cg_ir_", "cg_mixed_", "cg_noalloc_", "compute_solution", "adjoint",
            "SFError", "PRECISIONS", "load_library"]
 
 

@@ -102,6 +102,7 @@ SIGNATURES = [
     ("sf_plan_set_preconditioner", _i, [_vp, _i]),
     ("sf_precond_apply", _i, [_vp, _vp, _vp, _vp]),
     ("sf_cg", _i, [_vp, _vp, _vp, _d, _i, ctypes.POINTER(SolveStats), _vp]),
+    ("sf_cg_noalloc", _i, [_vp, _vp, _vp, _d, _i, ctypes.POINTER(SolveStats), _vp]),
     ("sf_cg_ir", _i, [_vp, _vp, _vp, _d, _i, _i, _d, _i, ctypes.POINTER(SolveStats), _vp]),
     ("sf_cg_mixed", _i, [_vp, _vp, _vp, _d, _i, _i, _d, _i, ctypes.POINTER(SolveStats), _vp]),
 ]

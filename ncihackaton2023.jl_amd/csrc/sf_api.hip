@@ -13,6 +13,7 @@
 #include <climits>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <unordered_map>
@@ -24,6 +25,11 @@
 #include "sfhip.h"
 
 using namespace sf;
+
+namespace {
+constexpr size_t kScratchBytes = SF_NOALLOC_SCRATCH_BYTES;  // per GPU
+constexpr int kMaxDevices = 64;
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // errors
@@ -1438,6 +1444,65 @@ int cg_solve(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   return SF_OK;
 }
 
+// Offsets of the workspace pieces inside the scratch array, each on a
+// 256-byte boundary.
+struct ScratchCarve {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    off = (off + 255) & ~(size_t)255;
+    const size_t at = off;
+    off += bytes;
+    return at;
+  }
+};
+
+// A plan buffer pointed at scratch memory for one call, restored on exit
+// (the buffer never frees what it points at meanwhile: every alloc() it
+// sees asks for the size it already has).
+template <typename T>
+struct BufSwap {
+  DevBuf<T>& b;
+  T* p0;
+  int64_t n0;
+  BufSwap(DevBuf<T>& b_, T* p, int64_t n) : b(b_), p0(b_.p), n0(b_.n) {
+    b.p = p;
+    b.n = n;
+  }
+  ~BufSwap() {
+    b.p = p0;
+    b.n = n0;
+  }
+};
+
+}  // namespace
+
+// The no-allocation CG's workspace: one fixed array per GPU, held until the
+// process ends.  Host code never uses this name as an address (on the host
+// it is a stand-in): scratch_base() asks the runtime for each device's
+// copy once and keeps it.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+char* g_scratch_base[kMaxDevices] = {};
+
+// The device address of `scratch` on device dev (current), looked up once.
+int scratch_base(int dev, char** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxDevices) return fail(SF_ERR_ARG, "device %d out of range", dev);
+  if (!g_scratch_base[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(SF_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_base[dev] = (char*)ptr;
+  }
+  *out = g_scratch_base[dev];
+  return SF_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2027,6 +2092,53 @@ int sf_cg(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   hipStream_t s = (hipStream_t)stream;
   rc = ensure_solver_ws(P, false);
   if (rc) return rc;
+  return cg_solve(P, b, x, rtol, maxit, stats, s);
+  SF_CATCH
+}
+
+int sf_cg_noalloc(sf_plan* P, const double* b, double* x, double rtol, int maxit,
+                  sf_solve_stats* stats, void* stream) {
+  SF_TRY
+  int rc = check_ready(P);
+  if (rc) return rc;
+  if ((!b || !x) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
+  if (!(rtol >= 0.0) || maxit < 0) return fail(SF_ERR_ARG, "bad rtol / maxit");
+  if (P->pc_kind != SF_PC_NONE && !P->pc_valid && P->nfree > 0)
+    return fail(SF_ERR_NOT_READY,
+                "the preconditioner is not built yet (one sf_cg or sf_precond_apply call "
+                "builds it; the no-allocation CG cannot)");
+  DeviceGuard g(P->device);
+  hipStream_t s = (hipStream_t)stream;
+  // the workspace sf_cg allocates, carved out of this GPU's scratch array
+  const bool gll = P->pc_kind == SF_PC_GLL;
+  const int64_t nf = P->nfree;
+  const int64_t np = (P->dim == 3 && P->npart > 0) ? P->npart : 0;
+  ScratchCarve c;
+  const size_t o_r = c.take(nf * 8), o_p = c.take(nf * 8), o_q = c.take(nf * 8),
+               o_t = c.take(nf * 8);
+  const size_t o_z = gll ? c.take(nf * 8) : 0, o_w = gll ? c.take(nf * 8) : 0;
+  const size_t o_red = c.take(2 * (size_t)kPartSlots * 8), o_part = c.take(np * 8);
+  const size_t o_cnt = c.take(sizeof(unsigned)), o_sc = c.take(sizeof(CGScalars));
+  const size_t o_ds = c.take(4 * 8);
+  if (c.off > kScratchBytes)
+    return fail(SF_ERR_ARG,
+                "the no-allocation CG needs %zu bytes of workspace for %lld free dofs; its "
+                "scratch holds %zu",
+                c.off, (long long)nf, (size_t)kScratchBytes);
+  char* base = nullptr;
+  rc = scratch_base(P->device, &base);
+  if (rc) return rc;
+  BufSwap<double> s_r(P->w_r, (double*)(base + o_r), nf), s_p(P->w_p, (double*)(base + o_p), nf),
+      s_q(P->w_q, (double*)(base + o_q), nf), s_t(P->w_t, (double*)(base + o_t), nf),
+      s_z(P->w_z, gll ? (double*)(base + o_z) : P->w_z.p, gll ? nf : P->w_z.n),
+      s_w(P->w_w, gll ? (double*)(base + o_w) : P->w_w.p, gll ? nf : P->w_w.n),
+      s_red(P->red_part, (double*)(base + o_red), 2 * (int64_t)kPartSlots),
+      s_part(P->w_part, np ? (double*)(base + o_part) : P->w_part.p, np ? np : P->w_part.n),
+      s_ds(P->dscal, (double*)(base + o_ds), 4);
+  BufSwap<unsigned> s_cnt(P->red_counter, (unsigned*)(base + o_cnt), 1);
+  BufSwap<CGScalars> s_sc(P->scal, (CGScalars*)(base + o_sc), 1);
+  // the arrival counter of the single-launch reductions starts at zero
+  SF_HIP(launch_fill<unsigned>(P->red_counter.p, 1, 0u, s));
   return cg_solve(P, b, x, rtol, maxit, stats, s);
   SF_CATCH
 }

@@ -48,6 +48,21 @@ def cg_(x, A, b, rtol=1e-10, maxiter=None, raise_on_fail=False, Pl=None):
     return st.as_dict()
 
 
+def cg_noalloc_(x, A, b, rtol=1e-10, maxiter=None, raise_on_fail=False):
+    """cg_ with the plan's current preconditioner and no device allocation:
+    the workspace lives in the library's fixed per-GPU scratch array
+    (sf_cg_noalloc; concurrent calls on one GPU race)."""
+    _vec_ok(A, x)
+    _vec_ok(A, b)
+    st = _lib.SolveStats()
+    maxiter = A.n_free if maxiter is None else int(maxiter)
+    rc = A._L.sf_cg_noalloc(A.plan, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                            float(rtol), maxiter, ctypes.byref(st), _torch_stream(x))
+    if rc != _lib.SF_OK and (raise_on_fail or rc != _lib.SF_ERR_NOT_CONVERGED):
+        _lib.check(rc)
+    return st.as_dict()
+
+
 def cg_ir_(x, A, b, rtol=1e-10, inner="f32", inner_rtol=1e-3, inner_maxiter=None,
            max_outer=50, raise_on_fail=False, Pl=None):
     """Mixed-precision iterative refinement (FP64 outer, reduced inner,
