@@ -28,7 +28,9 @@
 // dofs gathered once (sorted ids, coalesced), three 1D contractions per
 // cell through LDS (one N-long pencil per thread), and one global access
 // per distinct dof on the way out (the transposed pass adds its shared
-// dofs with atomics into a zeroed w).
+// dofs with atomics into a zeroed w).  The LDS per block (its dof values
+// and one tensor per cell) is what bounds the occupancy of these light
+// passes.
 #include "sf_apply.hpp"
 #include "sf_common.hpp"
 #include "sf_kernels.h"
@@ -38,8 +40,13 @@ namespace sf {
 namespace {
 
 // Cells of the block run side by side: thread t owns pencil t % N^2 of
-// cell t / N^2 in each pass.  LDS (dynamic): the block's gathered inputs
-// (nu values), its outputs (nu values), then two N^3 tensors per cell.
+// cell t / N^2 in each pass.  LDS (dynamic): the block's nu distinct-dof
+// values (the gathered inputs, later the outputs), then one N x N x NP
+// tensor per cell (x stride padded to an odd NP: the x pass walks a
+// pencil per thread).  The y pass rewrites its own pencil in place (a
+// thread reads and writes only its pencil's N positions), so one tensor
+// per cell suffices.  The vertex rows of T1 are unit rows, so part of each
+// contraction is a copy (forward) or skips the two vertex inputs (TRANS).
 template <int N, typename TR, typename TZ, bool TRANS>
 __global__ void __launch_bounds__(1024)
     k_pc_gll(const TR* __restrict__ in, const float* __restrict__ scale,
@@ -48,10 +55,10 @@ __global__ void __launch_bounds__(1024)
              const int64_t* __restrict__ boff, const int32_t* __restrict__ bexcl,
              const int64_t* __restrict__ bcell, int nblk, int eb,
              const CGScalars* __restrict__ sc) {
-  constexpr int NN = N * N, NNN = N * N * N;
+  constexpr int NN = N * N, NNN = N * N * N, NP = N | 1, CS = NN * NP;
   if (sc && sc->done) return;  // uniform: written only by other kernels
   extern __shared__ double pc_smem[];
-  TZ* const sm = (TZ*)pc_smem;
+  TZ* const V = (TZ*)pc_smem;
   const int b = xcd_remap(blockIdx.x, nblk);
   const int64_t base = bcell[b];
   const int ncb = (int)(bcell[b + 1] - base);
@@ -59,87 +66,102 @@ __global__ void __launch_bounds__(1024)
   const int nu = (int)(boff[b + 1] - d0);
   const int nexcl = bexcl[b];
   const int32_t* __restrict__ dofs = bdofs + d0;
-  TZ* const vin = sm;             // gathered inputs
-  TZ* const vout = sm + nu;       // assembled (TRANS) or computed outputs
-  TZ* const S0 = sm + 2 * nu;     // per cell: two N^3 tensors
+  TZ* const S0 = V + ((nu + 1) & ~1);  // 8-byte aligned for FP32 too
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int i = tid; i < nu; i += nt) {
-    const int32_t d = __builtin_nontemporal_load(dofs + i);
-    vin[i] = (TZ)((double)in[d] * (double)scale[d]);
-    if (TRANS) vout[i] = TZ(0);
-  }
-  __syncthreads();
   const int e = tid / NN, r = tid - e * NN;
   const bool act = e < ncb && e < eb;
-  TZ* const A0 = S0 + (size_t)(act ? e : 0) * 2 * NNN;
-  TZ* const A1 = A0 + NNN;
+  TZ* const A = S0 + (size_t)(act ? e : 0) * CS;
   const uint16_t* lp = pidx + (base + (act ? e : 0)) * NNN;
-  // the cell's local values, tensor-lexicographic (x fastest)
-  if (act) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-      const uint16_t li = __builtin_nontemporal_load(lp + r * N + k);
-      A0[r * N + k] = li != kNoDof ? vin[li] : TZ(0);
-    }
-  }
-  __syncthreads();
-  // M row-major N x N: out[o] = sum_i M[o][i] in[i] along one direction
-  // (TRANS: M = T1^T, else M = T1)
   const int c1 = r / N, c0 = r - c1 * N;  // pencil coordinates (slow, fast)
-  // x pass: pencil (z, y) = (c1, c0) -> A1
-  if (act) {
-    TZ u[N];
+  // the local indices of this thread's x pencil (z, y) = (c1, c0) and z
+  // pencil (y, x) = (c1, c0), loaded with the gather so that their latency
+  // overlaps it
+  uint16_t lx[N], lz[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) u[i] = A0[r * N + i];
+  for (int i = 0; i < N; ++i) {
+    lx[i] = act ? __builtin_nontemporal_load(lp + r * N + i) : kNoDof;
+    lz[i] = act ? __builtin_nontemporal_load(lp + (i * N + c1) * N + c0) : kNoDof;
+  }
+  // the block's nu <= eb N^3 distinct dofs, at most N per thread (the
+  // launch has nt >= eb N^2 threads): all loads issued at once
+  int32_t dv[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int i = tid + k * nt;
+    dv[k] = i < nu ? __builtin_nontemporal_load(dofs + i) : -1;
+  }
+  TZ xv[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+    xv[k] = dv[k] >= 0 ? (TZ)((double)in[dv[k]] * (double)scale[dv[k]]) : TZ(0);
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+    if (dv[k] >= 0) V[tid + k * nt] = xv[k];
+  __syncthreads();
+  // out[o] = sum_i M[o][i] u[i] (TRANS: M = T1^T, else M = T1)
+  auto contract = [&](const TZ (&u)[N], TZ (&w)[N]) {
 #pragma unroll
     for (int o = 0; o < N; ++o) {
-      TZ v = TZ(0);
+      if (!TRANS && o < 2) {
+        w[o] = u[o];  // unit rows of T1
+        continue;
+      }
+      TZ v = TRANS && o < 2 ? u[o] : TZ(0);
 #pragma unroll
-      for (int i = 0; i < N; ++i) v = fma((TZ)M[o * N + i], u[i], v);
-      A1[r * N + o] = v;
+      for (int i = TRANS ? 2 : 0; i < N; ++i) v = fma((TZ)M[o * N + i], u[i], v);
+      w[o] = v;
     }
+  };
+  // x pass: pencil (z, y) = (c1, c0), gathered straight from V
+  if (act) {
+    TZ u[N], w[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) u[i] = lx[i] != kNoDof ? V[lx[i]] : TZ(0);
+    contract(u, w);
+#pragma unroll
+    for (int o = 0; o < N; ++o) A[r * NP + o] = w[o];
   }
   __syncthreads();
-  // y pass: pencil (z, x) = (c1, c0) -> A0
+  if (TRANS) {  // V is read: it becomes the accumulator
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+      if (dv[k] >= 0) V[tid + k * nt] = TZ(0);
+  }
+  // y pass, in place: pencil (z, x) = (c1, c0)
   if (act) {
-    TZ u[N];
+    TZ u[N], w[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) u[i] = A1[(c1 * N + i) * N + c0];
+    for (int i = 0; i < N; ++i) u[i] = A[(c1 * N + i) * NP + c0];
+    contract(u, w);
 #pragma unroll
-    for (int o = 0; o < N; ++o) {
-      TZ v = TZ(0);
-#pragma unroll
-      for (int i = 0; i < N; ++i) v = fma((TZ)M[o * N + i], u[i], v);
-      A0[(c1 * N + o) * N + c0] = v;
-    }
+    for (int o = 0; o < N; ++o) A[(c1 * N + o) * NP + c0] = w[o];
   }
   __syncthreads();
-  // z pass: pencil (y, x) = (c1, c0), straight into the block's outputs
+  // z pass: pencil (y, x) = (c1, c0), into the block's values
   if (act) {
-    TZ u[N];
+    TZ u[N], w[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) u[i] = A0[(i * N + c1) * N + c0];
+    for (int i = 0; i < N; ++i) u[i] = A[(i * N + c1) * NP + c0];
+    contract(u, w);
 #pragma unroll
     for (int o = 0; o < N; ++o) {
-      TZ v = TZ(0);
-#pragma unroll
-      for (int i = 0; i < N; ++i) v = fma((TZ)M[o * N + i], u[i], v);
-      const uint16_t li = lp[(o * N + c1) * N + c0];
-      if (li != kNoDof) {
+      if (lz[o] != kNoDof) {
         if (TRANS)
-          lds_atomic_add(vout + li, v);
+          lds_atomic_add(V + lz[o], w[o]);
         else
-          vout[li] = v;  // every cell holding the dof computes the same value
+          V[lz[o]] = w[o];  // every cell holding the dof computes the same value
       }
     }
   }
   __syncthreads();
-  for (int i = tid; i < nu; i += nt) {
-    const int32_t d = dofs[i];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int i = tid + k * nt;
+    if (dv[k] < 0) continue;
     if (!TRANS || i < nexcl)
-      out[d] = vout[i];
+      out[dv[k]] = V[i];
     else
-      atomic_add(out + d, vout[i]);
+      atomic_add(out + dv[k], V[i]);
   }
 }
 
@@ -180,8 +202,9 @@ hipError_t launch_pass(int n, const TR* in, const float* scale, TZ* out, const d
                        hipStream_t s) {
   if (k.nblk <= 0) return hipSuccess;
   const int nt = round_up(eb * n * n, 64);
-  const size_t lds = (size_t)(2 * max_nu + 2 * eb * n * n * n) * sizeof(TZ);
-  if (nt > 1024 || lds > 65536) return hipErrorNotSupported;
+  const size_t lds = (size_t)(((max_nu + 1) & ~1) + eb * n * n * (n | 1)) * sizeof(TZ);
+  // every thread gathers at most n of the block's distinct dofs
+  if (nt > 1024 || lds > 65536 || max_nu > n * nt) return hipErrorNotSupported;
   switch (n) {
 #define SF_PC_CASE(N_)                                                                     \
   case N_:                                                                                 \
