@@ -67,10 +67,19 @@ def build_problem(pkg, D, m, p, q, perturb=0.2, kappa=None):
     return model, space, mp, A
 
 
-def time_apply(torch, A, x, y, steps, warmup, reduced=None):
-    """Wall time per step and mean apply time from HIP events recorded on
-    the launch stream around each y = A x (the fill of y and the apply
-    dispatch)."""
+def stats_ms(ts):
+    ts = sorted(ts)
+    return {"mean": sum(ts) / len(ts), "min": ts[0], "median": ts[len(ts) // 2],
+            "max": ts[-1]}
+
+
+def time_apply(torch, A, x, y, steps, warmup, reduced=None, detail=False):
+    """Wall time per step, and the apply kernel's mean duration from the
+    library's HIP events recorded on the launch stream around the kernel
+    alone (not the fill of y).  detail: also the min / median / max of the
+    kernel and of each step (fill + kernel, HIP events on the stream around
+    each y = A x), as the reference's benchmark reports minima and maxima
+    (profiling/benchmark.jl:7-14)."""
     stream = torch.cuda.current_stream()
 
     def step():
@@ -84,6 +93,7 @@ def time_apply(torch, A, x, y, steps, warmup, reduced=None):
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for _ in range(steps)]
     torch.cuda.synchronize()
+    A.timing(True)
     t0 = time.perf_counter()
     for i in range(steps):
         ev[i][0].record(stream)
@@ -91,21 +101,47 @@ def time_apply(torch, A, x, y, steps, warmup, reduced=None):
         ev[i][1].record(stream)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / steps
-    kern = sum(a.elapsed_time(b) for a, b in ev) / steps / 1e3
-    return wall, kern
+    A.timing(False)
+    kern = A.read_timing()
+    assert len(kern) == steps, (len(kern), steps)
+    kstat = stats_ms(kern)
+    if not detail:
+        return wall, kstat["mean"] / 1e3
+    step_ms = [a.elapsed_time(b) for a, b in ev]
+    return wall, kstat["mean"] / 1e3, {"kernel_ms": kstat, "step_ms": stats_ms(step_ms)}
+
+
+def cpu_info():
+    """Usable host threads (this process's CPU affinity, capped by
+    OMP_NUM_THREADS where set: the box's CPU share) and the CPU model."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return max(1, n), model
 
 
 def mode_apply(args, torch, pkg):
     import numpy as np
     D, m, p, q = 3, 64, 4, 6
-    log("setting up 3D 64^3 Q4 q=6 perturbed ...")
+    if args.order != 4:  # another order of the sweep (~16M dofs, q = p + 2)
+        p, q, m = args.order, args.order + 2, int(round(256 / args.order))
+    log("setting up 3D %d^3 Q%d q=%d perturbed ..." % (m, p, q))
     model, space, mp, A = build_problem(pkg, D, m, p, q)
     n = p + 1
     xh = pkg.uniform(0, space.n_free, -1.0, 1.0)
     x = torch.from_numpy(xh).cuda()
     y = torch.empty_like(x)
     log("timing %d steps after %d warmup ..." % (args.steps, args.warmup))
-    wall, kern = time_apply(torch, A, x, y, args.steps, args.warmup)
+    wall, kern, det = time_apply(torch, A, x, y, args.steps, args.warmup, detail=True)
     # the default apply recomputes G from the vertices: bytes are x, y, ids
     # and vertices, and the bound is FP64 arithmetic (the reference's FLOP
     # model, which counts the metric work at every quadrature point)
@@ -123,15 +159,25 @@ def mode_apply(args, torch, pkg):
         "ms_per_step": wall * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded perturbed mesh, u ~ U[-1,1])",
-        "config": {"workload": "3D unit cube 64^3 hexes perturbed +-0.2h, Q4 (5 nodes/dir), "
-                               "6 Gauss pts/dir, FP64 apply y = A u",
+        "config": {"workload": "3D unit cube %d^3 hexes perturbed +-0.2h, Q%d (%d nodes/dir), "
+                               "%d Gauss pts/dir, FP64 apply y = A u" % (m, p, p + 1, q),
                    "n_free": space.n_free, "n_cells": space.n_cells, "order": p, "nquad": q},
-        "roofline": {"bound": "mfma" if t_flop >= t_hbm else "hbm",
+        "roofline": {"bound": "fp64" if t_flop >= t_hbm else "hbm",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": read_traffic(),
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": read_traffic() if args.order == 4 else None,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this "
+                                       "kernel on this workload, scaled by a 1 GiB copy "
+                                       "(profiles/apply3d_f64_fusedgeom_q4_64cube_traffic.txt); "
+                                       "not measured in this run",
                      "peak_note": "FP64 peak, vector ALU = matrix cores on MI355X",
-                     "kernel": "k_apply3d<5,6,double,double,fused geometry>",
+                     "kernel": "k_apply3d<%d,%d,double,double,fused geometry>" % (n, q),
                      "kernel_ms": kern * 1e3,
+                     "kernel_ms_min": det["kernel_ms"]["min"],
+                     "kernel_ms_median": det["kernel_ms"]["median"],
+                     "kernel_timing": "HIP events on the launch stream around the apply kernel "
+                                      "alone, every timed step (mean above)",
+                     "step_ms": det["step_ms"],
                      "dispatches_per_apply": A.info()["launches_per_apply"],
                      "alg_bytes": Balg, "alg_flops": Falg,
                      "achieved_GBs": Balg / kern / 1e9},
@@ -157,7 +203,8 @@ def mode_apply(args, torch, pkg):
     out["stored_g_variant"] = {"gdofs": space.n_free / wall_s / 1e9, "kernel_ms": kern_s * 1e3,
                                "alg_bytes": Bs, "hbm_GBs": Bs / kern_s / 1e9,
                                "hbm_frac": Bs / kern_s / 1e9 / HBM_PEAK_GBS,
-                               "traffic": read_traffic(STORED_G_TRAFFIC_FILE),
+                               "traffic": (read_traffic(STORED_G_TRAFFIC_FILE)
+                                           if args.order == 4 else None),
                                "max_rel_diff_vs_fused": float(torch.linalg.norm(y_s - y) /
                                                               torch.linalg.norm(y))}
     A_s.free()
@@ -165,19 +212,27 @@ def mode_apply(args, torch, pkg):
     yd = y.cpu().numpy()
     if not args.no_cpu:
         import oracle
-        threads = min(16, os.cpu_count() or 1)
+        threads, model_name = cpu_info()
         X = model.cell_vertices()
-        log("CPU oracle (%d threads) ..." % threads)
+        log("CPU oracle (%d threads, %s) ..." % (threads, model_name))
         y_ref = oracle.apply(D, n, q, space.cell_dofs, X, xh, mp.tables, nthreads=threads)
-        reps = 2
-        t0 = time.perf_counter()
-        for _ in range(reps):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
             y_ref = oracle.apply(D, n, q, space.cell_dofs, X, xh, mp.tables, nthreads=threads)
-        tc = (time.perf_counter() - t0) / reps
-        out["cpu_baseline"] = {"value": space.n_free / tc / 1e9, "unit": "GDOF/s",
+            ts.append(time.perf_counter() - t0)
+        log("CPU oracle, 1 thread ...")
+        t0 = time.perf_counter()
+        oracle.apply(D, n, q, space.cell_dofs, X, xh, mp.tables, nthreads=1)
+        t1c = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": space.n_free / min(ts) / 1e9, "unit": "GDOF/s",
                                "cores": threads, "kind": "port",
-                               "sample": "full 64^3 Q4 q=6 apply (the same workload), "
-                                         "mean of %d after 1 warm-up, OpenMP C oracle" % reps}
+                               "sample": "full 64^3 Q4 q=6 apply (the same workload), OpenMP C "
+                                         "oracle; value = best of 3 after 1 warm-up on the "
+                                         "usable host threads",
+                               "mean_gdofs": space.n_free / (sum(ts) / len(ts)) / 1e9,
+                               "one_core_gdofs": space.n_free / t1c / 1e9,
+                               "one_core_s": t1c, "cpu_model": model_name}
         out["parity_rel_l2"] = float(np.linalg.norm(yd - y_ref) / np.linalg.norm(y_ref))
         A.free()
         out["assembled_comparator"] = assembled_comparator(torch, pkg, threads)
@@ -590,6 +645,9 @@ def main():
     ap.add_argument("--mode", default="apply",
                     choices=["apply", "sweep", "cg", "adjoint", "q9", "copycal"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--order", type=int, default=4,
+                    help="apply mode: polynomial order (4: the 64^3 Q4 contract workload; "
+                         "others: the sweep's mesh of about 16M dofs with q = p + 2)")
     ap.add_argument("--maxit", type=int, default=300000)
     ap.add_argument("--max-outer", type=int, default=50)
     ap.add_argument("--fd-dirs", type=int, default=64,

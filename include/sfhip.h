@@ -141,6 +141,19 @@ int sf_plan_destroy(sf_plan* plan);
 
 int sf_plan_get_info(const sf_plan* plan, sf_plan_info* info);
 
+/* Kernel timing of the applies: with enable = 1 the plan records HIP
+ * events on the launch stream around every apply kernel (FP64 and reduced
+ * precision, in sf_apply, sf_apply_add and the solvers; not the fill of y),
+ * restarting the record; 0 stops recording.  sf_plan_read_timing waits for
+ * the recorded events, writes up to max per-launch durations in ms (oldest
+ * first), sets *count to the number recorded and clears the record.  A
+ * measurement aid that changes no result; the reference times its kernels
+ * with CUDA.@profile (test/GPU_v3.jl:96-100).  The library also marks the
+ * apply, the solves, their chunks of iterations and the preconditioner
+ * set-up as ranges for the profiler's marker trace. */
+int sf_plan_set_timing(sf_plan* plan, int enable);
+int sf_plan_read_timing(sf_plan* plan, double* ms, int max, int* count);
+
 /* Replace the 1D tables (host arrays, q x n row-major) -- e.g. Gridap's own
  * A0 / B0 so the device uses bit-identical tables.
  * Reference: m.mats = [evalMats, gradMats], src/SumFactorizationMaps.jl:57-58. */

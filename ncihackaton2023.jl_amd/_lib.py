@@ -86,6 +86,8 @@ SIGNATURES = [
     ("sf_plan_create", _i, [ctypes.POINTER(_vp), _i, _i, _i, _i64, _i64, _i]),
     ("sf_plan_destroy", _i, [_vp]),
     ("sf_plan_get_info", _i, [_vp, ctypes.POINTER(PlanInfo)]),
+    ("sf_plan_set_timing", _i, [_vp, _i]),
+    ("sf_plan_read_timing", _i, [_vp, _vp, _i, ctypes.POINTER(_i)]),
     ("sf_plan_set_tables", _i, [_vp, _vp, _vp, _vp]),
     ("sf_plan_set_cells", _i, [_vp, _vp, _i, _vp]),
     ("sf_plan_set_vertices", _i, [_vp, _vp]),

@@ -19,6 +19,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "sf_apply.hpp"
 #include "sf_common.hpp"
 #include "sf_kernels.h"
@@ -65,6 +67,15 @@ int fail(int code, const char* fmt, ...) {
   catch (...) {                                                        \
     return fail(SF_ERR_ARG, "unexpected C++ exception");               \
   }
+
+// A named range on the profiler's marker timeline (rocprofv3
+// --marker-trace): the apply, the solves, their chunks of CG iterations,
+// the preconditioner set-up.  Costs a call into the marker library when no
+// profiler listens.
+struct MarkerRange {
+  explicit MarkerRange(const char* name) { roctxRangePushA(name); }
+  ~MarkerRange() { roctxRangePop(); }
+};
 
 // ---------------------------------------------------------------------------
 // 1D tables: Gauss-Legendre on [0, 1] and Lagrange polynomials on the
@@ -302,6 +313,11 @@ struct sf_plan {
   // the caller's order); corig is its device copy (for outputs per cell)
   std::vector<int32_t> perm;
   DevBuf<int32_t> corig;
+  // apply-kernel timing (sf_plan_set_timing): event pairs recorded around
+  // each apply kernel, the first `nrec` in use
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  size_t nrec = 0;
   hipStream_t own_stream = nullptr;
 
   const double* active_G() const { return has_kappa ? Gk.p : G.p; }
@@ -325,6 +341,9 @@ struct sf_plan {
     w_z.release(); w_w.release(); w_z32.release(); w_w32.release();
     if (own_stream) (void)hipStreamDestroy(own_stream);
     own_stream = nullptr;
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    nrec = 0;
   }
 
   int64_t device_bytes() const {
@@ -409,6 +428,27 @@ bool fused_f32(const sf_plan* P) {
   return P->fused_geom && P->dim == 3 && P->X32.p && (!P->has_kappa || P->k32_valid);
 }
 
+// Events around one apply kernel when the plan records timings.
+struct ApplyTimer {
+  sf_plan* P;
+  hipStream_t s;
+  hipEvent_t stop = nullptr;
+  ApplyTimer(sf_plan* P_, hipStream_t s_) : P(P_), s(s_) {
+    if (!P->timing) return;
+    if (P->nrec * 2 + 2 > P->ev.size()) {
+      hipEvent_t a = nullptr, b = nullptr;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+      P->ev.push_back(a);
+      P->ev.push_back(b);
+    }
+    if (hipEventRecord(P->ev[2 * P->nrec], s) != hipSuccess) return;
+    stop = P->ev[2 * P->nrec + 1];
+  }
+  ~ApplyTimer() {
+    if (stop && hipEventRecord(stop, s) == hipSuccess) ++P->nrec;
+  }
+};
+
 // y = A x (FP64) on stream s.  accumulate: y += A x.  zeroed: the caller
 // guarantees y == 0 already (the CG direction update clears it), so the
 // fill is skipped.
@@ -419,6 +459,7 @@ int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
   // the fused-geometry kernel when the plan has vertices (kappa, if set,
   // read per quadrature point)
   const bool geo = fused_f64(P);
+  ApplyTimer timer(P, s);
   hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
                            P->dB.p, P->dD.p, P->ncell, P->blocks(), accumulate, s,
                            geo ? P->X.p : nullptr, P->dPts.p, P->dW.p, dotp,
@@ -507,6 +548,7 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
                  hipStream_t s, bool zeroed = false, double* dotp = nullptr) {
   if (P->nfree == 0) return SF_OK;
   if (!zeroed) SF_HIP(launch_fill<float>(y, P->nfree, 0.0f, s));
+  ApplyTimer timer(P, s);
   hipError_t e;
   if (precision == SF_PREC_F32) {
     const bool geo = fused_f32(P);
@@ -852,6 +894,7 @@ int ensure_pc(sf_plan* P, hipStream_t s, const float** dinv, bool* gll = nullptr
   if (P->pc_kind == SF_PC_GLL && !gll)
     return fail(SF_ERR_UNSUPPORTED, "the Gauss-Lobatto preconditioner is not available here");
   if (!P->pc_valid) {
+    MarkerRange mark("sfhip preconditioner set-up");
     DevBuf<double> d, tb, td;
     SF_HIP(d.alloc(P->nfree));
     SF_HIP(P->dinv.alloc(P->nfree));
@@ -992,6 +1035,7 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   const bool fused = P->npart > 0 && P->w_part.p && fused_f64(P);
   while (!h.done && h.iters < maxit) {
     prog.tick(h.iters, bb > 0 ? sqrt(h.res2 / bb) : sqrt(h.res2));
+    MarkerRange mark("sfhip cg chunk (8 iterations)");
     const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
     for (int k = 0; k < todo; ++k) {
       if (fused) {
@@ -1406,6 +1450,7 @@ namespace {
 // sf_cg and sf_cg_noalloc.
 int cg_solve(sf_plan* P, const double* b, double* x, double rtol, int maxit,
              sf_solve_stats* stats, hipStream_t s) {
+  MarkerRange mark("sfhip cg");
   const float* dinv = nullptr;
   bool gll = false;
   int rc = ensure_pc(P, s, &dinv, &gll);
@@ -1635,6 +1680,33 @@ int sf_plan_get_info(const sf_plan* P, sf_plan_info* info) {
   SF_CATCH
 }
 
+int sf_plan_set_timing(sf_plan* P, int enable) {
+  SF_TRY
+  if (check_plan(P)) return SF_ERR_ARG;
+  P->timing = enable != 0;
+  if (P->timing) P->nrec = 0;  // a new record; stopping keeps what was recorded
+  return SF_OK;
+  SF_CATCH
+}
+
+int sf_plan_read_timing(sf_plan* P, double* ms, int max, int* count) {
+  SF_TRY
+  if (check_plan(P)) return SF_ERR_ARG;
+  if (!count || (max > 0 && !ms)) return fail(SF_ERR_ARG, "null output");
+  DeviceGuard g(P->device);
+  const int n = (int)std::min<size_t>(P->nrec, (size_t)std::max(max, 0));
+  for (int i = 0; i < n; ++i) {
+    SF_HIP(hipEventSynchronize(P->ev[2 * i + 1]));
+    float t = 0.f;
+    SF_HIP(hipEventElapsedTime(&t, P->ev[2 * i], P->ev[2 * i + 1]));
+    ms[i] = (double)t;
+  }
+  *count = (int)P->nrec;
+  P->nrec = 0;
+  return SF_OK;
+  SF_CATCH
+}
+
 int sf_plan_set_tables(sf_plan* P, const double* B, const double* Dm,
                        const double* weights) {
   SF_TRY
@@ -1729,7 +1801,9 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
     apply_shape_f64(3, P->n, P->q, &nt, &eb);
     const int64_t E = P->ncell;
     const int64_t nb_caller = (int64_t)block_starts(h, E, nl, eb).size() - 1;
-    if (eb > 0 && nb_caller > (E + eb - 1) / eb) {
+    // (a row-wise order is within a few per cent of the minimum count: the
+    // connectivity order is only worth building when it is well above)
+    if (eb > 0 && nb_caller > 5 * ((E + eb - 1) / eb) / 4) {
       std::vector<int32_t> ord = connectivity_order(h, E, P->n);
       std::vector<int32_t> hp(h.size());
       for (int64_t i = 0; i < E; ++i)
@@ -1939,6 +2013,7 @@ int sf_plan_prepare(sf_plan* P, int precision) {
 
 int sf_apply(sf_plan* P, double* y, const double* x, void* stream) {
   SF_TRY
+  MarkerRange mark("sfhip apply");
   int rc = check_ready(P);
   if (rc) return rc;
   if ((!x || !y) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
@@ -1980,6 +2055,7 @@ int sf_apply_host(sf_plan* P, double* y, const double* x) {
 int sf_apply_f32(sf_plan* P, float* y, const float* x, int precision,
                  void* stream) {
   SF_TRY
+  MarkerRange mark("sfhip apply (reduced precision)");
   int rc = check_ready(P);
   if (rc) return rc;
   if ((!x || !y) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
@@ -1995,6 +2071,7 @@ int sf_apply_f32(sf_plan* P, float* y, const float* x, int precision,
 int sf_adjoint(sf_plan* P, const double* u, const double* lam,
                double* dJdkappa, double* dJdu, void* stream) {
   SF_TRY
+  MarkerRange mark("sfhip adjoint");
   int rc = check_ready(P);
   if (rc) return rc;
   if ((!u || !lam) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
@@ -2162,6 +2239,7 @@ int sf_cg_mixed(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   if (rc) return rc;
   rc = ensure_solver_ws(P, true);
   if (rc) return rc;
+  MarkerRange mark("sfhip mixed cg");
   const float* dinv = nullptr;
   bool gll = false;
   rc = ensure_pc(P, s, &dinv, &gll);
@@ -2221,7 +2299,9 @@ int sf_cg_ir(sf_plan* P, const double* b, double* x, double rtol,
   const double bnorm = sqrt(bb);
   int inner_total = 0, outer = 0, nonfinite = 0;
   double rel = 0.0;
+  MarkerRange mark("sfhip refinement");
   for (;;) {
+    MarkerRange step("sfhip refinement outer step");
     // FP64 residual r = b - A x
     rc = do_apply_f64(P, P->w_t.p, x, s);
     if (rc) return rc;

@@ -121,6 +121,18 @@ class LazyMatrix:
                                             ctypes.c_void_p(r.data_ptr()), _torch_stream(r)))
         return z
 
+    def timing(self, enable=True):
+        """Record the duration of every apply kernel (HIP events around the
+        kernel alone) from now on; enable=False stops."""
+        _lib.check(self._L.sf_plan_set_timing(self.plan, 1 if enable else 0))
+
+    def read_timing(self, max_count=1 << 16):
+        """The recorded apply-kernel durations in ms (and clear them)."""
+        buf = (ctypes.c_double * max_count)()
+        n = ctypes.c_int(0)
+        _lib.check(self._L.sf_plan_read_timing(self.plan, buf, max_count, ctypes.byref(n)))
+        return [buf[i] for i in range(min(n.value, max_count))]
+
     def prepare(self, precision):
         _lib.check(self._L.sf_plan_prepare(self.plan, _lib.PRECISIONS.get(precision, precision)))
 
