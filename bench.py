@@ -330,12 +330,12 @@ def mode_sweep(args, torch, pkg):
                "f64_roofline_frac": max(t_hbm, t_flop) / kern,
                "f64_bound": "fp64" if t_flop >= t_hbm else "hbm",
                "f64_tflops": F / kern / 1e12, "f64_GBs": B64 / kern / 1e9}
-        for prec in ("f32", "f32_gbf16", "mfma_bf16x3", "mfma_bf16"):
+        for prec in ("f32", "f32_gbf16", "mfma_f32", "mfma_bf16x3", "mfma_bf16"):
             x32 = x.float()
             y32 = torch.empty_like(x32)
             A.prepare(prec)
             w32, k32 = time_apply(torch, A, x32, y32, args.steps, args.warmup, reduced=prec)
-            sg = 4 if prec == "f32" else 2
+            sg = 4 if prec in ("f32", "mfma_f32") else 2
             B32 = alg_bytes(space.n_free, space.n_cells, n, q, 3, s_vec=4, s_g=sg)
             row[prec + "_gdofs"] = space.n_free / w32 / 1e9
             row[prec + "_kernel_ms"] = k32 * 1e3
@@ -559,13 +559,17 @@ def mode_q9(args, torch, pkg):
     # tensor-contraction FMAs of the MFMA schedule (useful work, no padding)
     fma_cell = 2 * (2 * n ** 3 * q + 3 * n ** 2 * q ** 2 + 3 * n * q ** 3)
     x32 = x.float()
-    for prec in ("f32", "f32_gbf16", "mfma_bf16x3", "mfma_bf16"):
+    for prec in ("f32", "f32_gbf16", "mfma_f32", "mfma_bf16x3", "mfma_bf16"):
         A.prepare(prec)
         y32 = torch.empty_like(x32)
         w32, k32 = time_apply(torch, A, x32, y32, args.steps, args.warmup, reduced=prec)
         err = float(torch.linalg.norm(y32.double() - y64) / torch.linalg.norm(y64))
         r = {"gdofs": nf / w32 / 1e9, "kernel_ms": k32 * 1e3, "rel_l2_vs_f64": err}
-        if prec.startswith("mfma"):
+        if prec == "mfma_f32":
+            useful = 2.0 * fma_cell * E
+            r["useful_tflops"] = useful / k32 / 1e12
+            r["mfma_util_vs_fp32_matrix_peak"] = useful / k32 / 157.3e12
+        elif prec.startswith("mfma"):
             terms = 6 if prec.endswith("x3") else 1
             useful = 2.0 * fma_cell * E
             r["useful_tflops"] = useful / k32 / 1e12

@@ -56,8 +56,10 @@ enum sf_precision {
                             to bf16), FP32 accumulation, G in bf16               */
   SF_PREC_MFMA_BF16X3 = 4, /* 3D: as MFMA_BF16 with every operand split into three
                             bf16 terms (six MFMAs per tile): FP32-level accuracy */
-  SF_PREC_MFMA_BF16X2 = 5  /* 3D: two bf16 terms per operand (three MFMAs per
+  SF_PREC_MFMA_BF16X2 = 5, /* 3D: two bf16 terms per operand (three MFMAs per
                             tile): about 16 significant bits                     */
+  SF_PREC_MFMA_F32 = 6     /* 3D: FP32 operands on the matrix cores (four
+                            v_mfma_f32_16x16x4f32 per tile), G in FP32           */
 };
 
 /* Preconditioner of the device-resident solvers (sf_cg, sf_cg_mixed,
@@ -226,8 +228,9 @@ int sf_apply_add(sf_plan* plan, double* y, const double* x, void* stream);
 int sf_apply_host(sf_plan* plan, double* y, const double* x);
 
 /* y = A x with FP32 vectors (device pointers).  precision is SF_PREC_F32,
- * SF_PREC_F32_GBF16, SF_PREC_MFMA_BF16 or SF_PREC_MFMA_BF16X3 (the MFMA
- * forms are 3D only: SF_ERR_UNSUPPORTED in 2D).  No reference counterpart:
+ * SF_PREC_F32_GBF16, SF_PREC_MFMA_BF16, SF_PREC_MFMA_BF16X2,
+ * SF_PREC_MFMA_BF16X3 or SF_PREC_MFMA_F32 (the MFMA forms are 3D only:
+ * SF_ERR_UNSUPPORTED in 2D).  No reference counterpart:
  * the reduced-precision inner operator of the mixed-precision solver. */
 int sf_apply_f32(sf_plan* plan, float* y, const float* x, int precision,
                  void* stream);

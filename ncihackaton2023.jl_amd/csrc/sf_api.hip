@@ -277,6 +277,8 @@ struct sf_plan {
   DevBuf<float> G32;
   DevBuf<bf16s> Gbf;
   DevBuf<uint32_t> Gm;  // bf16 pairs in the layout of the MFMA apply (3D)
+  DevBuf<float> Gm32;   // FP32 G in the layout of the FP32-operand MFMA apply
+  bool gm32_valid = false;
   bool has_cells = false, has_geom = false, has_kappa = false;
   bool g32_valid = false, gbf_valid = false, gm_valid = false;
   DevBuf<int> flag;
@@ -329,7 +331,7 @@ struct sf_plan {
     cells.release(); G.release(); Gk.release(); kappa.release(); kappa32.release();
     X.release();
     X32.release(); Pts32.release(); W32.release();
-    G32.release(); Gbf.release(); Gm.release(); flag.release();
+    G32.release(); Gbf.release(); Gm.release(); Gm32.release(); flag.release();
     pidx.release(); bdofs.release(); bexcl.release(); cell_block.release();
     boff.release(); bcell.release();
     xs.release(); ys.release();
@@ -350,7 +352,7 @@ struct sf_plan {
     return dB.bytes() + dD.bytes() + dW.bytes() + dPts.bytes() + dB32.bytes() +
            dD32.bytes() + cells.bytes() + G.bytes() + Gk.bytes() + X.bytes() +
            X32.bytes() + Pts32.bytes() + W32.bytes() +
-           kappa.bytes() + kappa32.bytes() + G32.bytes() + Gbf.bytes() + Gm.bytes() + flag.bytes() +
+           kappa.bytes() + kappa32.bytes() + G32.bytes() + Gbf.bytes() + Gm.bytes() + Gm32.bytes() + flag.bytes() +
            pidx.bytes() + bdofs.bytes() + bexcl.bytes() + cell_block.bytes() +
            boff.bytes() + bcell.bytes() +
            xs.bytes() + ys.bytes() + w_r.bytes() + w_p.bytes() + w_q.bytes() +
@@ -473,12 +475,24 @@ int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
 bool is_reduced(int precision) {
   return precision == SF_PREC_F32 || precision == SF_PREC_F32_GBF16 ||
          precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X2 ||
-         precision == SF_PREC_MFMA_BF16X3;
+         precision == SF_PREC_MFMA_BF16X3 || precision == SF_PREC_MFMA_F32;
 }
 
 int prepare(sf_plan* P, int precision) {
   const int64_t ng = P->ncell * P->ng * P->nqc;
   if (precision == SF_PREC_F64) return SF_OK;
+  if (precision == SF_PREC_MFMA_F32) {
+    if (P->dim != 3)
+      return fail(SF_ERR_UNSUPPORTED, "the MFMA apply is 3D only (dim=%d)", P->dim);
+    if (!P->gm32_valid) {
+      SF_HIP(P->Gm32.alloc(P->ncell * P->nqc * 6));
+      SF_HIP(launch_g3_to_mfma32(P->active_G(), P->Gm32.p, P->q, P->ncell, P->cell_block.p,
+                                 P->bcell.p, nullptr));
+      SF_HIP(hipDeviceSynchronize());
+      P->gm32_valid = true;
+    }
+    return SF_OK;
+  }
   if (precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X2 ||
       precision == SF_PREC_MFMA_BF16X3) {
     if (P->dim != 3)
@@ -559,6 +573,8 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
   } else if (precision == SF_PREC_F32_GBF16) {
     e = apply_f32_gbf16(P->dim, P->n, P->q, x, y, P->cells.p, P->Gbf.p,
                         P->dB32.p, P->dD32.p, P->ncell, P->blocks(), 0, s);
+  } else if (precision == SF_PREC_MFMA_F32) {
+    e = apply_mfma(P->n, P->q, 0, x, y, P->Gm32.p, P->dB32.p, P->dD32.p, P->blocks(), s);
   } else if (precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X2 ||
              precision == SF_PREC_MFMA_BF16X3) {
     const int split = precision == SF_PREC_MFMA_BF16 ? 1 : (precision == SF_PREC_MFMA_BF16X2 ? 2 : 3);
@@ -1828,7 +1844,7 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
     P->fused_geom = false;
     P->X.release();
     P->X32.release();
-    P->g32_valid = P->gbf_valid = P->gm_valid = false;
+    P->g32_valid = P->gbf_valid = P->gm_valid = P->gm32_valid = false;
     int rc = build_blocks(P, h);
     if (rc) return rc;
   }
@@ -1838,7 +1854,7 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
 }
 
 static int finish_geometry(sf_plan* P) {
-  P->g32_valid = P->gbf_valid = P->gm_valid = false;
+  P->g32_valid = P->gbf_valid = P->gm_valid = P->gm32_valid = false;
   P->pc_valid = false;
   if (P->has_kappa) {
     SF_HIP(P->Gk.alloc(P->ncell * P->ng * P->nqc));
@@ -1997,7 +2013,7 @@ int sf_plan_set_kappa(sf_plan* P, const double* kappa) {
   P->k32_valid = false;
   P->pc_valid = false;
   if (P->has_geom) return finish_geometry(P);
-  P->g32_valid = P->gbf_valid = P->gm_valid = false;
+  P->g32_valid = P->gbf_valid = P->gm_valid = P->gm32_valid = false;
   return SF_OK;
   SF_CATCH
 }

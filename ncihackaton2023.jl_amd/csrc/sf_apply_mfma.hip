@@ -21,7 +21,15 @@
 // terms (hi + mid + lo, each the rounded remainder of the previous) and
 // keeps the six partial products of order >= 2^-24, so the contraction is
 // as accurate as FP32 arithmetic at six MFMAs per tile.  SPLIT = 2 (hi +
-// lo, three products) carries about 16 bits.
+// lo, three products) carries about 16 bits.  SPLIT = 0 keeps the operands
+// in FP32 and runs each tile as four v_mfma_f32_16x16x4f32 (no operand
+// conversion at all); G is then read in FP32 too.
+//
+// MFMA 16x16x4 f32 lane layout: A[m][k]: m = l % 16, k = g; B[k][n]:
+// n = l % 16, k = g; C as above.  Instruction j of a tile takes value j of
+// every lane, i.e. the contraction indices k = 4 g + j -- the same four
+// values per lane as one 16x16x16 bf16 instruction, so both forms share
+// every load and store below.
 //
 // MFMA 16x16x16 bf16 lane layout (wave64, lane l, g = l / 16, j = 0..3):
 //   A[m][k]: m = l % 16, k = 4 g + j      B[k][n]: n = l % 16, k = 4 g + j
@@ -43,6 +51,10 @@ template <int SPLIT>
 struct Op {
   s4 t[SPLIT];
 };
+template <>
+struct Op<0> {  // FP32 operands
+  f4 v;
+};
 
 // Two FP32 values -> one packed bf16 pair (v_cvt_pk_bf16_f32, round to
 // nearest).  The empty asm keeps the pair opaque, so the remainders below
@@ -60,6 +72,10 @@ __device__ __forceinline__ unsigned pk_bf16(float a, float b) {
 template <int SPLIT>
 __device__ __forceinline__ Op<SPLIT> split4(f4 v) {
   Op<SPLIT> o;
+  if constexpr (SPLIT == 0) {
+    o.v = v;
+    return o;
+  } else {
 #pragma unroll
   for (int s = 0; s < SPLIT; ++s) {
     const unsigned h0 = pk_bf16(v.x, v.y), h1 = pk_bf16(v.z, v.w);
@@ -73,6 +89,7 @@ __device__ __forceinline__ Op<SPLIT> split4(f4 v) {
     }
   }
   return o;
+  }
 }
 
 // Branch-free masked accesses.  A read is always issued (an out-of-range
@@ -97,16 +114,22 @@ __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) {
 // c += a . b with split operands, small terms first.
 template <int SPLIT>
 __device__ __forceinline__ f4 mma(const Op<SPLIT>& a, const Op<SPLIT>& b, f4 c) {
-  if constexpr (SPLIT >= 3) {
-    c = mfma(a.t[2], b.t[0], c);
-    c = mfma(a.t[1], b.t[1], c);
-    c = mfma(a.t[0], b.t[2], c);
+  if constexpr (SPLIT == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.v[j], b.v[j], c, 0, 0, 0);
+    return c;
+  } else {
+    if constexpr (SPLIT >= 3) {
+      c = mfma(a.t[2], b.t[0], c);
+      c = mfma(a.t[1], b.t[1], c);
+      c = mfma(a.t[0], b.t[2], c);
+    }
+    if constexpr (SPLIT >= 2) {
+      c = mfma(a.t[1], b.t[0], c);
+      c = mfma(a.t[0], b.t[1], c);
+    }
+    return mfma(a.t[0], b.t[0], c);
   }
-  if constexpr (SPLIT >= 2) {
-    c = mfma(a.t[1], b.t[0], c);
-    c = mfma(a.t[0], b.t[1], c);
-  }
-  return mfma(a.t[0], b.t[0], c);
 }
 
 template <int N, int Q, int SPLIT>
@@ -125,7 +148,8 @@ struct MfmaShape {
   static constexpr int RX = EB * NNN;   // the block's x values, later its y
   static constexpr int LDS_FLOATS = RX + EB * (2 * S1 + 3 * S2) + 64;  // + dummies
   static constexpr int KU = (EB * NNN + NT - 1) / NT;
-  static constexpr bool GPRE = UZ <= 4;  // G of every z tile loaded up front
+  // G of every z tile loaded up front (bf16 pairs: 12 registers a tile)
+  static constexpr bool GPRE = SPLIT > 0 && UZ <= 4;
   static_assert(N <= 16 && Q <= 16, "one MFMA tile per contraction");
   static_assert(LDS_FLOATS * 4 <= 65536, "apply3d_mfma: LDS above 64 KiB");
 };
@@ -151,6 +175,25 @@ __device__ __forceinline__ void load_gz(const uint32_t* __restrict__ Gm,
   }
 }
 
+// FP32 G of one Z tile (SPLIT = 0): the six entries of point (i3, col) at
+// cell * 6Q^3 + (i3 * 6 + k) * Q^2 + col.
+template <int N, int Q>
+__device__ __forceinline__ void load_gz32(const float* __restrict__ Gf, int64_t cell, int col,
+                                          int g, bool ok, float gv[24]) {
+  constexpr int QQ = Q * Q;
+  const float* p = Gf + (ok ? cell * 6 * QQ * Q : 0);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int i3 = 4 * g + j;
+    const bool v = ok && i3 < Q && col < QQ;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const float w = __builtin_nontemporal_load(p + (v ? (i3 * 6 + k) * QQ + col : 0));
+      gv[j * 6 + k] = v ? w : 0.f;
+    }
+  }
+}
+
 template <int N, int Q, int SPLIT>
 __global__ void __launch_bounds__((MfmaShape<N, Q, SPLIT>::NT))
     k_apply3d_mfma(const float* __restrict__ x, float* __restrict__ y,
@@ -159,9 +202,11 @@ __global__ void __launch_bounds__((MfmaShape<N, Q, SPLIT>::NT))
                    const int64_t* __restrict__ boff,
                    const int32_t* __restrict__ bexcl,
                    const int64_t* __restrict__ bcell, int nblk,
-                   const uint32_t* __restrict__ Gm,
+                   const void* __restrict__ Gv,
                    const float* __restrict__ tB, const float* __restrict__ tD) {
   using M = MfmaShape<N, Q, SPLIT>;
+  const uint32_t* __restrict__ Gm = (const uint32_t*)Gv;  // bf16 pairs (SPLIT > 0)
+  const float* __restrict__ Gf = (const float*)Gv;        // FP32 (SPLIT = 0)
   constexpr int NN = M::NN, NQ = M::NQ, QQ = M::QQ, NNN = M::NNN;
   constexpr int S1 = M::S1, S2 = M::S2, W = M::W;
   __shared__ float smem[M::LDS_FLOATS];
@@ -324,13 +369,6 @@ __global__ void __launch_bounds__((MfmaShape<N, Q, SPLIT>::NT))
     if (e >= ncb) break;
     const int col = (u - e * M::TZ) * 16 + cl;  // i1 + Q i2
     float* t = T2 + e * 3 * S2;
-    uint32_t gl[12];
-    if constexpr (M::GPRE) {
-#pragma unroll
-      for (int k = 0; k < 12; ++k) gl[k] = gv[i][k];
-    } else {
-      load_gz<N, Q>(Gm, base + e, col, g, true, gl);
-    }
     f4 v0, v1, v2;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -344,18 +382,37 @@ __global__ void __launch_bounds__((MfmaShape<N, Q, SPLIT>::NT))
     const f4 gy = mma(aB, split4<SPLIT>(v1), zero);
     const f4 gz = mma(aD, split4<SPLIT>(v2), zero);
     f4 h0, h1, h2;
+    if constexpr (SPLIT == 0) {
+      float gf[24];
+      load_gz32<N, Q>(Gf, base + e, col, g, true, gf);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      // entries xx, xy | xz, yy | yz, zz (zero outside the Q^3 points)
-      const float g0 = __uint_as_float(gl[3 * j] << 16);
-      const float g1 = __uint_as_float(gl[3 * j] & 0xffff0000u);
-      const float g2 = __uint_as_float(gl[3 * j + 1] << 16);
-      const float g3 = __uint_as_float(gl[3 * j + 1] & 0xffff0000u);
-      const float g4 = __uint_as_float(gl[3 * j + 2] << 16);
-      const float g5 = __uint_as_float(gl[3 * j + 2] & 0xffff0000u);
-      h0[j] = g0 * gx[j] + g1 * gy[j] + g2 * gz[j];
-      h1[j] = g1 * gx[j] + g3 * gy[j] + g4 * gz[j];
-      h2[j] = g2 * gx[j] + g4 * gy[j] + g5 * gz[j];
+      for (int j = 0; j < 4; ++j) {
+        const float* q6 = gf + 6 * j;  // xx xy xz yy yz zz
+        h0[j] = q6[0] * gx[j] + q6[1] * gy[j] + q6[2] * gz[j];
+        h1[j] = q6[1] * gx[j] + q6[3] * gy[j] + q6[4] * gz[j];
+        h2[j] = q6[2] * gx[j] + q6[4] * gy[j] + q6[5] * gz[j];
+      }
+    } else {
+      uint32_t gl[12];
+      if constexpr (M::GPRE) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) gl[k] = gv[i][k];
+      } else {
+        load_gz<N, Q>(Gm, base + e, col, g, true, gl);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        // entries xx, xy | xz, yy | yz, zz (zero outside the Q^3 points)
+        const float g0 = __uint_as_float(gl[3 * j] << 16);
+        const float g1 = __uint_as_float(gl[3 * j] & 0xffff0000u);
+        const float g2 = __uint_as_float(gl[3 * j + 1] << 16);
+        const float g3 = __uint_as_float(gl[3 * j + 1] & 0xffff0000u);
+        const float g4 = __uint_as_float(gl[3 * j + 2] << 16);
+        const float g5 = __uint_as_float(gl[3 * j + 2] & 0xffff0000u);
+        h0[j] = g0 * gx[j] + g1 * gy[j] + g2 * gz[j];
+        h1[j] = g1 * gx[j] + g3 * gy[j] + g4 * gz[j];
+        h2[j] = g2 * gx[j] + g4 * gy[j] + g5 * gz[j];
+      }
     }
     const f4 ax = mma(aBt, split4<SPLIT>(h0), zero);
     const f4 ay = mma(aBt, split4<SPLIT>(h1), zero);
@@ -445,7 +502,7 @@ __global__ void __launch_bounds__((MfmaShape<N, Q, SPLIT>::NT))
 }
 
 template <int N, int Q, int SPLIT>
-hipError_t launch_mfma_nq(const float* x, float* y, const uint32_t* Gm,
+hipError_t launch_mfma_nq(const float* x, float* y, const void* Gm,
                           const float* tB, const float* tD, const BlockArgs& k,
                           hipStream_t s) {
   if (k.nblk <= 0) return hipSuccess;
@@ -454,6 +511,24 @@ hipError_t launch_mfma_nq(const float* x, float* y, const uint32_t* Gm,
                      x, y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell, k.nblk,
                      Gm, tB, tD);
   return hipGetLastError();
+}
+
+// G (FP64, block-interleaved) -> FP32 in the layout of load_gz32.
+__global__ void k_g3_to_mfma32(const double* __restrict__ G, float* __restrict__ out, int q,
+                               int64_t ncell, const int32_t* __restrict__ cell_block,
+                               const int64_t* __restrict__ bcell) {
+  const int64_t QQ = (int64_t)q * q, QQQ = QQ * q;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncell * QQQ) return;
+  const int64_t cell = t / QQQ;
+  const int qp = (int)(t - cell * QQQ);
+  const int i3 = qp / (int)QQ, col = qp - i3 * (int)QQ;
+  const int blk = cell_block[cell];
+  const int64_t b0 = bcell[blk];
+  const int ncb = (int)(bcell[blk + 1] - b0);
+  float* o = out + cell * 6 * QQQ;
+  for (int k = 0; k < 6; ++k)
+    o[(i3 * 6 + k) * QQ + col] = (float)G[g3_index(b0, ncb, cell, k, i3, col, q)];
 }
 
 // G (FP64, block-interleaved) -> bf16 pairs in the MFMA layout of load_gz.
@@ -481,14 +556,15 @@ __global__ void k_g3_to_mfma(const double* __restrict__ G, uint32_t* __restrict_
 }  // namespace
 
 hipError_t apply_mfma(int n, int q, int split, const float* x, float* y,
-                      const uint32_t* Gm, const float* tB, const float* tD,
+                      const void* Gm, const float* tB, const float* tD,
                       const BlockArgs* k, hipStream_t s) {
   if (!k) return hipErrorInvalidValue;
-  if (split < 1 || split > 3) return hipErrorNotSupported;
+  if (split < 0 || split > 3) return hipErrorNotSupported;
   switch (n * 16 + q) {
 #define SF_CASE(N_, Q_)                                                      \
   case N_ * 16 + Q_:                                                         \
-    return split == 1   ? launch_mfma_nq<N_, Q_, 1>(x, y, Gm, tB, tD, *k, s) \
+    return split == 0   ? launch_mfma_nq<N_, Q_, 0>(x, y, Gm, tB, tD, *k, s) \
+           : split == 1 ? launch_mfma_nq<N_, Q_, 1>(x, y, Gm, tB, tD, *k, s) \
            : split == 2 ? launch_mfma_nq<N_, Q_, 2>(x, y, Gm, tB, tD, *k, s) \
                         : launch_mfma_nq<N_, Q_, 3>(x, y, Gm, tB, tD, *k, s);
     SF_FOR_EACH_NQ(SF_CASE)
@@ -496,6 +572,18 @@ hipError_t apply_mfma(int n, int q, int split, const float* x, float* y,
     default:
       return hipErrorNotSupported;
   }
+}
+
+hipError_t launch_g3_to_mfma32(const double* G, float* out, int q, int64_t ncell,
+                               const int32_t* cell_block, const int64_t* bcell,
+                               hipStream_t s) {
+  const int64_t n = ncell * (int64_t)q * q * q;
+  if (n <= 0) return hipSuccess;
+  const int64_t nb = (n + 255) / 256;
+  if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_g3_to_mfma32, dim3((unsigned)nb), dim3(256), 0, s, G, out, q, ncell,
+                     cell_block, bcell);
+  return hipGetLastError();
 }
 
 hipError_t launch_g3_to_mfma(const double* G, uint32_t* out, int q, int64_t ncell,

@@ -66,10 +66,15 @@ hipError_t apply_f32_gbf16(int dim, int n, int q, const float* x, float* y,
                            const BlockArgs* k, int accumulate, hipStream_t s);
 void apply_shape_f64(int dim, int n, int q, int* nt, int* eb);
 // 3D, matrix cores: y = A x over the block structure k (y zeroed by the
-// caller); split = 1 (bf16 operands) or 3 (three-term bf16 split).
+// caller); split = 1 / 2 / 3 (bf16 operands split in that many terms, G as
+// bf16 pairs) or 0 (FP32 operands and G, v_mfma_f32_16x16x4f32).
 hipError_t apply_mfma(int n, int q, int split, const float* x, float* y,
-                      const uint32_t* Gm, const float* tB, const float* tD,
+                      const void* Gm, const float* tB, const float* tD,
                       const BlockArgs* k, hipStream_t s);
+// G (FP64, block-interleaved) -> FP32 in the layout of the split = 0 form
+hipError_t launch_g3_to_mfma32(const double* G, float* out, int q, int64_t ncell,
+                               const int32_t* cell_block, const int64_t* bcell,
+                               hipStream_t s);
 // G (FP64, block-interleaved) -> the bf16-pair layout of apply_mfma
 hipError_t launch_g3_to_mfma(const double* G, uint32_t* out, int q,
                              int64_t ncell, const int32_t* cell_block,

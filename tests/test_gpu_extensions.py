@@ -38,7 +38,7 @@ def test_reduced_precision_apply_error(pkg, oracle_mod, cuda, D, m, p, q):
     errs = {}
     cases = [("f32", 1e-5), ("f32_gbf16", 2e-2)]
     if D == 3:
-        cases += [("mfma_bf16x3", 2e-2), ("mfma_bf16", 5e-2)]
+        cases += [("mfma_f32", 1e-5), ("mfma_bf16x3", 2e-2), ("mfma_bf16", 5e-2)]
     for prec, tol in cases:
         A.mul_reduced(y32, x32, prec)
         cuda.cuda.synchronize()
@@ -64,7 +64,9 @@ def test_mfma_every_instantiated_kernel(pkg, cuda):
     """Every (n, q) of the MFMA apply against the FP32 vector-ALU apply that
     reads the same bf16 G: with three-term operands the two differ by FP32
     rounding only, so a wrong lane layout, table transpose or tile bound
-    shows at once.  With one-term operands the gap is bf16 rounding."""
+    shows at once.  With one-term operands the gap is bf16 rounding.  The
+    FP32-operand form (FP32 G) against the FP32 vector-ALU apply: FP32
+    rounding."""
     worst3 = worst1 = 0.0
     for n, q in NQ:
         p = n - 1
@@ -73,11 +75,14 @@ def test_mfma_every_instantiated_kernel(pkg, cuda):
         model, space, mp, A = make(pkg, 3, m, p, q)
         x32 = cuda.from_numpy(pkg.uniform(0, space.n_free, -1, 1).astype(np.float32)).cuda()
         ys = {}
-        for prec in ("f32_gbf16", "mfma_bf16x3", "mfma_bf16x2", "mfma_bf16"):
+        for prec in ("f32", "mfma_f32", "f32_gbf16", "mfma_bf16x3", "mfma_bf16x2", "mfma_bf16"):
             y = cuda.empty_like(x32)
             A.mul_reduced(y, x32, prec)
             cuda.cuda.synchronize()
             ys[prec] = y.cpu().numpy().astype(np.float64)
+        # FP32 operands on the matrix cores against the FP32 vector-ALU apply
+        ef = rel_l2(ys["mfma_f32"], ys["f32"])
+        assert ef < 1e-5, (p, q, ef)
         e3 = rel_l2(ys["mfma_bf16x3"], ys["f32_gbf16"])
         e2 = rel_l2(ys["mfma_bf16x2"], ys["f32_gbf16"])
         e1 = rel_l2(ys["mfma_bf16"], ys["f32_gbf16"])
