@@ -215,6 +215,37 @@ std::vector<double> gll_to_equispaced(int p) {
   return T;
 }
 
+// Inverse of a small dense matrix (row-major n x n) by Gauss-Jordan with
+// partial pivoting.
+std::vector<double> invert_small(std::vector<double> A, int n) {
+  std::vector<double> I(n * n, 0.0);
+  for (int i = 0; i < n; ++i) I[i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (fabs(A[r * n + c]) > fabs(A[piv * n + c])) piv = r;
+    for (int k = 0; k < n; ++k) {
+      std::swap(A[c * n + k], A[piv * n + k]);
+      std::swap(I[c * n + k], I[piv * n + k]);
+    }
+    const double d = 1.0 / A[c * n + c];
+    for (int k = 0; k < n; ++k) {
+      A[c * n + k] *= d;
+      I[c * n + k] *= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = A[r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        A[r * n + k] -= f * A[c * n + k];
+        I[r * n + k] -= f * I[c * n + k];
+      }
+    }
+  }
+  return I;
+}
+
 // ---------------------------------------------------------------------------
 // device buffer helper
 
@@ -306,10 +337,12 @@ struct sf_plan {
   DevBuf<float> dinv;  // Jacobi: 1 / diag(A); GLL: 1 / diag(T^T A T)
   // GLL preconditioner: the n x n table and its transpose, 1 / (cells per
   // dof), and the z / scratch vectors of the solvers
-  DevBuf<double> dT1, dT1t;
+  DevBuf<double> dT1, dT1t, dT1i;  // T1, its transpose, its inverse (n x n)
+  DevBuf<double> dBg, dDg;         // B T1, D T1 (+ transposes): the GLL basis
+  DevBuf<float> dBg32, dDg32;      //   at the quadrature points, FP64 / FP32
   DevBuf<float> minv;
-  DevBuf<double> w_z, w_w;
-  DevBuf<float> w_z32, w_w32;
+  DevBuf<double> w_w;              // scratch of sf_precond_apply
+  DevBuf<double> g_b, g_y;         // a solve in the GLL basis: T^T b and y
   int max_nu = 0;  // most distinct dofs of one block
   // 3D cell order: the plan's cell i is the caller's cell perm[i] (empty:
   // the caller's order); corig is its device copy (for outputs per cell)
@@ -339,8 +372,9 @@ struct sf_plan {
     w_r32.release(); w_s32.release(); w_d32.release(); w_p32.release(); w_q32.release();
     red_part.release(); w_part.release(); red_counter.release(); scal.release();
     dscal.release(); dinv.release();
-    dT1.release(); dT1t.release(); minv.release(); corig.release();
-    w_z.release(); w_w.release(); w_z32.release(); w_w32.release();
+    dT1.release(); dT1t.release(); dT1i.release(); minv.release(); corig.release();
+    dBg.release(); dDg.release(); dBg32.release(); dDg32.release();
+    w_w.release(); g_b.release(); g_y.release();
     if (own_stream) (void)hipStreamDestroy(own_stream);
     own_stream = nullptr;
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -359,8 +393,8 @@ struct sf_plan {
            w_t.bytes() + w_r32.bytes() + w_s32.bytes() + w_d32.bytes() + w_p32.bytes() +
            w_q32.bytes() + red_part.bytes() + w_part.bytes() + red_counter.bytes() +
            scal.bytes() + dscal.bytes() + dinv.bytes() + dT1.bytes() + dT1t.bytes() +
-           minv.bytes() + w_z.bytes() + w_w.bytes() + w_z32.bytes() + w_w32.bytes() +
-           corig.bytes();
+           dT1i.bytes() + dBg.bytes() + dDg.bytes() + dBg32.bytes() + dDg32.bytes() +
+           minv.bytes() + w_w.bytes() + g_b.bytes() + g_y.bytes() + corig.bytes();
   }
 };
 
@@ -864,12 +898,8 @@ int ensure_solver_ws(sf_plan* P, bool mixed) {
   SF_HIP(P->w_q.alloc(P->nfree));
   SF_HIP(P->w_t.alloc(P->nfree));
   if (P->pc_kind == SF_PC_GLL) {
-    SF_HIP(P->w_z.alloc(P->nfree));
-    SF_HIP(P->w_w.alloc(P->nfree));
-    if (mixed) {
-      SF_HIP(P->w_z32.alloc(P->nfree));
-      SF_HIP(P->w_w32.alloc(P->nfree));
-    }
+    SF_HIP(P->g_b.alloc(P->nfree));
+    SF_HIP(P->g_y.alloc(P->nfree));
   }
   if (mixed) {
     SF_HIP(P->w_r32.alloc(P->nfree));
@@ -901,8 +931,9 @@ int compute_diagonal(sf_plan* P, double* diag, hipStream_t s) {
 }
 
 // The solvers' preconditioner, rebuilt after the operator changed.  Jacobi:
-// *dinv = the FP32 inverse diagonal; GLL: *gll = true (the passes read the
-// plan's tables); none: both unset.
+// *dinv = the FP32 inverse diagonal of A; GLL: *gll = true and *dinv = that
+// of T^T A T (the solve runs in the Gauss-Lobatto basis, solve_gll); none:
+// both unset.
 int ensure_pc(sf_plan* P, hipStream_t s, const float** dinv, bool* gll = nullptr) {
   *dinv = nullptr;
   if (gll) *gll = false;
@@ -932,15 +963,44 @@ int ensure_pc(sf_plan* P, hipStream_t s, const float** dinv, bool* gll = nullptr
           bg[i * n + b] = vb;
           dg[i * n + b] = vd;
         }
+      // T1^{-1} has unit vertex rows as well (the forward pass relies on it)
+      std::vector<double> Ti = invert_small(T, n);
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < n; ++b) Ti[a * n + b] = (a == b) ? 1.0 : 0.0;
       SF_HIP(P->dT1.alloc(n * n));
       SF_HIP(P->dT1t.alloc(n * n));
+      SF_HIP(P->dT1i.alloc(n * n));
       SF_HIP(tb.alloc(q * n));
       SF_HIP(td.alloc(q * n));
       SF_HIP(P->minv.alloc(P->nfree));
       SF_HIP(hipMemcpy(P->dT1.p, T.data(), n * n * 8, hipMemcpyHostToDevice));
       SF_HIP(hipMemcpy(P->dT1t.p, Tt.data(), n * n * 8, hipMemcpyHostToDevice));
+      SF_HIP(hipMemcpy(P->dT1i.p, Ti.data(), n * n * 8, hipMemcpyHostToDevice));
       SF_HIP(hipMemcpy(tb.p, bg.data(), q * n * 8, hipMemcpyHostToDevice));
       SF_HIP(hipMemcpy(td.p, dg.data(), q * n * 8, hipMemcpyHostToDevice));
+      // the GLL tables in the apply's layout (q x n table, then its transpose)
+      {
+        const int64_t tn = (int64_t)q * n;
+        std::vector<double> b64(2 * tn), d64(2 * tn);
+        std::vector<float> b32(2 * tn), d32(2 * tn);
+        for (int i = 0; i < q; ++i)
+          for (int j = 0; j < n; ++j) {
+            b64[i * n + j] = b64[tn + j * q + i] = bg[i * n + j];
+            d64[i * n + j] = d64[tn + j * q + i] = dg[i * n + j];
+          }
+        for (int64_t i = 0; i < 2 * tn; ++i) {
+          b32[i] = (float)b64[i];
+          d32[i] = (float)d64[i];
+        }
+        SF_HIP(P->dBg.alloc(2 * tn));
+        SF_HIP(P->dDg.alloc(2 * tn));
+        SF_HIP(P->dBg32.alloc(2 * tn));
+        SF_HIP(P->dDg32.alloc(2 * tn));
+        SF_HIP(hipMemcpy(P->dBg.p, b64.data(), 2 * tn * 8, hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(P->dDg.p, d64.data(), 2 * tn * 8, hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(P->dBg32.p, b32.data(), 2 * tn * 4, hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(P->dDg32.p, d32.data(), 2 * tn * 4, hipMemcpyHostToDevice));
+      }
       SF_HIP(hipMemsetAsync(P->minv.p, 0, P->nfree * sizeof(float), s));
       SF_HIP(launch_mult_inverse(P->cells.p, P->ncell * P->nloc, P->minv.p, P->nfree, s));
       SF_HIP(launch_fill<double>(d.p, P->nfree, 0.0, s));
@@ -964,10 +1024,7 @@ int ensure_pc(sf_plan* P, hipStream_t s, const float** dinv, bool* gll = nullptr
     if (bad) return fail(SF_ERR_NONFINITE, "preconditioner diagonal not positive and finite");
     P->pc_valid = true;
   }
-  if (P->pc_kind == SF_PC_GLL) {
-    *gll = true;
-    return SF_OK;
-  }
+  if (gll) *gll = P->pc_kind == SF_PC_GLL;
   *dinv = P->dinv.p;
   return SF_OK;
 }
@@ -983,6 +1040,33 @@ int pc_gll(sf_plan* P, const V* r, V* z, V* w, const CGScalars* sc, hipStream_t 
   SF_HIP(e);
   return SF_OK;
 }
+
+// One Gauss-Lobatto pass over the free-dof vectors (launch_gll_pass).
+int gll_pass(sf_plan* P, bool trans, const double* in, const float* scale, double* out,
+             const double* M, hipStream_t s) {
+  hipError_t e = launch_gll_pass(P->n, trans, in, scale, out, M, P->nfree, P->kargs, P->eb,
+                                 P->max_nu, s);
+  if (e == hipErrorNotSupported)
+    return fail(SF_ERR_UNSUPPORTED, "no Gauss-Lobatto pass for n=%d "
+                "(or its blocks exceed the workgroup's LDS)", P->n);
+  SF_HIP(e);
+  return SF_OK;
+}
+
+// The apply's 1D tables exchanged with the Gauss-Lobatto basis's for one
+// scope: meanwhile every apply (FP64 and reduced precision) computes
+// T^T A T.
+struct GllTables {
+  sf_plan* P;
+  explicit GllTables(sf_plan* p) : P(p) { flip(); }
+  ~GllTables() { flip(); }
+  void flip() {
+    std::swap(P->dB, P->dBg);
+    std::swap(P->dD, P->dDg);
+    std::swap(P->dB32, P->dBg32);
+    std::swap(P->dD32, P->dDg32);
+  }
+};
 
 int read_scalars(sf_plan* P, CGScalars* h, hipStream_t s) {
   SF_HIP(hipMemcpyAsync(h, P->scal.p, sizeof(CGScalars), hipMemcpyDeviceToHost, s));
@@ -1025,7 +1109,7 @@ struct Progress {
 // iterations are queued between host checks of the device scalars; once
 // converged the remaining queued updates are no-ops.
 int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
-           hipStream_t s, int* iters_out, int* nonfinite_out, const float* dinv, bool gll) {
+           hipStream_t s, int* iters_out, int* nonfinite_out, const float* dinv) {
   Reducer red{P->red_part.p, P->red_counter.p};
   double bb = 0.0;
   int rc = dot_host(P, b, b, &bb, s);
@@ -1035,17 +1119,10 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   if (rc) return rc;
   SF_HIP(launch_cg_start<double>(b, P->w_t.p, P->w_r.p, P->w_p.p, P->w_q.p,
                                  P->nfree, red, P->scal.p, stop_rr, s, dinv));
-  double* const z = P->w_z.p;
-  if (gll) {  // p = z = M^{-1} r, rr = r.z
-    rc = pc_gll<double>(P, P->w_r.p, z, P->w_w.p, P->scal.p, s);
-    if (rc) return rc;
-    SF_HIP((launch_cg_set_rz<double, double>(P->w_r.p, z, P->nfree, red, P->scal.p, s)));
-    SF_HIP(hipMemcpyAsync(P->w_p.p, z, P->nfree * 8, hipMemcpyDeviceToDevice, s));
-  }
   CGScalars h{};
   rc = read_scalars(P, &h, s);
   if (rc) return rc;
-  Progress prog(gll ? "cg (gll)" : dinv ? "cg (jacobi)" : "cg");
+  Progress prog(dinv ? "cg (preconditioned)" : "cg");
   const int chunk = 8;
   // 3D fused-geometry apply: p.Ap from the apply (alpha_parts / resid / xdir)
   const bool fused = P->npart > 0 && P->w_part.p && fused_f64(P);
@@ -1058,16 +1135,6 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
         rc = do_apply_f64(P, P->w_q.p, P->w_p.p, s, 0, true, P->w_part.p);
         if (rc) return rc;
         SF_HIP(launch_cg_alpha_parts(P->w_part.p, P->npart, red, P->scal.p, s));
-        if (gll) {
-          SF_HIP((launch_cg_resid_shares<double, double>(P->w_r.p, P->w_q.p, P->nfree, red,
-                                                         P->scal.p, s)));
-          rc = pc_gll<double>(P, P->w_r.p, z, P->w_w.p, P->scal.p, s);
-          if (rc) return rc;
-          SF_HIP((launch_rz_shares<double, double>(P->w_r.p, z, P->nfree, red, P->scal.p, s)));
-          SF_HIP(launch_cg_finish(P->nfree, red, P->scal.p, s));
-          SF_HIP(launch_cg_xdir<double>(x, P->w_p.p, z, P->w_q.p, P->nfree, P->scal.p, s));
-          continue;
-        }
         SF_HIP(launch_cg_resid<double>(P->w_r.p, P->w_q.p, P->nfree, red, P->scal.p, s, dinv));
         SF_HIP(launch_cg_xdir<double>(x, P->w_p.p, P->w_r.p, P->w_q.p, P->nfree, P->scal.p, s,
                                       dinv));
@@ -1076,16 +1143,6 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
       rc = do_apply_f64(P, P->w_q.p, P->w_p.p, s, 0, true);
       if (rc) return rc;
       SF_HIP(launch_cg_alpha<double>(P->w_p.p, P->w_q.p, P->nfree, red, P->scal.p, s));
-      if (gll) {
-        SF_HIP((launch_cg_update_shares<double, double>(x, P->w_r.p, P->w_p.p, P->w_q.p,
-                                                        P->nfree, red, P->scal.p, s)));
-        rc = pc_gll<double>(P, P->w_r.p, z, P->w_w.p, P->scal.p, s);
-        if (rc) return rc;
-        SF_HIP((launch_rz_shares<double, double>(P->w_r.p, z, P->nfree, red, P->scal.p, s)));
-        SF_HIP(launch_cg_finish(P->nfree, red, P->scal.p, s));
-        SF_HIP(launch_cg_dir<double>(P->w_p.p, z, P->w_q.p, P->nfree, P->scal.p, s));
-        continue;
-      }
       SF_HIP(launch_cg_update<double>(x, P->w_r.p, P->w_p.p, P->w_q.p, P->nfree, red, P->scal.p,
                                       s, dinv));
       SF_HIP(launch_cg_dir<double>(P->w_p.p, P->w_r.p, P->w_q.p, P->nfree, P->scal.p, s, dinv));
@@ -1102,8 +1159,7 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
 // b32 = w_r32 of unit norm, d = 0 initially, to ||r|| <= inner_rtol or
 // maxit iterations.
 int cg_f32_inner(sf_plan* P, int precision, double inner_rtol, int maxit,
-                 hipStream_t s, int* iters_out, int* nonfinite_out, const float* dinv,
-                 bool gll) {
+                 hipStream_t s, int* iters_out, int* nonfinite_out, const float* dinv) {
   Reducer red{P->red_part.p, P->red_counter.p};
   const float* b32 = P->w_r32.p;
   float* r = P->w_s32.p;
@@ -1113,13 +1169,6 @@ int cg_f32_inner(sf_plan* P, int precision, double inner_rtol, int maxit,
   SF_HIP(launch_fill<float>(d, P->nfree, 0.0f, s));
   SF_HIP(launch_cg_start<float>(b32, d, r, p, q, P->nfree, red, P->scal.p,
                                 inner_rtol * inner_rtol, s, dinv));
-  float* const z = P->w_z32.p;
-  if (gll) {
-    int rc0 = pc_gll<float>(P, r, z, P->w_w32.p, P->scal.p, s);
-    if (rc0) return rc0;
-    SF_HIP((launch_cg_set_rz<float, float>(r, z, P->nfree, red, P->scal.p, s)));
-    SF_HIP(hipMemcpyAsync(p, z, P->nfree * 4, hipMemcpyDeviceToDevice, s));
-  }
   CGScalars h{};
   int rc = read_scalars(P, &h, s);
   if (rc) return rc;
@@ -1132,15 +1181,6 @@ int cg_f32_inner(sf_plan* P, int precision, double inner_rtol, int maxit,
       rc = do_apply_f32(P, q, p, precision, s, true);
       if (rc) return rc;
       SF_HIP(launch_cg_alpha<float>(p, q, P->nfree, red, P->scal.p, s));
-      if (gll) {
-        SF_HIP((launch_cg_update_shares<float, float>(d, r, p, q, P->nfree, red, P->scal.p, s)));
-        rc = pc_gll<float>(P, r, z, P->w_w32.p, P->scal.p, s);
-        if (rc) return rc;
-        SF_HIP((launch_rz_shares<float, float>(r, z, P->nfree, red, P->scal.p, s)));
-        SF_HIP(launch_cg_finish(P->nfree, red, P->scal.p, s));
-        SF_HIP(launch_cg_dir<float>(p, z, q, P->nfree, P->scal.p, s));
-        continue;
-      }
       SF_HIP(launch_cg_update<float>(d, r, p, q, P->nfree, red, P->scal.p, s, dinv));
       SF_HIP(launch_cg_dir<float>(p, r, q, P->nfree, P->scal.p, s, dinv));
     }
@@ -1193,7 +1233,7 @@ int true_residual(sf_plan* P, const double* b, const double* x, double* rn,
 int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
                 double rtol, int maxit, double delta, hipStream_t s,
                 int* iters_out, int* updates_out, int* nonfinite_out,
-                double* rel_out, const float* dinv, bool gll) {
+                double* rel_out, const float* dinv) {
   Reducer red{P->red_part.p, P->red_counter.p};
   float* r = P->w_s32.p;
   float* xl = P->w_d32.p;
@@ -1220,13 +1260,6 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
   const double t0 = threshold(rn);
   SF_HIP(launch_cg_start<float>(P->w_r32.p, xl, r, p, q, P->nfree, red, P->scal.p,
                                 t0 * t0, s, dinv));
-  float* const z = P->w_z32.p;
-  if (gll) {
-    rc = pc_gll<float>(P, r, z, P->w_w32.p, P->scal.p, s);
-    if (rc) return rc;
-    SF_HIP((launch_cg_set_rz<float, float>(r, z, P->nfree, red, P->scal.p, s)));
-    SF_HIP(hipMemcpyAsync(p, z, P->nfree * 4, hipMemcpyDeviceToDevice, s));
-  }
   CGScalars h{};
   rc = read_scalars(P, &h, s);
   if (rc) return rc;
@@ -1240,16 +1273,6 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
         rc = do_apply_f32(P, q, p, precision, s, true);
         if (rc) return rc;
         SF_HIP(launch_cg_alpha<float>(p, q, P->nfree, red, P->scal.p, s));
-        if (gll) {
-          SF_HIP((launch_cg_update_shares<float, float>(xl, r, p, q, P->nfree, red, P->scal.p,
-                                                        s)));
-          rc = pc_gll<float>(P, r, z, P->w_w32.p, P->scal.p, s);
-          if (rc) return rc;
-          SF_HIP((launch_rz_shares<float, float>(r, z, P->nfree, red, P->scal.p, s)));
-          SF_HIP(launch_cg_finish(P->nfree, red, P->scal.p, s));
-          SF_HIP(launch_cg_dir<float>(p, z, q, P->nfree, P->scal.p, s));
-          continue;
-        }
         SF_HIP(launch_cg_update<float>(xl, r, p, q, P->nfree, red, P->scal.p, s, dinv));
         SF_HIP(launch_cg_dir<float>(p, r, q, P->nfree, P->scal.p, s, dinv));
       }
@@ -1287,14 +1310,9 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
       SF_HIP(launch_wdot<float>(r, dinv, P->nfree, red, P->dscal.p, s));
       SF_HIP(hipMemcpyAsync(&rz, P->dscal.p, sizeof(double), hipMemcpyDeviceToHost, s));
       SF_HIP(hipStreamSynchronize(s));
-    } else if (gll) {  // z of the replaced residual (the device scalars say done)
-      rc = pc_gll<float>(P, r, z, P->w_w32.p, nullptr, s);
-      if (rc) return rc;
-      rc = dot_host_f32(P, r, z, &rz, s);
-      if (rc) return rc;
     }
     const double c = (h.rr_new > 0.0) ? (rn_new / rn) * rz * h.beta / h.rr_new : 0.0;
-    SF_HIP(launch_dir_coef(p, gll ? z : r, q, (float)c, P->nfree, s, dinv));
+    SF_HIP(launch_dir_coef(p, r, q, (float)c, P->nfree, s, dinv));
     SF_HIP(launch_fill<float>(xl, P->nfree, 0.0f, s));
     rn = rn_new;
     const double t = threshold(rn);
@@ -1319,7 +1337,7 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
 int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
                        double rtol, int maxit, double delta, hipStream_t s,
                        int* iters_out, int* updates_out, int* nonfinite_out,
-                       double* rel_out, bool p32only, const float* dinv, bool gll) {
+                       double* rel_out, bool p32only, const float* dinv) {
   Reducer red{P->red_part.p, P->red_counter.p};
   double* r = P->w_r.p;
   double* p = P->w_p.p;
@@ -1341,13 +1359,7 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
   const double t0 = threshold(rn);
   SF_HIP(launch_cg_start<double>(b, P->w_t.p, r, p, P->w_q.p, P->nfree, red, P->scal.p,
                                  t0 * t0, s, dinv));
-  double* const z = P->w_z.p;
-  if (gll) {
-    rc = pc_gll<double>(P, r, z, P->w_w.p, P->scal.p, s);
-    if (rc) return rc;
-    SF_HIP((launch_cg_set_rz<double, double>(r, z, P->nfree, red, P->scal.p, s)));
-    SF_HIP(hipMemcpyAsync(p, z, P->nfree * 8, hipMemcpyDeviceToDevice, s));
-  }
+
   SF_HIP(launch_to_f32(p, P->nfree, p32, s));
   SF_HIP(launch_fill<float>(q32, P->nfree, 0.0f, s));
   CGScalars h{};
@@ -1368,18 +1380,6 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
           rc = do_apply_f32(P, q32, p32, precision, s, true, P->w_part.p);
           if (rc) return rc;
           SF_HIP(launch_cg_alpha_parts(P->w_part.p, P->npart, red, P->scal.p, s));
-          if (gll) {
-            SF_HIP((launch_cg_resid_shares<double, float>(r, q32, P->nfree, red, P->scal.p, s)));
-            rc = pc_gll<double>(P, r, z, P->w_w.p, P->scal.p, s);
-            if (rc) return rc;
-            SF_HIP((launch_rz_shares<double, double>(r, z, P->nfree, red, P->scal.p, s)));
-            SF_HIP(launch_cg_finish(P->nfree, red, P->scal.p, s));
-            if (p32only)
-              SF_HIP(launch_cg_xdir_p32(x, p32, z, q32, P->nfree, P->scal.p, s));
-            else
-              SF_HIP(launch_cg_xdir_mixed(x, p, p32, z, q32, P->nfree, P->scal.p, s));
-            continue;
-          }
           SF_HIP(launch_cg_resid_mixed(r, q32, P->nfree, red, P->scal.p, s, dinv));
           if (p32only)
             SF_HIP(launch_cg_xdir_p32(x, p32, r, q32, P->nfree, P->scal.p, s, dinv));
@@ -1390,16 +1390,6 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
         rc = do_apply_f32(P, q32, p32, precision, s, true);
         if (rc) return rc;
         SF_HIP(launch_cg_alpha_mixed(p, q32, P->nfree, red, P->scal.p, s));
-        if (gll) {
-          SF_HIP((launch_cg_update_shares<double, float>(x, r, p, q32, P->nfree, red, P->scal.p,
-                                                         s)));
-          rc = pc_gll<double>(P, r, z, P->w_w.p, P->scal.p, s);
-          if (rc) return rc;
-          SF_HIP((launch_rz_shares<double, double>(r, z, P->nfree, red, P->scal.p, s)));
-          SF_HIP(launch_cg_finish(P->nfree, red, P->scal.p, s));
-          SF_HIP(launch_cg_dir_mixed(p, p32, z, q32, P->nfree, P->scal.p, s));
-          continue;
-        }
         SF_HIP(launch_cg_update_mixed(x, r, p, q32, P->nfree, red, P->scal.p, s, dinv));
         SF_HIP(launch_cg_dir_mixed(p, p32, r, q32, P->nfree, P->scal.p, s, dinv));
       }
@@ -1428,18 +1418,12 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
       SF_HIP(launch_wdot<double>(r, dinv, P->nfree, red, P->dscal.p, s));
       SF_HIP(hipMemcpyAsync(&rz, P->dscal.p, sizeof(double), hipMemcpyDeviceToHost, s));
       SF_HIP(hipStreamSynchronize(s));
-    } else if (gll) {  // z of the replaced residual (the device scalars say done)
-      rc = pc_gll<double>(P, r, z, P->w_w.p, nullptr, s);
-      if (rc) return rc;
-      rc = dot_host(P, r, z, &rz, s);
-      if (rc) return rc;
     }
     const double c = (h.rr_new > 0.0) ? rz * h.beta / h.rr_new : 0.0;
-    const double* zr = gll ? z : r;
     if (p32only)
-      SF_HIP(launch_dir_coef_p32(p32, zr, q32, c, P->nfree, s, dinv));
+      SF_HIP(launch_dir_coef_p32(p32, r, q32, c, P->nfree, s, dinv));
     else
-      SF_HIP(launch_dir_coef_mixed(p, p32, zr, q32, c, P->nfree, s, dinv));
+      SF_HIP(launch_dir_coef_mixed(p, p32, r, q32, c, P->nfree, s, dinv));
     const double t = threshold(rn_new);
     h.res2 = rn_new * rn_new;
     h.rr = rz;
@@ -1454,6 +1438,60 @@ int cg_reliable_f64rec(sf_plan* P, int precision, const double* b, double* x,
 double now_s() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// A solve with the Gauss-Lobatto preconditioner.  PCG on A with
+// M^{-1} = T diag(T^T A T)^{-1} T^T is, in y = T^{-1} x, Jacobi PCG on
+// A_g = T^T A T with right-hand side T^T b, and A_g is the same apply on
+// the GLL tables: run(bg, y, rtol_g, maxit, &its, &nonfinite) runs one of
+// the solvers on it, with no pass per iteration.  y starts at T^{-1} x and
+// x = T y after.  The solver's own target rtol_g (relative to ||T^T b||)
+// starts at rtol ||b|| / ||T^T b|| and tightens while the true
+// ||b - A x|| / ||b|| stays above rtol.  Budget and *iters_out in the
+// units of run's count.
+template <typename Run>
+int solve_gll(sf_plan* P, const double* b, double* x, double rtol, int maxit, hipStream_t s,
+              Run&& run, int* iters_out, double* rel_out, int* nonfinite_out) {
+  double* bg = P->g_b.p;
+  double* y = P->g_y.p;
+  *iters_out = 0;
+  *nonfinite_out = 0;
+  double bb = 0.0, gg = 0.0;
+  int rc = dot_host(P, b, b, &bb, s);
+  if (rc) return rc;
+  rc = gll_pass(P, true, b, P->minv.p, bg, P->dT1t.p, s);
+  if (rc) return rc;
+  rc = gll_pass(P, false, x, nullptr, y, P->dT1i.p, s);
+  if (rc) return rc;
+  rc = dot_host(P, bg, bg, &gg, s);
+  if (rc) return rc;
+  const double bnorm = sqrt(bb);
+  double rtol_g = gg > 0 ? rtol * bnorm / sqrt(gg) : rtol;
+  double rel = 0.0;
+  for (;;) {
+    int it = 0, nonfinite = 0;
+    {
+      GllTables swap(P);
+      rc = run(bg, y, rtol_g, maxit - *iters_out, &it, &nonfinite);
+      if (rc) return rc;
+    }
+    *iters_out += it;
+    rc = gll_pass(P, false, y, nullptr, x, P->dT1.p, s);
+    if (rc) return rc;
+    double rn = 0.0;
+    rc = true_residual(P, b, x, &rn, s);
+    if (rc) return rc;
+    rel = bnorm > 0 ? rn / bnorm : rn;
+    if (nonfinite || !isfinite(rel)) {
+      *nonfinite_out = 1;
+      break;
+    }
+    if (!(rel > rtol) || *iters_out >= maxit || it == 0) break;
+    const double f = 0.8 * rtol / rel;
+    rtol_g *= f > 1e-3 ? f : 1e-3;
+  }
+  *rel_out = rel;
+  return SF_OK;
 }
 
 }  // namespace
@@ -1475,14 +1513,22 @@ int cg_solve(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   const double t0 = now_s();
   int iters = 0, nonfinite = 0;
   double bb = 0.0, rel = 0.0;
-  rc = dot_host(P, b, b, &bb, s);
-  if (rc) return rc;
+  if (gll) {
+    auto run = [&](const double* bg, double* y, double rt, int mi, int* it, int* nf) {
+      return cg_f64(P, bg, y, rt, mi, s, it, nf, dinv);
+    };
+    rc = solve_gll(P, b, x, rtol, maxit, s, run, &iters, &rel, &nonfinite);
+    if (rc) return rc;
+  } else {
+    rc = dot_host(P, b, b, &bb, s);
+    if (rc) return rc;
+  }
   // The recurred residual drifts from the true one; when the recurrence
   // stops short of rtol in the true residual, restart from x (the restart
   // recomputes r = b - A x).
-  for (;;) {
+  while (!gll) {
     int it = 0;
-    rc = cg_f64(P, b, x, rtol, maxit - iters, s, &it, &nonfinite, dinv, gll);
+    rc = cg_f64(P, b, x, rtol, maxit - iters, s, &it, &nonfinite, dinv);
     if (rc) return rc;
     iters += it;
     double rn = 0.0;
@@ -1502,6 +1548,50 @@ int cg_solve(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   if (nonfinite || !isfinite(rel)) return fail(SF_ERR_NONFINITE, "CG met a non-finite value");
   if (rel > rtol && iters >= maxit)
     return fail(SF_ERR_NOT_CONVERGED, "CG did not reach rtol %g in %d iterations (rel. residual %g)", rtol, maxit, rel);
+  return SF_OK;
+}
+
+// Iterative refinement: FP64 residuals r = b - A x, inner CG on r / ||r||
+// in reduced precision (cg_f32_inner), x += ||r|| d, until ||r|| <= rtol ||b||
+// or max_outer inner solves.
+int refine(sf_plan* P, const double* b, double* x, double rtol, int max_outer,
+           int inner_precision, double inner_rtol, int inner_maxit, hipStream_t s,
+           int* inner_total, int* outer_out, int* nonfinite_out, double* rel_out,
+           const float* dinv) {
+  double bb = 0.0;
+  int rc = dot_host(P, b, b, &bb, s);
+  if (rc) return rc;
+  const double bnorm = sqrt(bb);
+  int outer = 0, nonfinite = 0;
+  double rel = 0.0;
+  MarkerRange mark("sfhip refinement");
+  for (;;) {
+    MarkerRange step("sfhip refinement outer step");
+    double rn = 0.0;
+    rc = true_residual(P, b, x, &rn, s);
+    if (rc) return rc;
+    rel = bnorm > 0 ? rn / bnorm : rn;
+    if (!isfinite(rel)) {
+      nonfinite = 1;
+      break;
+    }
+    if (rel <= rtol || outer >= max_outer || rn == 0.0) break;
+    // inner solve on the unit-norm residual in reduced precision
+    SF_HIP(launch_scale_to_f32(P->w_r.p, 1.0 / rn, P->w_r32.p, P->nfree, s));
+    int it = 0, nf = 0;
+    rc = cg_f32_inner(P, inner_precision, inner_rtol, inner_maxit, s, &it, &nf, dinv);
+    if (rc) return rc;
+    *inner_total += it;
+    ++outer;
+    if (nf) {
+      nonfinite = 1;
+      break;
+    }
+    SF_HIP(launch_axpy_f32_to_f64(x, P->w_d32.p, rn, P->nfree, s));
+  }
+  *outer_out = outer;
+  *nonfinite_out = nonfinite;
+  *rel_out = rel;
   return SF_OK;
 }
 
@@ -2209,7 +2299,7 @@ int sf_cg_noalloc(sf_plan* P, const double* b, double* x, double rtol, int maxit
   ScratchCarve c;
   const size_t o_r = c.take(nf * 8), o_p = c.take(nf * 8), o_q = c.take(nf * 8),
                o_t = c.take(nf * 8);
-  const size_t o_z = gll ? c.take(nf * 8) : 0, o_w = gll ? c.take(nf * 8) : 0;
+  const size_t o_gb = gll ? c.take(nf * 8) : 0, o_gy = gll ? c.take(nf * 8) : 0;
   const size_t o_red = c.take(2 * (size_t)kPartSlots * 8), o_part = c.take(np * 8);
   const size_t o_cnt = c.take(sizeof(unsigned)), o_sc = c.take(sizeof(CGScalars));
   const size_t o_ds = c.take(4 * 8);
@@ -2223,8 +2313,8 @@ int sf_cg_noalloc(sf_plan* P, const double* b, double* x, double rtol, int maxit
   if (rc) return rc;
   BufSwap<double> s_r(P->w_r, (double*)(base + o_r), nf), s_p(P->w_p, (double*)(base + o_p), nf),
       s_q(P->w_q, (double*)(base + o_q), nf), s_t(P->w_t, (double*)(base + o_t), nf),
-      s_z(P->w_z, gll ? (double*)(base + o_z) : P->w_z.p, gll ? nf : P->w_z.n),
-      s_w(P->w_w, gll ? (double*)(base + o_w) : P->w_w.p, gll ? nf : P->w_w.n),
+      s_gb(P->g_b, gll ? (double*)(base + o_gb) : P->g_b.p, gll ? nf : P->g_b.n),
+      s_gy(P->g_y, gll ? (double*)(base + o_gy) : P->g_y.p, gll ? nf : P->g_y.n),
       s_red(P->red_part, (double*)(base + o_red), 2 * (int64_t)kPartSlots),
       s_part(P->w_part, np ? (double*)(base + o_part) : P->w_part.p, np ? np : P->w_part.n),
       s_ds(P->dscal, (double*)(base + o_ds), 4);
@@ -2264,11 +2354,22 @@ int sf_cg_mixed(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   const double t0 = now_s();
   int iters = 0, updates = 0, nonfinite = 0;
   double rel = 0.0;
-  rc = fp64_recurrence
-           ? cg_reliable_f64rec(P, inner_precision, b, x, rtol, maxit, delta, s, &iters,
-                                &updates, &nonfinite, &rel, fp64_recurrence == 2, dinv, gll)
-           : cg_reliable(P, inner_precision, b, x, rtol, maxit, delta, s, &iters,
-                         &updates, &nonfinite, &rel, dinv, gll);
+  auto run = [&](const double* rb, double* rx, double rt, int mi, int* it, int* nf) {
+    int up = 0;
+    double rl = 0.0;
+    const int r = fp64_recurrence
+                      ? cg_reliable_f64rec(P, inner_precision, rb, rx, rt, mi, delta, s, it,
+                                           &up, nf, &rl, fp64_recurrence == 2, dinv)
+                      : cg_reliable(P, inner_precision, rb, rx, rt, mi, delta, s, it, &up,
+                                    nf, &rl, dinv);
+    updates += up;
+    if (!gll) rel = rl;
+    return r;
+  };
+  if (gll)
+    rc = solve_gll(P, b, x, rtol, maxit, s, run, &iters, &rel, &nonfinite);
+  else
+    rc = run(b, x, rtol, maxit, &iters, &nonfinite);
   if (rc) return rc;
   SF_HIP(hipStreamSynchronize(s));
   const double t1 = now_s();
@@ -2309,36 +2410,22 @@ int sf_cg_ir(sf_plan* P, const double* b, double* x, double rtol,
   if (rc) return rc;
   SF_HIP(hipStreamSynchronize(s));
   const double t0 = now_s();
-  double bb = 0.0;
-  rc = dot_host(P, b, b, &bb, s);
-  if (rc) return rc;
-  const double bnorm = sqrt(bb);
   int inner_total = 0, outer = 0, nonfinite = 0;
   double rel = 0.0;
-  MarkerRange mark("sfhip refinement");
-  for (;;) {
-    MarkerRange step("sfhip refinement outer step");
-    // FP64 residual r = b - A x
-    rc = do_apply_f64(P, P->w_t.p, x, s);
-    if (rc) return rc;
-    SF_HIP(launch_residual(b, P->w_t.p, P->w_r.p, P->nfree, s));
-    double rr = 0.0;
-    rc = dot_host(P, P->w_r.p, P->w_r.p, &rr, s);
-    if (rc) return rc;
-    const double rn = sqrt(rr);
-    rel = bnorm > 0 ? rn / bnorm : rn;
-    if (!isfinite(rel)) { nonfinite = 1; break; }
-    if (rel <= rtol || outer >= max_outer || rn == 0.0) break;
-    // inner solve on the unit-norm residual in reduced precision
-    SF_HIP(launch_scale_to_f32(P->w_r.p, 1.0 / rn, P->w_r32.p, P->nfree, s));
-    int it = 0, nf = 0;
-    rc = cg_f32_inner(P, inner_precision, inner_rtol, inner_maxit, s, &it, &nf, dinv, gll);
-    if (rc) return rc;
-    inner_total += it;
-    ++outer;
-    if (nf) { nonfinite = 1; break; }
-    SF_HIP(launch_axpy_f32_to_f64(x, P->w_d32.p, rn, P->nfree, s));
-  }
+  auto run = [&](const double* rb, double* rx, double rt, int mo, int* it, int* nf) {
+    int inner = 0;
+    double rl = 0.0;
+    const int r = refine(P, rb, rx, rt, mo, inner_precision, inner_rtol, inner_maxit, s,
+                         &inner, it, nf, &rl, dinv);
+    inner_total += inner;
+    if (!gll) rel = rl;
+    return r;
+  };
+  if (gll)
+    rc = solve_gll(P, b, x, rtol, max_outer, s, run, &outer, &rel, &nonfinite);
+  else
+    rc = run(b, x, rtol, max_outer, &outer, &nonfinite);
+  if (rc) return rc;
   SF_HIP(hipStreamSynchronize(s));
   const double t1 = now_s();
   if (stats) {

@@ -160,24 +160,6 @@ __global__ void __launch_bounds__(kRedThreads)
   }
 }
 
-// One workgroup: rr = rr_new = r.z from the np shares (the start of a CG
-// whose preconditioner runs as separate passes).
-__global__ void __launch_bounds__(kRedThreads)
-    k_cg_set_rz(const double* __restrict__ part, int np, CGScalars* __restrict__ sc) {
-  if (sc->done) return;  // converged at the start: no shares were written
-  double s = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
-  s = block_sum(s);
-  if (threadIdx.x == 0) {
-    sc->rr = s;
-    sc->rr_new = s;
-    if (!isfinite(s)) {
-      sc->nonfinite = 1;
-      sc->done = 1;
-    }
-  }
-}
-
 // p.q on the streaming grid, per-workgroup shares (finished by
 // k_cg_alpha_finish; see k_cg_resid for why there is no arrival counter).
 template <typename V, typename W = V>
@@ -798,73 +780,6 @@ hipError_t launch_axpy_f32_to_f64(double* x, const float* d32, double scale,
                      x, d32, scale, n);
   return hipGetLastError();
 }
-
-// ---- pieces for a preconditioner that runs as separate passes between
-// the residual update and the finish (the Gauss-Lobatto-basis one):
-//   resid/update shares -> z = M^{-1} r -> r.z shares -> finish(pc)
-
-template <typename V, typename W>
-hipError_t launch_cg_resid_shares(V* r, const W* q, int64_t n, Reducer red, CGScalars* sc,
-                                  hipStream_t s) {
-  hipLaunchKernelGGL((k_cg_resid<V, W, false>), dim3(stream_grid(n)), dim3(256), 0, s, r, q,
-                     nullptr, n, red.part, sc);
-  return hipGetLastError();
-}
-
-template <typename V, typename W>
-hipError_t launch_cg_update_shares(V* x, V* r, const V* p, const W* q, int64_t n, Reducer red,
-                                   CGScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL((k_cg_update<V, W, false>), dim3(stream_grid(n)), dim3(256), 0, s, x, r,
-                     p, q, nullptr, n, red.part, sc);
-  return hipGetLastError();
-}
-
-template <typename V, typename Z>
-hipError_t launch_rz_shares(const V* r, const Z* z, int64_t n, Reducer red, CGScalars* sc,
-                            hipStream_t s) {
-  const unsigned g = stream_grid(n);
-  hipLaunchKernelGGL((k_cg_alpha<V, Z>), dim3(g), dim3(256), 0, s, r, z, n, red.part + g, sc);
-  return hipGetLastError();
-}
-
-hipError_t launch_cg_finish(int64_t n, Reducer red, CGScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_resid_finish, dim3(1), dim3(kRedThreads), 0, s, red.part,
-                     (int)stream_grid(n), 1, sc);
-  return hipGetLastError();
-}
-
-template <typename V, typename Z>
-hipError_t launch_cg_set_rz(const V* r, const Z* z, int64_t n, Reducer red, CGScalars* sc,
-                            hipStream_t s) {
-  const unsigned g = stream_grid(n);
-  hipLaunchKernelGGL((k_cg_alpha<V, Z>), dim3(g), dim3(256), 0, s, r, z, n, red.part, sc);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_cg_set_rz, dim3(1), dim3(kRedThreads), 0, s, red.part, (int)g, sc);
-  return hipGetLastError();
-}
-
-template hipError_t launch_cg_resid_shares<double, double>(double*, const double*, int64_t,
-                                                           Reducer, CGScalars*, hipStream_t);
-template hipError_t launch_cg_resid_shares<double, float>(double*, const float*, int64_t,
-                                                          Reducer, CGScalars*, hipStream_t);
-template hipError_t launch_cg_update_shares<double, double>(double*, double*, const double*,
-                                                            const double*, int64_t, Reducer,
-                                                            CGScalars*, hipStream_t);
-template hipError_t launch_cg_update_shares<double, float>(double*, double*, const double*,
-                                                           const float*, int64_t, Reducer,
-                                                           CGScalars*, hipStream_t);
-template hipError_t launch_cg_update_shares<float, float>(float*, float*, const float*,
-                                                          const float*, int64_t, Reducer,
-                                                          CGScalars*, hipStream_t);
-template hipError_t launch_rz_shares<double, double>(const double*, const double*, int64_t,
-                                                     Reducer, CGScalars*, hipStream_t);
-template hipError_t launch_rz_shares<float, float>(const float*, const float*, int64_t,
-                                                   Reducer, CGScalars*, hipStream_t);
-template hipError_t launch_cg_set_rz<double, double>(const double*, const double*, int64_t,
-                                                     Reducer, CGScalars*, hipStream_t);
-template hipError_t launch_cg_set_rz<float, float>(const float*, const float*, int64_t,
-                                                   Reducer, CGScalars*, hipStream_t);
 
 #define SF_INST(V)                                                            \
   template hipError_t launch_fill<V>(V*, int64_t, V, hipStream_t);            \

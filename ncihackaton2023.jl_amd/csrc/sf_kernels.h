@@ -129,6 +129,12 @@ hipError_t launch_pc_gll(int n, const TR* r, TZ* z, TZ* w, int64_t nfree, const 
                          const float* dginv, const double* T1, const double* T1t,
                          const BlockArgs& k, int eb, int max_nu, const CGScalars* sc,
                          hipStream_t s);
+// One pass: trans: out = T^T (scale . in) (out zeroed first, shared dofs
+// added); else out = M (scale . in) with M = T1 or T1^{-1} (unit vertex
+// rows).  scale may be null (1).  FP64.
+hipError_t launch_gll_pass(int n, bool trans, const double* in, const float* scale, double* out,
+                           const double* M, int64_t nfree, const BlockArgs& k, int eb,
+                           int max_nu, hipStream_t s);
 // z = dinv . r (the Jacobi preconditioner as a stand-alone product)
 hipError_t launch_scale_vec(const double* r, const float* dinv, double* z, int64_t n,
                             hipStream_t s);
@@ -249,23 +255,6 @@ hipError_t launch_cg_dir_mixed(double* p, float* p32, const double* r, float* q3
 hipError_t launch_dir_coef_mixed(double* p, float* p32, const double* r, float* q32,
                                  double c, int64_t n, hipStream_t s,
                                  const float* dinv = nullptr);
-// Pieces for a preconditioner that runs as separate passes (the GLL one):
-// residual / update without the finish, r.z shares into the second half of
-// red.part, the finish (res2, rr_new = r.z, beta, the stop test), and at
-// the start rr = rr_new = r.z.
-template <typename V, typename W>
-hipError_t launch_cg_resid_shares(V* r, const W* q, int64_t n, Reducer red, CGScalars* sc,
-                                  hipStream_t s);
-template <typename V, typename W>
-hipError_t launch_cg_update_shares(V* x, V* r, const V* p, const W* q, int64_t n, Reducer red,
-                                   CGScalars* sc, hipStream_t s);
-template <typename V, typename Z>
-hipError_t launch_rz_shares(const V* r, const Z* z, int64_t n, Reducer red, CGScalars* sc,
-                            hipStream_t s);
-hipError_t launch_cg_finish(int64_t n, Reducer red, CGScalars* sc, hipStream_t s);
-template <typename V, typename Z>
-hipError_t launch_cg_set_rz(const V* r, const Z* z, int64_t n, Reducer red, CGScalars* sc,
-                            hipStream_t s);
 // p = z + c p, q = 0 (FP32)
 hipError_t launch_dir_coef(float* p, const float* r, float* q, float c,
                            int64_t n, hipStream_t s, const float* dinv = nullptr);

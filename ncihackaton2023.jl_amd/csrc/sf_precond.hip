@@ -23,7 +23,12 @@
 //   * k_pc_gll<.., TRANS = false>: z = T (dginv w): every cell holding a
 //     dof computes the same value for it (the unit vertex rows make the
 //     contraction along a face normal an exact copy), so each block stores
-//     its dofs' values with plain stores.
+//     its dofs' values with plain stores.  (scale may be null: 1.)
+// The solvers do not apply M^{-1} per iteration: PCG with this M on A is CG
+// with the Jacobi preconditioner on T^T A T, so they run in the GLL basis
+// (the apply with the GLL tables) and use these passes only to form T^T b,
+// T^{-1} x0 (the forward pass with T1^{-1}, whose vertex rows are unit rows
+// too) and x = T y; sf_precond_apply runs both passes.
 // Each pass: one workgroup per block of the apply, the block's distinct
 // dofs gathered once (sorted ids, coalesced), three 1D contractions per
 // cell through LDS (one N-long pencil per thread), and one global access
@@ -93,7 +98,8 @@ __global__ void __launch_bounds__(1024)
   TZ xv[N];
 #pragma unroll
   for (int k = 0; k < N; ++k)
-    xv[k] = dv[k] >= 0 ? (TZ)((double)in[dv[k]] * (double)scale[dv[k]]) : TZ(0);
+    xv[k] = dv[k] >= 0 ? (TZ)((double)in[dv[k]] * (scale ? (double)scale[dv[k]] : 1.0))
+                       : TZ(0);
 #pragma unroll
   for (int k = 0; k < N; ++k)
     if (dv[k] >= 0) V[tid + k * nt] = xv[k];
@@ -258,9 +264,17 @@ template hipError_t launch_pc_gll<double, double>(int, const double*, double*, d
                                                   const float*, const float*, const double*,
                                                   const double*, const BlockArgs&, int, int,
                                                   const CGScalars*, hipStream_t);
-template hipError_t launch_pc_gll<float, float>(int, const float*, float*, float*, int64_t,
-                                                const float*, const float*, const double*,
-                                                const double*, const BlockArgs&, int, int,
-                                                const CGScalars*, hipStream_t);
+
+hipError_t launch_gll_pass(int n, bool trans, const double* in, const float* scale, double* out,
+                           const double* M, int64_t nfree, const BlockArgs& k, int eb,
+                           int max_nu, hipStream_t s) {
+  if (nfree <= 0) return hipSuccess;
+  if (trans) {
+    hipError_t e = launch_fill<double>(out, nfree, 0.0, s);
+    if (e != hipSuccess) return e;
+    return launch_pass<double, double, true>(n, in, scale, out, M, k, eb, max_nu, nullptr, s);
+  }
+  return launch_pass<double, double, false>(n, in, scale, out, M, k, eb, max_nu, nullptr, s);
+}
 
 }  // namespace sf
