@@ -94,7 +94,11 @@ typedef struct sf_plan_info {
 
 typedef struct sf_solve_stats {
   int iterations;        /* CG iterations (IR: total inner iterations)      */
-  int outer_iterations;  /* IR outer (FP64 residual) steps; 0 for plain CG  */
+  int outer_iterations;  /* IR: outer (FP64 residual) steps; mixed CG: the
+                            residual replacements; CG: how often the true
+                            residual, checked when the recurrence met its
+                            target, sent it on to a lower one (0 when the
+                            first target sufficed)                          */
   int converged;         /* 1 when ||b - A x|| / ||b|| <= rtol               */
   double rel_residual;   /* final FP64 ||b - A x|| / ||b||                   */
   double seconds;        /* wall time of the solve (device synchronised)     */

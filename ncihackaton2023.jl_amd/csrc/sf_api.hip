@@ -12,6 +12,7 @@
 #include <chrono>
 #include <climits>
 #include <deque>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <new>
@@ -1105,11 +1106,21 @@ struct Progress {
   }
 };
 
+int write_scalars(sf_plan* P, const CGScalars* h, hipStream_t s);
+
+// Called when the recurrence has met its target (res2 = its r.r): sets
+// *stop_rr to a lower target to continue the same recurrence, or to 0 to
+// accept.
+using CgCheck = std::function<int(double res2, double* stop_rr)>;
+
 // Plain CG on the FP64 operator, device vectors.  Chunks of `chunk`
 // iterations are queued between host checks of the device scalars; once
-// converged the remaining queued updates are no-ops.
+// converged the remaining queued updates are no-ops.  check (optional):
+// see CgCheck; continuing costs one direction update (the converging step
+// moved x and r but not p).
 int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
-           hipStream_t s, int* iters_out, int* nonfinite_out, const float* dinv) {
+           hipStream_t s, int* iters_out, int* nonfinite_out, const float* dinv,
+           const CgCheck* check = nullptr) {
   Reducer red{P->red_part.p, P->red_counter.p};
   double bb = 0.0;
   int rc = dot_host(P, b, b, &bb, s);
@@ -1126,7 +1137,22 @@ int cg_f64(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   const int chunk = 8;
   // 3D fused-geometry apply: p.Ap from the apply (alpha_parts / resid / xdir)
   const bool fused = P->npart > 0 && P->w_part.p && fused_f64(P);
-  while (!h.done && h.iters < maxit) {
+  for (;;) {
+    if (h.done) {
+      if (!check || h.nonfinite) break;
+      double stop = 0.0;
+      rc = (*check)(h.res2, &stop);
+      if (rc) return rc;
+      if (!(stop > 0.0)) break;
+      // continue: p = z + beta p and q = 0, which the converging step left
+      h.done = 0;
+      h.xpend = 0;
+      h.stop_rr = stop;
+      rc = write_scalars(P, &h, s);
+      if (rc) return rc;
+      SF_HIP(launch_cg_dir<double>(P->w_p.p, P->w_r.p, P->w_q.p, P->nfree, P->scal.p, s, dinv));
+    }
+    if (h.iters >= maxit) break;
     prog.tick(h.iters, bb > 0 ? sqrt(h.res2 / bb) : sqrt(h.res2));
     MarkerRange mark("sfhip cg chunk (8 iterations)");
     const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
@@ -1447,15 +1473,17 @@ double now_s() {
 // the solvers on it, with no pass per iteration.  y starts at T^{-1} x and
 // x = T y after.  The solver's own target rtol_g (relative to ||T^T b||)
 // starts at rtol ||b|| / ||T^T b|| and tightens while the true
-// ||b - A x|| / ||b|| stays above rtol.  Budget and *iters_out in the
-// units of run's count.
+// ||b - A x|| / ||b|| stays above rtol (*restarts_out: the runs after the
+// first).  Budget and *iters_out in the units of run's count.
 template <typename Run>
 int solve_gll(sf_plan* P, const double* b, double* x, double rtol, int maxit, hipStream_t s,
-              Run&& run, int* iters_out, double* rel_out, int* nonfinite_out) {
+              Run&& run, int* iters_out, double* rel_out, int* nonfinite_out,
+              int* restarts_out) {
   double* bg = P->g_b.p;
   double* y = P->g_y.p;
   *iters_out = 0;
   *nonfinite_out = 0;
+  *restarts_out = 0;
   double bb = 0.0, gg = 0.0;
   int rc = dot_host(P, b, b, &bb, s);
   if (rc) return rc;
@@ -1489,6 +1517,7 @@ int solve_gll(sf_plan* P, const double* b, double* x, double rtol, int maxit, hi
     if (!(rel > rtol) || *iters_out >= maxit || it == 0) break;
     const double f = 0.8 * rtol / rel;
     rtol_g *= f > 1e-3 ? f : 1e-3;
+    ++*restarts_out;
   }
   *rel_out = rel;
   return SF_OK;
@@ -1511,14 +1540,64 @@ int cg_solve(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   if (rc) return rc;
   SF_HIP(hipStreamSynchronize(s));
   const double t0 = now_s();
-  int iters = 0, nonfinite = 0;
+  int iters = 0, nonfinite = 0, restarts = 0;
   double bb = 0.0, rel = 0.0;
   if (gll) {
-    auto run = [&](const double* bg, double* y, double rt, int mi, int* it, int* nf) {
-      return cg_f64(P, bg, y, rt, mi, s, it, nf, dinv);
-    };
-    rc = solve_gll(P, b, x, rtol, maxit, s, run, &iters, &rel, &nonfinite);
+    // CG in the Gauss-Lobatto basis (solve_gll); each time the recurrence
+    // meets its target the true residual of x = T y is checked with the
+    // apply's own tables, and the same recurrence continues to a lower
+    // target while it is above rtol (no restart, no Krylov space lost)
+    double* bg = P->g_b.p;
+    double* y = P->g_y.p;
+    double gg = 0.0;
+    rc = dot_host(P, b, b, &bb, s);
     if (rc) return rc;
+    rc = gll_pass(P, true, b, P->minv.p, bg, P->dT1t.p, s);
+    if (rc) return rc;
+    rc = gll_pass(P, false, x, nullptr, y, P->dT1i.p, s);
+    if (rc) return rc;
+    rc = dot_host(P, bg, bg, &gg, s);
+    if (rc) return rc;
+    const double bnorm = sqrt(bb);
+    auto true_rel = [&](double* out) -> int {
+      int r = gll_pass(P, false, y, nullptr, x, P->dT1.p, s);
+      if (r) return r;
+      r = do_apply_f64(P, P->w_t.p, x, s);
+      if (r) return r;
+      Reducer red{P->red_part.p, P->red_counter.p};
+      SF_HIP(launch_diff_norm2(b, P->w_t.p, P->nfree, red, P->dscal.p, s));
+      double rr = 0.0;
+      SF_HIP(hipMemcpyAsync(&rr, P->dscal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+      SF_HIP(hipStreamSynchronize(s));
+      *out = bnorm > 0 ? sqrt(rr) / bnorm : sqrt(rr);
+      return SF_OK;
+    };
+    bool fresh = false;  // rel is that of the current y
+    CgCheck check = [&](double res2, double* stop_rr) -> int {
+      *stop_rr = 0.0;
+      GllTables own(P);  // flipped back: the apply's own tables
+      int r = true_rel(&rel);
+      if (r) return r;
+      fresh = true;
+      if (rel > rtol && isfinite(rel)) {
+        const double f = 0.8 * rtol / rel;
+        const double g = f > 1e-3 ? f : 1e-3;
+        *stop_rr = res2 * g * g;
+        ++restarts;
+        fresh = false;
+      }
+      return SF_OK;
+    };
+    {
+      GllTables swap(P);
+      rc = cg_f64(P, bg, y, gg > 0 ? rtol * bnorm / sqrt(gg) : rtol, maxit, s, &iters,
+                  &nonfinite, dinv, &check);
+    }
+    if (rc) return rc;
+    if (!fresh) {  // stopped by maxit or a non-finite value: x = T y still to form
+      rc = true_rel(&rel);
+      if (rc) return rc;
+    }
   } else {
     rc = dot_host(P, b, b, &bb, s);
     if (rc) return rc;
@@ -1536,11 +1615,12 @@ int cg_solve(sf_plan* P, const double* b, double* x, double rtol, int maxit,
     if (rc) return rc;
     rel = bb > 0 ? rn / sqrt(bb) : rn;
     if (nonfinite || !(rel > rtol) || iters >= maxit || it == 0) break;
+    ++restarts;
   }
   const double t1 = now_s();
   if (stats) {
     stats->iterations = iters;
-    stats->outer_iterations = 0;
+    stats->outer_iterations = restarts;
     stats->rel_residual = rel;
     stats->converged = (rel <= rtol) ? 1 : 0;
     stats->seconds = t1 - t0;
@@ -2366,8 +2446,9 @@ int sf_cg_mixed(sf_plan* P, const double* b, double* x, double rtol, int maxit,
     if (!gll) rel = rl;
     return r;
   };
+  int restarts = 0;
   if (gll)
-    rc = solve_gll(P, b, x, rtol, maxit, s, run, &iters, &rel, &nonfinite);
+    rc = solve_gll(P, b, x, rtol, maxit, s, run, &iters, &rel, &nonfinite, &restarts);
   else
     rc = run(b, x, rtol, maxit, &iters, &nonfinite);
   if (rc) return rc;
@@ -2421,8 +2502,9 @@ int sf_cg_ir(sf_plan* P, const double* b, double* x, double rtol,
     if (!gll) rel = rl;
     return r;
   };
+  int restarts = 0;
   if (gll)
-    rc = solve_gll(P, b, x, rtol, max_outer, s, run, &outer, &rel, &nonfinite);
+    rc = solve_gll(P, b, x, rtol, max_outer, s, run, &outer, &rel, &nonfinite, &restarts);
   else
     rc = run(b, x, rtol, max_outer, &outer, &nonfinite);
   if (rc) return rc;

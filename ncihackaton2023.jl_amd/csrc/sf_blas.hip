@@ -79,6 +79,23 @@ __global__ void __launch_bounds__(kRedThreads)
   }
 }
 
+// ||b - t||^2 (the true residual's norm without storing it).
+__global__ void __launch_bounds__(kRedThreads)
+    k_diff_norm2(const double* __restrict__ b, const double* __restrict__ t, int64_t n,
+                 Reducer red, double* __restrict__ out) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double d = b[i] - t[i];
+    s += d * d;
+  }
+  s = block_sum(s);
+  if (arrive(red, s)) {
+    const double tot = gather_partials(red);
+    if (threadIdx.x == 0) *out = tot;
+  }
+}
+
 // a . (w a) with an FP32 weight (r.z of the Jacobi preconditioner).
 template <typename V>
 __global__ void __launch_bounds__(kRedThreads)
@@ -541,6 +558,13 @@ hipError_t launch_dot(const V* a, const V* b, int64_t n, Reducer red,
                       double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_dot<V>, dim3(red_grid(n)), dim3(kRedThreads), 0, s, a,
                      b, n, red, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_norm2(const double* b, const double* t, int64_t n, Reducer red,
+                            double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_diff_norm2, dim3(red_grid(n)), dim3(kRedThreads), 0, s, b, t, n, red,
+                     out);
   return hipGetLastError();
 }
 

@@ -173,6 +173,9 @@ template <typename V>
 hipError_t launch_zero_list(V* y, const int32_t* idx, int64_t n, hipStream_t s);
 
 // out = a . b  (FP64 accumulation); result in *out (device)
+// *out = ||b - t||^2
+hipError_t launch_diff_norm2(const double* b, const double* t, int64_t n, Reducer red,
+                            double* out, hipStream_t s);
 template <typename V>
 hipError_t launch_dot(const V* a, const V* b, int64_t n, Reducer red,
                       double* out, hipStream_t s);

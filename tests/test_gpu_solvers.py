@@ -149,8 +149,14 @@ def test_high_order_solvers_every_form(pkg, oracle_mod, cuda, monkeypatch, p, q,
         rel = oracle_rel_residual(oracle_mod, 3, p, q, model, space, mp, kappa, sols[key], bh)
         assert rel < 2e-10, (key, rel)
         assert rel_l2(sols[key], x64) < 1e-6, key
-    it_f, it_u = stats["f64", True]["iterations"], stats["f64", False]["iterations"]
-    assert abs(it_f - it_u) <= max(5, 0.02 * it_u), (it_f, it_u)
+    # fused and unfused steps take the same iterations up to the spread of the
+    # atomics, unless one of them restarted (outer_iterations: the plain CG's
+    # restarts, e.g. a Gauss-Lobatto solve whose first target left the true
+    # residual just above rtol), which adds the few iterations of the restart
+    sf, su = stats["f64", True], stats["f64", False]
+    it_f, it_u = sf["iterations"], su["iterations"]
+    slack = 0.02 if sf["outer_iterations"] == su["outer_iterations"] else 0.08
+    assert abs(it_f - it_u) <= max(5, slack * it_u), (sf, su)
 
 
 @pytest.mark.parametrize("m,p,q,with_kappa", [(3, 2, 4, True), (2, 4, 6, True), (2, 7, 9, False),
@@ -227,16 +233,20 @@ def test_jacobi_pcg(pkg, oracle_mod, cuda, monkeypatch, D, m, p, q, with_kappa):
     if D == 3:
         it_f, it_u = runs["cg"][0]["iterations"], runs["cg_unfused"][0]["iterations"]
         assert abs(it_f - it_u) <= max(5, 0.02 * it_u), (it_f, it_u)
-    # switching the preconditioner off again gives the plain iteration
+    # switching the preconditioner off again gives the plain iteration (the
+    # Gauss-Lobatto one first: its count is far from the plain one; the plain
+    # count itself moves by about 1% with the order of the apply's atomics)
     x2 = cuda.zeros_like(b)
     A = lazy(pkg, mp, space, kappa, monkeypatch, True)
-    s2 = pkg.cg_(x2, A, b, rtol=1e-10, maxiter=200000, Pl="jacobi")
+    s2 = pkg.cg_(x2, A, b, rtol=1e-10, maxiter=200000, Pl="gll" if D == 3 else "jacobi")
     s3 = pkg.cg_(cuda.zeros_like(b), A, b, rtol=1e-10, maxiter=200000, Pl=None)
-    assert abs(s3["iterations"] - s0["iterations"]) <= max(3, 0.01 * s0["iterations"])
+    say("  switched", s2, s3)
+    assert abs(s3["iterations"] - s0["iterations"]) <= max(3, 0.03 * s0["iterations"])
+    if D == 3:
+        assert s2["iterations"] < 0.5 * s0["iterations"]
     with pytest.raises(ValueError):
         pkg.cg_(x2, A, b, Pl="ilu")
     A.free()
-    _ = s2
 
 
 def test_adjoint_fd_64_directions(pkg, oracle_mod, cuda):
