@@ -69,9 +69,12 @@ enum sf_preconditioner {
   SF_PC_JACOBI = 1, /* z = diag(A)^{-1} r: the matrix-free diagonal of A(kappa),
                        inverted once in FP32 and used by every solver form     */
   SF_PC_GLL = 2     /* 3D: Jacobi in the Gauss-Lobatto nodal basis of the same
-                       space, z = T diag(T^T A T)^{-1} T^T r with T the
+                       space, M^{-1} = T diag(T^T A T)^{-1} T^T with T the
                        tensor-product interpolation from the Gauss-Lobatto to
-                       the equispaced nodes (two block-assembled passes)      */
+                       the equispaced nodes.  The solvers run that PCG as
+                       Jacobi CG on T^T A T (the same apply on the tables
+                       B T1, D T1), so it costs no pass per iteration; only
+                       T^T b, T^{-1} x0 and x = T y are formed            */
 };
 
 typedef struct sf_plan sf_plan; /* opaque */
@@ -262,11 +265,12 @@ int sf_adjoint(sf_plan* plan, const double* u, const double* lam,
 int sf_diagonal(sf_plan* plan, double* diag, void* stream);
 
 /* Preconditioner of sf_cg, sf_cg_mixed and sf_cg_ir on this plan
- * (SF_PC_NONE or SF_PC_JACOBI): the Pl keyword of IterativeSolvers.cg!,
- * which the reference's compute_solution leaves at the identity
- * (profiling/drivers.jl:85-95).  The Jacobi vector is built at the next
- * solve and again after any change of the operator (tables, cells,
- * geometry, kappa). */
+ * (SF_PC_NONE, SF_PC_JACOBI, or SF_PC_GLL on 3D plans): the Pl keyword of
+ * IterativeSolvers.cg!, which the reference's compute_solution leaves at
+ * the identity (profiling/drivers.jl:85-95).  Its data (the inverse
+ * diagonal; for GLL also the basis tables) is built at the next solve or
+ * sf_precond_apply and again after any change of the operator (tables,
+ * cells, geometry, kappa). */
 int sf_plan_set_preconditioner(sf_plan* plan, int kind);
 
 /* z = M^{-1} r for the plan's preconditioner (FP64 device vectors, z
@@ -275,9 +279,11 @@ int sf_plan_set_preconditioner(sf_plan* plan, int kind);
  * that IterativeSolvers.cg! applies to its Pl in every iteration. */
 int sf_precond_apply(sf_plan* plan, double* z, const double* r, void* stream);
 
-/* Unpreconditioned CG on the FP64 operator, x used as the initial guess,
- * stop when ||r|| <= rtol ||b||.  Replaces IterativeSolvers.cg!(x, A, b)
- * as called by compute_solution, profiling/drivers.jl:85-95. */
+/* CG on the FP64 operator, x used as the initial guess, stop when the FP64
+ * ||b - A x|| <= rtol ||b||.  Unpreconditioned unless the plan has a
+ * preconditioner (sf_plan_set_preconditioner).  Replaces
+ * IterativeSolvers.cg!(x, A, b) as called by compute_solution,
+ * profiling/drivers.jl:85-95 (with Pl: cg!(x, A, b; Pl = ...)). */
 int sf_cg(sf_plan* plan, const double* b, double* x, double rtol, int maxit,
           sf_solve_stats* stats, void* stream);
 
@@ -305,7 +311,8 @@ int sf_cg_noalloc(sf_plan* plan, const double* b, double* x, double rtol, int ma
  * `inner_precision` (FP32 vectors) to relative tolerance inner_rtol or
  * inner_maxit iterations, FP64 correction x += d.  Stops when the FP64
  * ||b - A x|| <= rtol ||b|| or after max_outer steps.  x is the initial
- * guess.  No reference counterpart. */
+ * guess; the plan's preconditioner applies to the inner CG.  No reference
+ * counterpart. */
 int sf_cg_ir(sf_plan* plan, const double* b, double* x, double rtol,
              int max_outer, int inner_precision, double inner_rtol,
              int inner_maxit, sf_solve_stats* stats, void* stream);
@@ -322,8 +329,8 @@ int sf_cg_ir(sf_plan* plan, const double* b, double* x, double rtol,
  * replaced by the FP64 b - A x; the search direction is kept, so no Krylov
  * information is lost (unlike sf_cg_ir's restarts).  Stops when the FP64
  * ||b - A x|| <= rtol ||b|| or after maxit iterations;
- * stats->outer_iterations counts the replacements.  x is the initial guess.
- * No reference counterpart. */
+ * stats->outer_iterations counts the replacements.  x is the initial guess;
+ * the plan's preconditioner applies.  No reference counterpart. */
 int sf_cg_mixed(sf_plan* plan, const double* b, double* x, double rtol,
                 int maxit, int inner_precision, double delta,
                 int fp64_recurrence, sf_solve_stats* stats, void* stream);
