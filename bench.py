@@ -310,7 +310,7 @@ def assembled_comparator(torch, pkg, threads):
 
 def mode_sweep(args, torch, pkg):
     rows = []
-    for p in range(2, 8):
+    for p in [int(t) for t in args.orders.split(",") if t]:
         q = p + 2
         m = int(round(256 / p))
         model, space, mp, A = build_problem(pkg, 3, m, p, q)
@@ -330,7 +330,9 @@ def mode_sweep(args, torch, pkg):
                "f64_roofline_frac": max(t_hbm, t_flop) / kern,
                "f64_bound": "fp64" if t_flop >= t_hbm else "hbm",
                "f64_tflops": F / kern / 1e12, "f64_GBs": B64 / kern / 1e9}
-        for prec in ("f32", "f32_gbf16", "mfma_f32", "mfma_bf16x3", "mfma_bf16"):
+        for prec in ("f32", "f32_gbf16", "mfma_f32", "mfma_bf16x3", "mfma_bf16", "mfma_f16"):
+            if prec == "mfma_f16" and n != 8:  # the FP16 GEMM apply is built for n = 8
+                continue
             x32 = x.float()
             y32 = torch.empty_like(x32)
             A.prepare(prec)
@@ -652,6 +654,8 @@ def main():
     ap.add_argument("--order", type=int, default=4,
                     help="apply mode: polynomial order (4: the 64^3 Q4 contract workload; "
                          "others: the sweep's mesh of about 16M dofs with q = p + 2)")
+    ap.add_argument("--orders", default="2,3,4,5,6,7",
+                    help="sweep mode: polynomial orders (comma list; q = p + 2)")
     ap.add_argument("--maxit", type=int, default=300000)
     ap.add_argument("--max-outer", type=int, default=50)
     ap.add_argument("--fd-dirs", type=int, default=64,
@@ -663,10 +667,10 @@ def main():
     ap.add_argument("--pl", default=None, choices=[None, "jacobi", "gll"],
                     help="cg / q9 modes: preconditioner of every solve (None: the "
                          "reference's unpreconditioned CG)")
-    ap.add_argument("--inner", default="f32:1e-2,f32_gbf16:1e-2,mfma_bf16x3:1e-2,mfma_bf16:1e-2",
+    ap.add_argument("--inner", default="f32:1e-2,f32_gbf16:1e-2,mfma_f16:1e-2,mfma_bf16x3:1e-2,mfma_bf16:1e-2",
                     help="cg mode: restarted refinement runs, comma list of "
                          "inner_precision:inner_rtol")
-    ap.add_argument("--mixed", default="f32:0.1:f64p32,f32:0.1:f64,f32:0.1,f32_gbf16:0.1:f64,mfma_bf16x3:0.1:f64",
+    ap.add_argument("--mixed", default="f32:0.1:f64p32,f32:0.1:f64,f32:0.1,mfma_f16:0.1:f64,mfma_f16:0.1,f32_gbf16:0.1:f64,mfma_bf16x3:0.1:f64",
                     help="cg mode: reliable-update mixed CG runs, comma list of "
                          "inner_precision:delta[:recurrence f32|f64|f64p32]")
     args = ap.parse_args()

@@ -58,8 +58,12 @@ enum sf_precision {
                             bf16 terms (six MFMAs per tile): FP32-level accuracy */
   SF_PREC_MFMA_BF16X2 = 5, /* 3D: two bf16 terms per operand (three MFMAs per
                             tile): about 16 significant bits                     */
-  SF_PREC_MFMA_F32 = 6     /* 3D: FP32 operands on the matrix cores (four
+  SF_PREC_MFMA_F32 = 6,    /* 3D: FP32 operands on the matrix cores (four
                             v_mfma_f32_16x16x4f32 per tile), G in FP32           */
+  SF_PREC_MFMA_F16 = 7     /* 3D, n = 8 (Q7), q = 8..10: every contraction an
+                            FP16 GEMM over 32 pencils batched across cells
+                            (v_mfma_f32_32x32x8f16, FP32 accumulation), G in
+                            bf16; about 11 significant bits                      */
 };
 
 /* Preconditioner of the device-resident solvers (sf_cg, sf_cg_mixed,
@@ -236,7 +240,7 @@ int sf_apply_host(sf_plan* plan, double* y, const double* x);
 
 /* y = A x with FP32 vectors (device pointers).  precision is SF_PREC_F32,
  * SF_PREC_F32_GBF16, SF_PREC_MFMA_BF16, SF_PREC_MFMA_BF16X2,
- * SF_PREC_MFMA_BF16X3 or SF_PREC_MFMA_F32 (the MFMA forms are 3D only:
+ * SF_PREC_MFMA_BF16X3, SF_PREC_MFMA_F32 or SF_PREC_MFMA_F16 (the MFMA forms are 3D only:
  * SF_ERR_UNSUPPORTED in 2D).  No reference counterpart:
  * the reduced-precision inner operator of the mixed-precision solver. */
 int sf_apply_f32(sf_plan* plan, float* y, const float* x, int precision,

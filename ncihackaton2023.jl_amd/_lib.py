@@ -34,6 +34,7 @@ SF_PREC_MFMA_BF16 = 3
 SF_PREC_MFMA_BF16X3 = 4
 SF_PREC_MFMA_BF16X2 = 5
 SF_PREC_MFMA_F32 = 6
+SF_PREC_MFMA_F16 = 7
 
 SF_PC_NONE = 0
 SF_PC_JACOBI = 1
@@ -43,7 +44,8 @@ PRECONDITIONERS = {None: SF_PC_NONE, "none": SF_PC_NONE, "jacobi": SF_PC_JACOBI,
 
 PRECISIONS = {"f64": SF_PREC_F64, "f32": SF_PREC_F32, "f32_gbf16": SF_PREC_F32_GBF16,
               "mfma_bf16": SF_PREC_MFMA_BF16, "mfma_bf16x2": SF_PREC_MFMA_BF16X2,
-              "mfma_bf16x3": SF_PREC_MFMA_BF16X3, "mfma_f32": SF_PREC_MFMA_F32}
+              "mfma_bf16x3": SF_PREC_MFMA_BF16X3, "mfma_f32": SF_PREC_MFMA_F32,
+              "mfma_f16": SF_PREC_MFMA_F16}
 
 
 class SFError(RuntimeError):

@@ -311,6 +311,11 @@ struct sf_plan {
   DevBuf<uint32_t> Gm;  // bf16 pairs in the layout of the MFMA apply (3D)
   DevBuf<float> Gm32;   // FP32 G in the layout of the FP32-operand MFMA apply
   bool gm32_valid = false;
+  float gmax = 0.f;     // max |G| of Gm (the FP16 GEMM apply's flux scale)
+  // max absolute row sum (forward) and column sum (transposed) of the 1D
+  // tables in use, the Gauss-Lobatto ones included once built: the bounds
+  // behind the FP16 GEMM apply's scales
+  double tab_rowB = 0.0, tab_rowD = 0.0, tab_colB = 0.0, tab_colD = 0.0;
   bool has_cells = false, has_geom = false, has_kappa = false;
   bool g32_valid = false, gbf_valid = false, gm_valid = false;
   DevBuf<int> flag;
@@ -401,6 +406,25 @@ struct sf_plan {
 
 namespace {
 
+// Raise the plan's table bounds to cover the q x n tables B, D.
+void table_bounds(sf_plan* P, const double* B, const double* D) {
+  const int q = P->q, n = P->n;
+  auto bound = [&](const double* T, double* row, double* col) {
+    for (int i = 0; i < q; ++i) {
+      double s = 0.0;
+      for (int a = 0; a < n; ++a) s += fabs(T[i * n + a]);
+      *row = s > *row ? s : *row;
+    }
+    for (int a = 0; a < n; ++a) {
+      double s = 0.0;
+      for (int i = 0; i < q; ++i) s += fabs(T[i * n + a]);
+      *col = s > *col ? s : *col;
+    }
+  };
+  bound(B, &P->tab_rowB, &P->tab_colB);
+  bound(D, &P->tab_rowD, &P->tab_colD);
+}
+
 int upload_tables(sf_plan* P) {
   // device tables: the Q x N table followed by its N x Q transpose (the
   // transposed contractions of the 3D apply read rows of the latter)
@@ -418,6 +442,8 @@ int upload_tables(sf_plan* P) {
     b32[i] = (float)b64[i];
     d32[i] = (float)d64[i];
   }
+  P->tab_rowB = P->tab_rowD = P->tab_colB = P->tab_colD = 0.0;
+  table_bounds(P, P->hB.data(), P->hD.data());
   SF_HIP(P->dB.alloc(2 * tn));
   SF_HIP(P->dD.alloc(2 * tn));
   SF_HIP(P->dW.alloc(q));
@@ -510,7 +536,44 @@ int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
 bool is_reduced(int precision) {
   return precision == SF_PREC_F32 || precision == SF_PREC_F32_GBF16 ||
          precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X2 ||
-         precision == SF_PREC_MFMA_BF16X3 || precision == SF_PREC_MFMA_F32;
+         precision == SF_PREC_MFMA_BF16X3 || precision == SF_PREC_MFMA_F32 ||
+         precision == SF_PREC_MFMA_F16;
+}
+
+// 2^-e with v 2^-e in [0.5, 1) (1 for v = 0): the scale that brings a
+// bound v below 1.
+float pow2_below(double v) {
+  if (!(v > 0.0) || !isfinite(v)) return 1.f;
+  int e = 0;
+  (void)frexp(v, &e);
+  return (float)ldexp(1.0, -e);
+}
+
+// The FP16 GEMM apply's conversion scales from the bounds of each table
+// (row sums forward, column sums transposed) and max |G|.  Scales along the
+// way: U 1 (x scaled per block), gradient paths x: pD pB, y: pB pD,
+// z: pB pB, flux s3, W_x, W_y: s3 tB, W_z: s3 tD, S_D: s3 tB^2,
+// S_B: s3 tB tD / 2.
+GemmScales gemm_scales(const sf_plan* P) {
+  GemmScales g;
+  const double rB = P->tab_rowB, rD = P->tab_rowD, cB = P->tab_colB, cD = P->tab_colD;
+  const double pB = pow2_below(rB), pD = pow2_below(rD), tB = pow2_below(cB),
+               tD = pow2_below(cD);
+  // |raw gradient| / its path scale: x, y: rB / (pD pB), z: rD / pB^2
+  const double gb = fmax(rB / (pD * pB), rD / (pB * pB));
+  const double s3 = pow2_below(3.0 * (double)P->gmax * gb);
+  g.pB = (float)pB;
+  g.pD = (float)pD;
+  g.tB = (float)tB;
+  g.tD = (float)tD;
+  g.cx = (float)(s3 / (pD * pB));
+  g.cy = (float)(s3 / (pB * pD));
+  g.cz = (float)(s3 / (pB * pB));
+  g.hD = (float)(0.5 * tD);  // the y term (scale s3 tB) times tD / 2
+  g.hB = (float)(0.5 * tB);  // the z term (scale s3 tD) times tB / 2
+  g.bx = (float)(2.0 * tB / tD);  // S_B (scale s3 tB tD / 2) to S_D's s3 tB^2
+  g.vD = (float)(1.0 / (s3 * tB * tB));
+  return g;
 }
 
 int prepare(sf_plan* P, int precision) {
@@ -529,14 +592,23 @@ int prepare(sf_plan* P, int precision) {
     return SF_OK;
   }
   if (precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X2 ||
-      precision == SF_PREC_MFMA_BF16X3) {
+      precision == SF_PREC_MFMA_BF16X3 || precision == SF_PREC_MFMA_F16) {
     if (P->dim != 3)
       return fail(SF_ERR_UNSUPPORTED, "the MFMA apply is 3D only (dim=%d)", P->dim);
+    if (precision == SF_PREC_MFMA_F16 && !gemm_supported(P->n, P->q))
+      return fail(SF_ERR_UNSUPPORTED,
+                  "the FP16 GEMM apply is built for n = 8, q = 8..10 (got n=%d q=%d)", P->n,
+                  P->q);
     if (!P->gm_valid) {
       SF_HIP(P->Gm.alloc(P->ncell * P->nqc * 3));
       SF_HIP(launch_g3_to_mfma(P->active_G(), P->Gm.p, P->q, P->ncell,
                                P->cell_block.p, P->bcell.p, nullptr));
-      SF_HIP(hipDeviceSynchronize());
+      SF_HIP(P->flag.alloc(1));
+      SF_HIP(launch_absmax_bf16pairs(P->Gm.p, P->ncell * P->nqc * 3, (unsigned*)P->flag.p,
+                                     nullptr));
+      unsigned bits = 0;
+      SF_HIP(hipMemcpy(&bits, P->flag.p, sizeof(unsigned), hipMemcpyDeviceToHost));
+      memcpy(&P->gmax, &bits, sizeof(float));
       P->gm_valid = true;
     }
     return SF_OK;
@@ -610,6 +682,9 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
                         P->dB32.p, P->dD32.p, P->ncell, P->blocks(), 0, s);
   } else if (precision == SF_PREC_MFMA_F32) {
     e = apply_mfma(P->n, P->q, 0, x, y, P->Gm32.p, P->dB32.p, P->dD32.p, P->blocks(), s);
+  } else if (precision == SF_PREC_MFMA_F16) {
+    e = apply_gemm(P->n, P->q, x, y, P->Gm.p, P->dB32.p, P->dD32.p, P->blocks(),
+                   gemm_scales(P), s);
   } else if (precision == SF_PREC_MFMA_BF16 || precision == SF_PREC_MFMA_BF16X2 ||
              precision == SF_PREC_MFMA_BF16X3) {
     const int split = precision == SF_PREC_MFMA_BF16 ? 1 : (precision == SF_PREC_MFMA_BF16X2 ? 2 : 3);
@@ -979,6 +1054,7 @@ int ensure_pc(sf_plan* P, hipStream_t s, const float** dinv, bool* gll = nullptr
       SF_HIP(hipMemcpy(P->dT1i.p, Ti.data(), n * n * 8, hipMemcpyHostToDevice));
       SF_HIP(hipMemcpy(tb.p, bg.data(), q * n * 8, hipMemcpyHostToDevice));
       SF_HIP(hipMemcpy(td.p, dg.data(), q * n * 8, hipMemcpyHostToDevice));
+      table_bounds(P, bg.data(), dg.data());
       // the GLL tables in the apply's layout (q x n table, then its transpose)
       {
         const int64_t tn = (int64_t)q * n;

@@ -80,6 +80,26 @@ hipError_t launch_g3_to_mfma(const double* G, uint32_t* out, int q,
                              int64_t ncell, const int32_t* cell_block,
                              const int64_t* bcell, hipStream_t s);
 
+// Scales of the FP16 conversions of apply_gemm (powers of two; each
+// brings the bound of the converted value below 1, per table).
+struct GemmScales {
+  float pB, pD;      // forward contraction outputs (1 / row-sum bound)
+  float tB, tD;      // transposed contraction outputs (1 / column-sum bound)
+  float cx, cy, cz;  // gradient components to the flux's scale
+  float hB, hD;      // the two terms of the transposed y pass to one scale
+  float bx;          // the x pass's B path to the scale of its D path
+  float vD;          // that scale back to the units of x
+};
+// 3D at n = 8, q in [8, 10]: the contractions as FP16 GEMMs on the matrix
+// cores over pencils batched across cells (v_mfma_f32_32x32x8f16), G as
+// the bf16 pairs of apply_mfma; y zeroed by the caller.
+bool gemm_supported(int n, int q);
+hipError_t apply_gemm(int n, int q, const float* x, float* y, const uint32_t* Gm,
+                      const float* tB, const float* tD, const BlockArgs* k,
+                      const GemmScales& sc, hipStream_t s);
+// *out = bit pattern of max |v| over n bf16 pairs (device, stream ordered)
+hipError_t launch_absmax_bf16pairs(const uint32_t* a, int64_t n, unsigned* out, hipStream_t s);
+
 // ---- adjoint (sf_adjoint.hip) ----
 // dJdk[cell][qp] = grad(lam) . G grad(u);  ybar += A(kappa) lam
 // 3D reads G through the block-interleaved layout (cell_block, bcell).
