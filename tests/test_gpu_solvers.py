@@ -459,7 +459,8 @@ def test_shuffled_cell_order(pkg, oracle_mod, cuda, monkeypatch):
         x0, x1 = cuda.zeros_like(b), cuda.zeros_like(b)
         s0 = pkg.cg_(x0, A0, b, rtol=1e-10, maxiter=20000, Pl="gll")
         s1 = pkg.cg_(x1, As, b, rtol=1e-10, maxiter=20000, Pl="gll")
-        assert s1["converged"] == 1 and abs(s1["iterations"] - s0["iterations"]) <= 2
+        assert s1["converged"] == 1
+        assert abs(s1["iterations"] - s0["iterations"]) <= max(2, 0.03 * s0["iterations"])
         assert rel_l2(x1.cpu().numpy(), x0.cpu().numpy()) < 1e-8
         A0.free()
         As.free()

@@ -59,7 +59,11 @@ def test_cg_noalloc_matches_cg(pkg, oracle_mod, cuda, Pl):
         xr = cuda.zeros_like(b)
         sr = pkg.cg_(xr, A, b, rtol=1e-10, maxiter=20000, Pl=Pl)
         say("  noalloc", sn, "cg", sr)
-        assert sn["converged"] == 1 and abs(sn["iterations"] - sr["iterations"]) <= 2
+        # to convergence the counts may differ a little: the apply's FP64
+        # atomics leave the summation order free, and the stop (a check of
+        # the true residual, continued when above rtol) amplifies that
+        assert sn["converged"] == 1
+        assert abs(sn["iterations"] - sr["iterations"]) <= max(2, 0.03 * sr["iterations"])
         bh = b.cpu().numpy()
         r = bh - oracle_mod.apply(3, p + 1, q, space.cell_dofs, model.cell_vertices(),
                                   xn.cpu().numpy(), mp.tables, kappa=kappa)
