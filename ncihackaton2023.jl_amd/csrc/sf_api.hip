@@ -1367,11 +1367,22 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
   if (rc) return rc;
   const int chunk = 8;
   Progress prog("mixed cg");
+  // 3D FP32 fused-geometry operator: p.Ap from the apply, then the FP32
+  // forms of the FP64 fused step (resid, xdir)
+  const bool fused = P->npart > 0 && P->w_part.p && precision == SF_PREC_F32 && fused_f32(P);
   for (;;) {
     while (!h.done && h.iters < maxit) {
       prog.tick(h.iters, (bnorm > 0 ? rn / bnorm : rn) * sqrt(h.res2));
       const int todo = (maxit - h.iters) < chunk ? (maxit - h.iters) : chunk;
       for (int k = 0; k < todo; ++k) {
+        if (fused) {
+          rc = do_apply_f32(P, q, p, precision, s, true, P->w_part.p);
+          if (rc) return rc;
+          SF_HIP(launch_cg_alpha_parts(P->w_part.p, P->npart, red, P->scal.p, s));
+          SF_HIP(launch_cg_resid<float>(r, q, P->nfree, red, P->scal.p, s, dinv));
+          SF_HIP(launch_cg_xdir<float>(xl, p, r, q, P->nfree, P->scal.p, s, dinv));
+          continue;
+        }
         rc = do_apply_f32(P, q, p, precision, s, true);
         if (rc) return rc;
         SF_HIP(launch_cg_alpha<float>(p, q, P->nfree, red, P->scal.p, s));
@@ -1422,6 +1433,7 @@ int cg_reliable(sf_plan* P, int precision, const double* b, double* x,
     h.rr = rz;
     h.stop_rr = t * t;
     h.done = 0;
+    h.xpend = 0;
     rc = write_scalars(P, &h, s);
     if (rc) return rc;
   }
