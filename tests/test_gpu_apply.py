@@ -302,3 +302,33 @@ def test_gridap_convention_cells_and_local_permutation(pkg, oracle_mod, cuda):
     assert L.sf_apply_host(h, y.ctypes.data, x.ctypes.data) == 0
     L.sf_plan_destroy(h)
     assert rel_l2(y, gpu_apply(cuda, A, x)) < 1e-15
+
+
+def test_apply_kernel_timing(pkg, cuda):
+    """sf_plan_set_timing / sf_plan_read_timing: one duration per apply
+    kernel while enabled (FP64 and reduced precision), none while disabled,
+    the record cleared by a read; each duration is positive and no longer
+    than the host-timed span of the applies."""
+    import time
+    say("apply kernel timing")
+    _, space, _, A = make(pkg, 3, 6, 4, 6)
+    x = cuda.from_numpy(pkg.uniform(0, space.n_free, -1, 1)).cuda()
+    y = cuda.empty_like(x)
+    x32, y32 = x.float(), cuda.empty_like(x.float())
+    A.mul(y, x)  # set-up outside the record
+    cuda.cuda.synchronize()
+    A.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        A.mul(y, x)
+    A.mul_reduced(y32, x32, "f32")
+    cuda.cuda.synchronize()
+    span_ms = (time.perf_counter() - t0) * 1e3
+    A.timing(False)
+    A.mul(y, x)  # not recorded
+    ms = A.read_timing()
+    say("  kernel ms", ms, "span ms", span_ms)
+    assert len(ms) == 4
+    assert all(0.0 < m <= span_ms for m in ms)
+    assert A.read_timing() == []  # the read cleared the record
+    A.free()
