@@ -315,7 +315,8 @@ def test_apply_kernel_timing(pkg, cuda):
     x = cuda.from_numpy(pkg.uniform(0, space.n_free, -1, 1)).cuda()
     y = cuda.empty_like(x)
     x32, y32 = x.float(), cuda.empty_like(x.float())
-    A.mul(y, x)  # set-up outside the record
+    A.mul(y, x)  # set-up and first launches outside the record
+    A.mul_reduced(y32, x32, "f32")
     cuda.cuda.synchronize()
     A.timing(True)
     t0 = time.perf_counter()
