@@ -95,8 +95,9 @@ typedef struct sf_plan_info {
   int launches_per_apply;  /* kernel launches of one apply (besides the fill) */
   int cells_reordered;     /* 3D: 1 when the plan keeps its cells in an order
                               built from the connectivity (the caller's order
-                              gave more blocks); per-cell arrays still use
-                              the caller's order at the boundary            */
+                              kept fewer neighbours in consecutive cells);
+                              per-cell arrays still use the caller's order
+                              at the boundary                               */
 } sf_plan_info;
 
 typedef struct sf_solve_stats {

@@ -701,23 +701,25 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
 
 // Block structure of the 3D apply from the tensor-ordered, 0-based cell
 // dofs h (negative = Dirichlet):
-//  * blocks of at most eb consecutive cells (eb = the kernel's cells per
-//    workgroup): the cell sequence is cut where two consecutive cells share
-//    no free dof (e.g. at the end of a mesh row) and every run is split into
-//    near-equal blocks;
+//  * blocks of eb consecutive cells (eb = the kernel's cells per workgroup;
+//    the last block may hold fewer).  A block may span the end of a mesh
+//    row: the kernel computes all eb cell slots of a workgroup whether or
+//    not they hold a cell, so cutting at row ends (e.g. a 64-cell row at
+//    eb = 7 as 10 blocks of 6-7) left up to 9% of the slots empty, while a
+//    block over two rows only lists a few more distinct dofs;
 //  * per block, its distinct free dofs, sorted, the ones no other block
 //    touches first (the kernel stores those and adds the shared ones with
 //    atomics);
 //  * per cell dof, its index in the block's list (uint16, 0xffff =
 //    Dirichlet); per cell, its block.
-std::vector<int64_t> block_starts(const std::vector<int32_t>& h, int64_t E, int64_t nl, int eb);
+std::vector<int64_t> block_starts(int64_t E, int eb);
 
 int build_blocks(sf_plan* P, const std::vector<int32_t>& h) {
   int nt = 0, eb = 0;
   apply_shape_f64(3, P->n, P->q, &nt, &eb);
   if (eb <= 0) return fail(SF_ERR_UNSUPPORTED, "no kernel for n=%d q=%d", P->n, P->q);
   const int64_t E = P->ncell, nl = P->nloc, nf = P->nfree;
-  const std::vector<int64_t> bcell = block_starts(h, E, nl, eb);
+  const std::vector<int64_t> bcell = block_starts(E, eb);
   const int64_t nb = (int64_t)bcell.size() - 1;
   if (nb > 0x7fffffffLL) return fail(SF_ERR_ARG, "too many cell blocks");
   // distinct sorted dofs per block
@@ -805,19 +807,26 @@ const T* to_plan_order(const sf_plan* P, const T* src, int64_t width, std::vecto
   return tmp.data();
 }
 
-// Starts of the runs of consecutive cells (tensor-ordered 0-based dofs h,
-// cell c at rows order[c]) that share a free dof with their predecessor,
-// each run split into near-equal blocks of at most eb cells; the last entry
-// is E.  This is the apply's block structure for that cell order.
-std::vector<int64_t> block_starts(const std::vector<int32_t>& h, int64_t E, int64_t nl, int eb) {
+// Starts of the apply's cell blocks: eb consecutive cells each, the last
+// entry E.
+std::vector<int64_t> block_starts(int64_t E, int eb) {
   std::vector<int64_t> bcell;
+  for (int64_t c = 0; c < E; c += eb) bcell.push_back(c);
+  bcell.push_back(E);
+  return bcell;
+}
+
+// How well a cell order keeps neighbours together: the number of blocks of
+// at most eb cells when the sequence (tensor-ordered 0-based dofs h) is cut
+// wherever two consecutive cells share no free dof and every run is split
+// into near-equal blocks.  A row-wise order is within a few per cent of
+// ceil(E / eb); an order whose consecutive cells are not neighbours gives
+// about one block per cell.
+int64_t run_block_count(const std::vector<int32_t>& h, int64_t E, int64_t nl, int eb) {
+  int64_t count = 0;
   std::vector<int32_t> prev, cur;
   int64_t run0 = 0;
-  auto close_run = [&](int64_t r0, int64_t r1) {
-    const int64_t len = r1 - r0;
-    const int64_t nbk = (len + eb - 1) / eb;
-    for (int64_t k = 0; k < nbk; ++k) bcell.push_back(r0 + (k * len) / nbk);
-  };
+  auto close_run = [&](int64_t r0, int64_t r1) { count += (r1 - r0 + eb - 1) / eb; };
   for (int64_t c = 0; c < E; ++c) {
     cur.clear();
     for (int64_t i = c * nl; i < (c + 1) * nl; ++i)
@@ -838,8 +847,7 @@ std::vector<int64_t> block_starts(const std::vector<int32_t>& h, int64_t E, int6
     prev.swap(cur);
   }
   if (E > 0) close_run(run0, E);
-  bcell.push_back(E);
-  return bcell;
+  return count;
 }
 
 // A cell order built from the connectivity alone (3D hexahedra, dofs
@@ -2063,18 +2071,20 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
     }
   }
   P->pc_valid = false;
-  // 3D: blocks follow the cell order, so a caller order whose consecutive
-  // cells are not neighbours (any numbering but a row-wise one) would
-  // leave one cell per block.  The plan then keeps its cells in the order
-  // built from the connectivity when that gives fewer blocks, and maps
-  // per-cell inputs and outputs between the two orders.
+  // 3D: blocks are runs of consecutive cells, so a caller order whose
+  // consecutive cells are not neighbours (any numbering but a row-wise one)
+  // would give blocks whose cells share no dof: each would gather and add
+  // eb n^3 dofs, every one through an atomic.  The plan then keeps its cells
+  // in the order built from the connectivity when that order keeps more
+  // neighbours together (fewer run-split blocks), and maps per-cell inputs
+  // and outputs between the two orders.
   P->perm.clear();
   P->corig.release();
   if (P->dim == 3 && P->ncell > 1) {
     int nt = 0, eb = 0;
     apply_shape_f64(3, P->n, P->q, &nt, &eb);
     const int64_t E = P->ncell;
-    const int64_t nb_caller = (int64_t)block_starts(h, E, nl, eb).size() - 1;
+    const int64_t nb_caller = run_block_count(h, E, nl, eb);
     // (a row-wise order is within a few per cent of the minimum count: the
     // connectivity order is only worth building when it is well above)
     if (eb > 0 && nb_caller > 5 * ((E + eb - 1) / eb) / 4) {
@@ -2083,7 +2093,7 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
       for (int64_t i = 0; i < E; ++i)
         std::copy(h.begin() + (int64_t)ord[i] * nl, h.begin() + ((int64_t)ord[i] + 1) * nl,
                   hp.begin() + i * nl);
-      const int64_t nb_conn = (int64_t)block_starts(hp, E, nl, eb).size() - 1;
+      const int64_t nb_conn = run_block_count(hp, E, nl, eb);
       if (nb_conn < nb_caller) {
         h.swap(hp);
         P->perm.swap(ord);
