@@ -12,7 +12,8 @@ apply) and the CPU oracle timed on the host cores.
 
 Other modes (one JSON line each): --mode sweep (Q2..Q7 at ~16M dofs, FP64 and
 FP32), --mode cg (Q7 32^3: FP64 CG vs mixed-precision refinement to 1e-10),
---mode adjoint (Q5 48^3 kappa-gradient kernel).
+--mode adjoint (Q5 48^3 kappa-gradient kernel), --mode q9 (24^3 Q9 solves and
+adjoint), --mode ref2d (the reference's own 2D 1024^2 Q4-Q7 benchmark).
 """
 import argparse
 import json
@@ -629,6 +630,67 @@ def mode_q9(args, torch, pkg):
     return res
 
 
+# The reference's own GPU figures (test/my_plot.jl:4-7): seconds per apply of
+# its CUDA.jl kernel on the 2D unit square, 1024^2 Cartesian cells, orders 4-7
+# with Gauss degree 2(p - 1) (test/benchmark_v3.jl:37-49), on the V100 of its
+# job scripts, and its one-core CPU sum-factorized mul!.
+REF2D_GPU_S = {4: 0.003, 5: 0.0051, 6: 0.00805, 7: 0.014}
+REF2D_CPU_S = {4: 0.5, 5: 0.74, 6: 0.91, 7: 1.09}
+
+
+def mode_ref2d(args, torch, pkg):
+    """The reference's benchmark workload: 2D unit square, 1024^2 Cartesian
+    cells, Q4-Q7, Gauss degree 2(p - 1) -> q = p points per direction
+    (Gridap's Gauss rule takes ceil((degree + 1) / 2) points), homogeneous
+    Dirichlet, FP64 y = A u.  The reference's kernel applied only the
+    quadrature weights (a Cartesian mesh needs no Jacobian); this apply reads
+    the geometric factor recomputed from the cell vertices, so it does
+    more work per cell.  Timed like
+    test/benchmark_v3.jl:20-33: a warm-up, then the minimum (and maximum)
+    over the timed applies, next to the mean and median."""
+    rows = []
+    for p in [int(t) for t in args.orders.split(",") if t]:
+        if p not in REF2D_GPU_S:
+            continue
+        q = p
+        model, space, mp, A = build_problem(pkg, 2, 1024, p, q, perturb=0.0)
+        x = torch.from_numpy(pkg.uniform(0, space.n_free, -1, 1)).cuda()
+        y = torch.empty_like(x)
+        wall, kern, det = time_apply(torch, A, x, y, args.steps, args.warmup, detail=True)
+        step = det["step_ms"]
+        nf = space.n_free
+        # roofline of the fused-geometry kernel: its bytes, and the
+        # reference's FLOP model (Utils.jl:33-72, general geometry)
+        B = alg_bytes_fused(nf, space.n_cells, p + 1, 2)
+        F = pkg.count_manual_flops_poisson_matrix_free(space.n_cells, 2, (q,) * 2, (p + 1,) * 2,
+                                                       count_only_fma_ops=False,
+                                                       structured_mesh_opt=False)
+        km = det["kernel_ms"]["median"] * 1e-3
+        t_hbm, t_flop = B / (HBM_PEAK_GBS * 1e9), F / (FP64_PEAK_TFLOPS * 1e12)
+        row = {"order": p, "nquad": q, "cells": space.n_cells, "n_free": nf,
+               "kernel_ms": det["kernel_ms"], "step_ms": step,
+               "gdofs_min_step": nf / (step["min"] * 1e-3) / 1e9,
+               "gdofs_median_step": nf / (step["median"] * 1e-3) / 1e9,
+               "reference_v100_s": REF2D_GPU_S[p],
+               "reference_v100_gdofs": nf / REF2D_GPU_S[p] / 1e9,
+               "speedup_vs_reference_gpu": REF2D_GPU_S[p] / (step["min"] * 1e-3),
+               "reference_cpu_1core_s": REF2D_CPU_S[p],
+               "alg_bytes": B, "alg_flops": F,
+               "hbm_GBs_median": B / km / 1e9, "tflops_median": F / km / 1e12,
+               "roofline_bound": "fp64" if t_flop >= t_hbm else "hbm",
+               "roofline_frac_median": max(t_hbm, t_flop) / km}
+        log(json.dumps(row))
+        rows.append(row)
+        A.free()
+        del x, y
+        torch.cuda.empty_cache()
+    return {"metric": "the reference's 2D benchmark: apply of 1024^2 Q4-Q7 (q = p), FP64",
+            "rows": rows, "unit": "GDOF/s", "n_gpus": 1,
+            "note": "step = zero fill of y + the apply kernel; the reference's times are "
+                    "test/my_plot.jl's hand-entered figures (V100, CUDA.jl gpu_mul_v3!, "
+                    "weights only)"}
+
+
 def mode_copycal(args, torch, pkg):
     """A plain device copy of 1 GiB, three times: the reference dispatch that
     calibrates the FETCH_SIZE / WRITE_SIZE counters (bytes moved are known)."""
@@ -649,13 +711,14 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--mode", default="apply",
-                    choices=["apply", "sweep", "cg", "adjoint", "q9", "copycal"])
+                    choices=["apply", "sweep", "cg", "adjoint", "q9", "ref2d", "copycal"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--order", type=int, default=4,
                     help="apply mode: polynomial order (4: the 64^3 Q4 contract workload; "
                          "others: the sweep's mesh of about 16M dofs with q = p + 2)")
     ap.add_argument("--orders", default="2,3,4,5,6,7",
-                    help="sweep mode: polynomial orders (comma list; q = p + 2)")
+                    help="sweep mode: polynomial orders (comma list; q = p + 2); "
+                         "ref2d mode: orders among 4-7")
     ap.add_argument("--maxit", type=int, default=300000)
     ap.add_argument("--max-outer", type=int, default=50)
     ap.add_argument("--fd-dirs", type=int, default=64,
@@ -684,7 +747,7 @@ def main():
     pkg = __graft_entry__.import_package()
     torch.cuda.set_device(0)
     fn = {"apply": mode_apply, "sweep": mode_sweep, "cg": mode_cg, "adjoint": mode_adjoint,
-          "q9": mode_q9, "copycal": mode_copycal}
+          "q9": mode_q9, "ref2d": mode_ref2d, "copycal": mode_copycal}
     out = fn[args.mode](args, torch, pkg)
     print(json.dumps(out), flush=True)
 

@@ -303,7 +303,7 @@ struct sf_plan {
   DevBuf<double> G, Gk, kappa;
   DevBuf<float> kappa32;  // FP32 copy for the FP32 fused-geometry apply
   bool k32_valid = false;
-  DevBuf<double> X;   // 3D cell vertices (kept for the fused-geometry apply)
+  DevBuf<double> X;   // cell vertices (kept for the fused-geometry apply)
   DevBuf<float> X32, Pts32, W32;  // FP32 copies for the FP32 fused apply
   bool fused_geom = false;
   DevBuf<float> G32;
@@ -319,7 +319,7 @@ struct sf_plan {
   bool has_cells = false, has_geom = false, has_kappa = false;
   bool g32_valid = false, gbf_valid = false, gm_valid = false;
   DevBuf<int> flag;
-  // 3D block structure (block-assembled apply)
+  // block structure of the (block-assembled) apply
   int eb = 0, nblk = 0;
   int64_t npart = 0;  // shares of p.Ap the 3D apply writes: one per wave
   DevBuf<uint16_t> pidx;
@@ -362,7 +362,7 @@ struct sf_plan {
   hipStream_t own_stream = nullptr;
 
   const double* active_G() const { return has_kappa ? Gk.p : G.p; }
-  const BlockArgs* blocks() const { return dim == 3 ? &kargs : nullptr; }
+  const BlockArgs* blocks() const { return &kargs; }
 
   void release_all() {
     dB.release(); dD.release(); dW.release(); dPts.release();
@@ -485,10 +485,19 @@ int clear_flag(sf_plan* P) {
   return SF_OK;
 }
 
-// The 3D fused-geometry applies (vertices kept, FP32 copies made).
+// The 3D fused-geometry applies (vertices kept, FP32 copies made): the
+// forms the fused CG step and the block-assembled adjoint build on.
 bool fused_f64(const sf_plan* P) { return P->fused_geom && P->dim == 3 && P->X.p; }
 bool fused_f32(const sf_plan* P) {
   return P->fused_geom && P->dim == 3 && P->X32.p && (!P->has_kappa || P->k32_valid);
+}
+// Whether the apply recomputes G from the vertices: the 3D forms above, and
+// in 2D without kappa (a 2D kappa scales the stored G).
+bool geo_f64(const sf_plan* P) {
+  return P->dim == 3 ? fused_f64(P) : (P->fused_geom && P->X.p && !P->has_kappa);
+}
+bool geo_f32(const sf_plan* P) {
+  return P->dim == 3 ? fused_f32(P) : (P->fused_geom && P->X32.p && !P->has_kappa);
 }
 
 // Events around one apply kernel when the plan records timings.
@@ -519,14 +528,14 @@ int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
                  int accumulate = 0, bool zeroed = false, double* dotp = nullptr) {
   if (P->nfree == 0) return SF_OK;  // nothing to compute (and x may be empty)
   if (!accumulate && !zeroed) SF_HIP(launch_fill<double>(y, P->nfree, 0.0, s));
-  // the fused-geometry kernel when the plan has vertices (kappa, if set,
-  // read per quadrature point)
-  const bool geo = fused_f64(P);
+  // the fused-geometry kernel when the plan has vertices (3D: kappa, if
+  // set, read per quadrature point)
+  const bool geo = geo_f64(P);
   ApplyTimer timer(P, s);
   hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
                            P->dB.p, P->dD.p, P->ncell, P->blocks(), accumulate, s,
                            geo ? P->X.p : nullptr, P->dPts.p, P->dW.p, dotp,
-                           geo && P->has_kappa ? P->kappa.p : nullptr);
+                           geo && P->dim == 3 && P->has_kappa ? P->kappa.p : nullptr);
   if (e == hipErrorNotSupported)
     return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
   SF_HIP(e);
@@ -614,12 +623,13 @@ int prepare(sf_plan* P, int precision) {
     return SF_OK;
   }
   if (precision == SF_PREC_F32) {
-    if (P->fused_geom && P->dim == 3 && !P->X32.p) {
+    if (P->fused_geom && !P->X32.p) {
       // FP32 vertices and 1D quadrature for the FP32 fused-geometry apply
-      SF_HIP(P->X32.alloc(P->ncell * 24));
+      const int64_t nv = P->ncell * (P->dim == 3 ? 24 : 8);
+      SF_HIP(P->X32.alloc(nv));
       SF_HIP(P->Pts32.alloc(P->q));
       SF_HIP(P->W32.alloc(P->q));
-      SF_HIP(launch_to_f32(P->X.p, P->ncell * 24, P->X32.p, nullptr));
+      SF_HIP(launch_to_f32(P->X.p, nv, P->X32.p, nullptr));
       std::vector<float> pw(2 * P->q);
       for (int i = 0; i < P->q; ++i) {
         pw[i] = (float)P->hPts[i];
@@ -672,11 +682,11 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
   ApplyTimer timer(P, s);
   hipError_t e;
   if (precision == SF_PREC_F32) {
-    const bool geo = fused_f32(P);
+    const bool geo = geo_f32(P);
     e = apply_f32(P->dim, P->n, P->q, x, y, P->cells.p, P->G32.p, P->dB32.p,
                   P->dD32.p, P->ncell, P->blocks(), 0, s, geo ? P->X32.p : nullptr,
                   P->Pts32.p, P->W32.p, dotp,
-                  geo && P->has_kappa ? P->kappa32.p : nullptr);
+                  geo && P->dim == 3 && P->has_kappa ? P->kappa32.p : nullptr);
   } else if (precision == SF_PREC_F32_GBF16) {
     e = apply_f32_gbf16(P->dim, P->n, P->q, x, y, P->cells.p, P->Gbf.p,
                         P->dB32.p, P->dD32.p, P->ncell, P->blocks(), 0, s);
@@ -699,7 +709,7 @@ int do_apply_f32(sf_plan* P, float* y, const float* x, int precision,
   return SF_OK;
 }
 
-// Block structure of the 3D apply from the tensor-ordered, 0-based cell
+// Block structure of the apply from the tensor-ordered, 0-based cell
 // dofs h (negative = Dirichlet):
 //  * blocks of eb consecutive cells (eb = the kernel's cells per workgroup;
 //    the last block may hold fewer).  A block may span the end of a mesh
@@ -716,7 +726,7 @@ std::vector<int64_t> block_starts(int64_t E, int eb);
 
 int build_blocks(sf_plan* P, const std::vector<int32_t>& h) {
   int nt = 0, eb = 0;
-  apply_shape_f64(3, P->n, P->q, &nt, &eb);
+  apply_shape_f64(P->dim, P->n, P->q, &nt, &eb);
   if (eb <= 0) return fail(SF_ERR_UNSUPPORTED, "no kernel for n=%d q=%d", P->n, P->q);
   const int64_t E = P->ncell, nl = P->nloc, nf = P->nfree;
   const std::vector<int64_t> bcell = block_starts(E, eb);
@@ -789,7 +799,7 @@ int build_blocks(sf_plan* P, const std::vector<int32_t>& h) {
   P->max_nu = 0;
   for (int64_t b = 0; b < nb; ++b)
     P->max_nu = std::max<int>(P->max_nu, (int)(off[b + 1] - off[b]));
-  P->npart = nb * (nt / 64);
+  P->npart = P->dim == 3 ? nb * (nt / 64) : 0;  // p.Ap shares: 3D fused step
   P->kargs = BlockArgs{P->pidx.p, P->bdofs.p, P->boff.p, P->bexcl.p, P->bcell.p, (int)nb};
   return SF_OK;
 }
@@ -2113,9 +2123,10 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
     P->X.release();
     P->X32.release();
     P->g32_valid = P->gbf_valid = P->gm_valid = P->gm32_valid = false;
-    int rc = build_blocks(P, h);
-    if (rc) return rc;
   }
+  // 2D: G is stored per cell, independent of the blocks, and stays valid
+  int rc = build_blocks(P, h);
+  if (rc) return rc;
   P->has_cells = true;
   return SF_OK;
   SF_CATCH
@@ -2183,11 +2194,12 @@ int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
     P->has_geom = true;
     rc = finish_geometry(P);
   } while (0);
-  // 3D: the FP64 apply recomputes G from the vertices (SFHIP_FUSED_GEOMETRY=0
-  // reads the stored G instead).  Every form keeps them (24 or 8 values per
-  // cell): new quadrature weights (sf_plan_set_tables) recompute G from them.
+  // The apply recomputes G from the vertices (SFHIP_FUSED_GEOMETRY=0 reads
+  // the stored G instead; in 2D a kappa, which scales the stored G, does
+  // too).  Every form keeps them (24 or 8 values per cell): new quadrature
+  // weights (sf_plan_set_tables) recompute G from them.
   const char* fg = getenv("SFHIP_FUSED_GEOMETRY");
-  P->fused_geom = rc == SF_OK && P->dim == 3 && !(fg && fg[0] == '0');
+  P->fused_geom = rc == SF_OK && !(fg && fg[0] == '0');
   if (rc != SF_OK) X.release();
   return rc;
   SF_CATCH

@@ -573,85 +573,173 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
 }
 
 // ---------------------------------------------------------------------------
-// 2D: per cell  A = 2 tensors N x QP  ([c][y][i]); the column pass works in
-// place (each thread owns one quadrature column i).
+// 2D: block-assembled, the design of the 3D kernel in two dimensions.  A
+// workgroup takes EB consecutive cells: it gathers the block's distinct x
+// values once (sorted ids: coalesced), and per cell one LDS tensor pair
+// [2][N][QP] ([c][y][i]) carries the work between three stages:
+//   1. x-pencils (one thread per node row y): B u, D u at the Q points;
+//   2. y-columns (one thread per quadrature column i): the gradients at the
+//      column's Q points, t = G g, the transposed y contraction, in place;
+//   3. transposed x-pencils: each thread adds its row's N values into the
+//      block's y in LDS (the x values' region, dead after stage 1).
+// The write-back stores the dofs exclusive to the block and adds the
+// shared ones with one atomic each (accumulate: exclusive dofs
+// read-modify-written).
+//
+// GEO: G is recomputed at every quadrature point from the cell's 4 vertices
+// (bilinear map, 8 values per cell instead of 24 Q^2 bytes): along the
+// column of xi = t_i, dx/dxi = E0 + eta dE is linear in eta and dx/deta is
+// constant, so a point costs two FMAs for J, its determinant, one
+// reciprocal and t = (w / |det J|) C^T (C g) with C the cofactor matrix.
+// Otherwise G is read, [cell][entry][k Q + i].
 template <int N, int Q, typename T>
 struct Apply2Shape {
-  static constexpr int NQ = N * Q, QQ = Q * Q;
-  static constexpr int QP = (Q % 2 == 0) ? Q + 1 : Q;
-  static constexpr int SA1 = N * QP;
-  static constexpr int SE = 2 * SA1;
+  static constexpr int NN = N * N, QQ = Q * Q;
+  static constexpr int QP = (Q % 2 == 0) ? Q + 1 : Q;  // odd row stride
+  static constexpr int SA1 = N * QP;                   // one tensor [y][i]
+  static constexpr int SE = 2 * SA1;                   // per cell
   static constexpr int MAXT = cmax(N, Q);
-  static constexpr int EB = cmax(1, 256 / MAXT);
+  // cells per workgroup: one task per thread in the wider stage, 256 lanes,
+  // within 64 KiB of FP64 LDS (the same EB for every T: FP64 and FP32
+  // applies share the blocks)
+  static constexpr int EB_T = cmax(1, 256 / MAXT);
+  static constexpr int EB_L = 65536 / ((SE + NN + 8) * 8);
+  static constexpr int EB = EB_T < EB_L ? EB_T : EB_L;
   static constexpr int NT = round_up(EB * MAXT, 64);
-  static constexpr int TK = 1;  // table rows per scalar-load group (G loads share the SGPRs)
+  static constexpr int RX = EB * NN;  // distinct dofs of a block, at most
+  static constexpr int KU = (RX + NT - 1) / NT;
+  static constexpr int TK = 1;  // table rows per scalar-load group
+  static constexpr int LDS_BYTES = (EB * SE + RX + EB * 8) * (int)sizeof(T);
+  static_assert(LDS_BYTES <= 65536, "apply2d: LDS per workgroup above 64 KiB");
+  static_assert(NT >= EB * MAXT, "one task per thread per stage");
 };
 
-template <int N, int Q, typename T, typename TG>
+template <int N, int Q, typename T, typename TG, bool GEO>
 __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
     k_apply2d(const T* __restrict__ x, T* __restrict__ y,
-              const int32_t* __restrict__ cells, const TG* __restrict__ G,
-              const T* __restrict__ tB, const T* __restrict__ tD,
-              int64_t ncell, int nblk) {
+              const uint16_t* __restrict__ pidx,
+              const int32_t* __restrict__ bdofs,
+              const int64_t* __restrict__ boff,
+              const int32_t* __restrict__ bexcl,
+              const int64_t* __restrict__ bcell, int nblk,
+              const TG* __restrict__ G, const T* __restrict__ tB,
+              const T* __restrict__ tD, const T* __restrict__ Xv,
+              const T* __restrict__ tP, const T* __restrict__ tW, int accumulate) {
   using S = Apply2Shape<N, Q, T>;
-  __shared__ T smem[S::EB * S::SE];
-  const int blk = xcd_remap(blockIdx.x, nblk);
-  const int64_t base = (int64_t)blk * S::EB;
+  __shared__ T smem[S::EB * S::SE + S::RX + (GEO ? S::EB * 8 : 0)];
+  T* const regA = smem;                    // per cell [2][N][QP]
+  T* const regX = smem + S::EB * S::SE;    // the block's x values, later y
+  T* const regV = regX + S::RX;            // GEO: the block's cell vertices
+  const int b = xcd_remap(blockIdx.x, nblk);
+  const int64_t base = bcell[b];
+  const int ncb = (int)(bcell[b + 1] - base);  // cells in this block, <= EB
   const int tid = threadIdx.x;
+  const int64_t d0 = boff[b];
+  const int nu = (int)(boff[b + 1] - d0);
+  const int nexcl = bexcl[b];
+  const int32_t* __restrict__ dofs = bdofs + d0;
 
-  static_assert(S::NT >= S::EB * S::MAXT, "one task per thread per stage");
+  // ---- prologue: every global read of the block, issued at once ---------
+  int32_t dv[S::KU];
+  T xv[S::KU];
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    dv[k] = i < nu ? __builtin_nontemporal_load(dofs + i) : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) xv[k] = x[dv[k] >= 0 ? dv[k] : 0];
 
-  // stage 1: gather + x-pencils (one per node row y); ids stay in registers
-  int ids[N];
+  // x-pencil thread: cell e1, node row r1 (local dofs r1 N .. r1 N + N - 1)
+  uint16_t li[N];
   const bool own1 = tid < S::EB * N;
   const int e1 = tid / N, r1 = tid - e1 * N;
-  const int64_t ge1 = base + e1;
-  const bool act1 = own1 && ge1 < ncell;
+  const bool act1 = own1 && e1 < ncb;
+  if (act1) {
+    const uint16_t* lp = pidx + (base + e1) * S::NN + r1 * N;
+#pragma unroll
+    for (int j = 0; j < N; ++j) li[j] = __builtin_nontemporal_load(lp + j);
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) li[j] = kNoDof;
+  }
+  // y-column thread: cell e2, quadrature column i2 (xi = t_i2)
+  const bool own2 = tid < S::EB * Q;
+  const int e2 = tid / Q, i2 = tid - e2 * Q;
+  const bool act2 = own2 && e2 < ncb;
+  T gq[GEO ? 1 : 3][GEO ? 1 : Q];
+  if constexpr (GEO) {
+    for (int t = tid; t < ncb * 8; t += S::NT)
+      regV[t] = __builtin_nontemporal_load(Xv + base * 8 + t);
+  } else {
+    const TG* g = G + (base + (act2 ? e2 : 0)) * 3 * S::QQ + i2;
+#pragma unroll
+    for (int k = 0; k < Q; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        gq[c][k] = act2 ? load_g<TG, T>(g + c * S::QQ + k * Q) : T(0);
+  }
+
+  // ---- stage 0: the block's distinct x values into LDS --------------------
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    if (i < nu) regX[i] = xv[k];
+  }
+  __syncthreads();
+
+  // ---- stage 1: x-pencils: Bx u, Dx u -------------------------------------
   if (own1) {
     T u[N];
-    if (act1) {
-      const int32_t* cid = cells + (ge1 * N + r1) * N;
 #pragma unroll
-      for (int j = 0; j < N; ++j) ids[j] = cid[j];
-#pragma unroll
-      for (int j = 0; j < N; ++j) u[j] = x[ids[j] >= 0 ? ids[j] : 0];
-#pragma unroll
-      for (int j = 0; j < N; ++j) u[j] = ids[j] >= 0 ? u[j] : T(0);
-    } else {
-#pragma unroll
-      for (int j = 0; j < N; ++j) u[j] = T(0);
-    }
-    T* sA = smem + e1 * S::SE;
+    for (int j = 0; j < N; ++j) u[j] = li[j] != kNoDof ? regX[li[j]] : T(0);
+    T* sA = regA + e1 * S::SE + r1 * S::QP;
     TableRows<T, S::TK> t_(tB, tD);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       t_.at(i);
-      T b = T(0), d = T(0);
+      T bq = T(0), dq = T(0);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        b = fma(t_.b[i * N + j], u[j], b);
-        d = fma(t_.d[i * N + j], u[j], d);
+        bq = fma(t_.b[i * N + j], u[j], bq);
+        dq = fma(t_.d[i * N + j], u[j], dq);
       }
-      sA[r1 * S::QP + i] = b;
-      sA[S::SA1 + r1 * S::QP + i] = d;
+      sA[i] = bq;
+      sA[S::SA1 + i] = dq;
     }
   }
   __syncthreads();
 
-  // stage 2: y-columns: gradients, G, transposed y contraction (in place)
-  if (tid < S::EB * Q) {
-    const int t = tid;
-    const int e = t / Q, i = t - e * Q;
-    const int64_t ge = base + e;
-    T* sA = smem + e * S::SE;
+  // the x values are dead: their region accumulates the block's y
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    if (i < nu) regX[i] = T(0);
+  }
+
+  // ---- stage 2: y-columns: gradients, G, transposed y contraction --------
+  if (own2) {
+    T* sA = regA + e2 * S::SE + i2;
     T a[N], d[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      a[j] = sA[j * S::QP + i];
-      d[j] = sA[S::SA1 + j * S::QP + i];
+      a[j] = sA[j * S::QP];
+      d[j] = sA[S::SA1 + j * S::QP];
     }
-    const bool valid = ge < ncell;
-    const TG* g = G + ge * 3 * S::QQ + i;
+    // GEO: J[c][0] = dx_c/dxi = E0[c] + eta dE[c], J[c][1] = dx_c/deta = H[c]
+    // (vertex v = b0 + 2 b1, 2 coordinates)
+    T E0[2], dE[2], H[2], wi = T(0);
+    if constexpr (GEO) {
+      const T xi = tP[i2];
+      wi = tW[i2];
+      const T* V = regV + (act2 ? e2 : 0) * 8;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        E0[c] = V[1 * 2 + c] - V[0 * 2 + c];
+        dE[c] = (V[3 * 2 + c] - V[2 * 2 + c]) - E0[c];
+        H[c] = (T(1) - xi) * (V[2 * 2 + c] - V[0 * 2 + c]) + xi * (V[3 * 2 + c] - V[1 * 2 + c]);
+      }
+    }
     T tx[Q], ty[Q];
     TableRows<T, S::TK> t_(tB, tD);
 #pragma unroll
@@ -660,53 +748,78 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
       T gx = T(0), gy = T(0);
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        gx = fma(t_.b[k * N + j], d[j], gx);
-        gy = fma(t_.d[k * N + j], a[j], gy);
+        gx = fma(t_.b[k * N + j], d[j], gx);  // d/dxi: D in x, B in y
+        gy = fma(t_.d[k * N + j], a[j], gy);  // d/deta: B in x, D in y
       }
-      T g0 = T(0), g1 = T(0), g2 = T(0);
-      if (valid) {
-        g0 = load_g<TG, T>(g + 0 * S::QQ + k * Q);
-        g1 = load_g<TG, T>(g + 1 * S::QQ + k * Q);
-        g2 = load_g<TG, T>(g + 2 * S::QQ + k * Q);
+      if constexpr (GEO) {
+        const T et = tP[k];
+        const T j00 = fma(et, dE[0], E0[0]), j10 = fma(et, dE[1], E0[1]);
+        const T j01 = H[0], j11 = H[1];
+        const T det = j00 * j11 - j01 * j10;
+        const T sc = act2 ? wi * tW[k] * recip_pos(fabs(det)) : T(0);
+        // C = [[j11, -j10], [-j01, j00]]; t = sc C^T (C g)
+        const T v0 = j11 * gx - j10 * gy;
+        const T v1 = j00 * gy - j01 * gx;
+        tx[k] = sc * (j11 * v0 - j01 * v1);
+        ty[k] = sc * (j00 * v1 - j10 * v0);
+      } else {
+        tx[k] = gq[0][k] * gx + gq[1][k] * gy;
+        ty[k] = gq[1][k] * gx + gq[2][k] * gy;
       }
-      tx[k] = g0 * gx + g1 * gy;
-      ty[k] = g1 * gx + g2 * gy;
     }
+    TableRows<T, S::TK> tt_(tB + Q * N, tD + Q * N);  // transposed tables
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      t_.at(j);
+      tt_.at(j);
       T rx = T(0), ry = T(0);
 #pragma unroll
       for (int k = 0; k < Q; ++k) {
-        rx = fma(t_.b[k * N + j], tx[k], rx);
-        ry = fma(t_.d[k * N + j], ty[k], ry);
+        rx = fma(tt_.b[j * Q + k], tx[k], rx);
+        ry = fma(tt_.d[j * Q + k], ty[k], ry);
       }
-      sA[j * S::QP + i] = rx;            // then D_x^T
-      sA[S::SA1 + j * S::QP + i] = ry;   // then B_x^T
+      sA[j * S::QP] = rx;            // then D_x^T
+      sA[S::SA1 + j * S::QP] = ry;   // then B_x^T
     }
   }
   __syncthreads();
 
-  // stage 3: transposed x-pencils + scatter-add
+  // ---- stage 3: transposed x-pencils, assembled in LDS --------------------
   if (act1) {
-    const T* sA = smem + e1 * S::SE;
+    const T* sA = regA + e1 * S::SE + r1 * S::QP;
     T sx[Q], so[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      sx[i] = sA[r1 * S::QP + i];
-      so[i] = sA[S::SA1 + r1 * S::QP + i];
+      sx[i] = sA[i];
+      so[i] = sA[S::SA1 + i];
     }
-    TableRows<T, S::TK> t_(tB, tD);
+    TableRows<T, S::TK> t_(tB + Q * N, tD + Q * N);  // transposed tables
 #pragma unroll
     for (int m = 0; m < N; ++m) {
       t_.at(m);
       T v = T(0);
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
-        v = fma(t_.d[i * N + m], sx[i], v);
-        v = fma(t_.b[i * N + m], so[i], v);
+        v = fma(t_.d[m * Q + i], sx[i], v);
+        v = fma(t_.b[m * Q + i], so[i], v);
       }
-      if (ids[m] >= 0) atomic_add(y + ids[m], v);
+      if (li[m] != kNoDof) lds_atomic_add(regX + li[m], v);
+    }
+  }
+  __syncthreads();
+
+  // ---- write-back: one global access per distinct dof of the block --------
+#pragma unroll
+  for (int k = 0; k < S::KU; ++k) {
+    const int i = tid + k * S::NT;
+    if (dv[k] < 0) continue;
+    const T v = regX[i];
+    if (i < nexcl) {
+      if (accumulate)
+        y[dv[k]] += v;
+      else
+        y[dv[k]] = v;
+    } else {
+      atomic_add(y + dv[k], v);
     }
   }
 }
@@ -723,8 +836,8 @@ struct ApplyArgs {
   const T* tB;
   const T* tD;
   int64_t ncell;
-  int accumulate;  // 3D: 0 -> y zeroed by the caller, 1 -> y += A x
-  // 3D: cell vertices, 1D points and weights; set -> G recomputed in the
+  int accumulate;  // 0 -> y zeroed by the caller, 1 -> y += A x
+  // cell vertices, 1D points and weights; set -> G recomputed in the
   // kernel (the G argument is then unused)
   const T* Xv = nullptr;
   const T* tP = nullptr;
@@ -736,7 +849,7 @@ struct ApplyArgs {
   const T* kap = nullptr;
 };
 
-// Host-built block structure of the 3D apply (see k_apply3d).
+// Host-built block structure of the apply (see k_apply3d, k_apply2d).
 struct BlockArgs {
   const uint16_t* pidx;
   const int32_t* bdofs;
@@ -781,21 +894,31 @@ hipError_t launch_apply3d_nq(const ApplyArgs<T, TG>& a, const BlockArgs& k,
 }
 
 template <int N, int Q, typename T, typename TG>
-hipError_t launch_apply2d_nq(const ApplyArgs<T, TG>& a, hipStream_t s) {
+hipError_t launch_apply2d_nq(const ApplyArgs<T, TG>& a, const BlockArgs& k, hipStream_t s) {
   using S = Apply2Shape<N, Q, T>;
-  const int64_t nblk64 = (a.ncell + S::EB - 1) / S::EB;
-  if (nblk64 <= 0) return hipSuccess;
-  if (nblk64 > 0x7fffffff) return hipErrorInvalidValue;
-  const int nblk = (int)nblk64;
-  hipLaunchKernelGGL((k_apply2d<N, Q, T, TG>), dim3(nblk), dim3(S::NT), 0, s,
-                     a.x, a.y, a.cells, a.G, a.tB, a.tD, a.ncell, nblk);
+  if (k.nblk <= 0) return hipSuccess;
+  // x.Ax shares and kappa per point: 3D only (a 2D kappa scales the stored G)
+  if (a.dotp || a.kap) return hipErrorInvalidValue;
+  if constexpr (sizeof(TG) == sizeof(T)) {
+    if (a.Xv) {
+      hipLaunchKernelGGL((k_apply2d<N, Q, T, TG, true>), dim3(k.nblk), dim3(S::NT), 0, s,
+                         a.x, a.y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell, k.nblk, a.G,
+                         a.tB, a.tD, a.Xv, a.tP, a.tW, a.accumulate);
+      return hipGetLastError();
+    }
+  } else {
+    if (a.Xv) return hipErrorInvalidValue;
+  }
+  hipLaunchKernelGGL((k_apply2d<N, Q, T, TG, false>), dim3(k.nblk), dim3(S::NT), 0, s,
+                     a.x, a.y, k.pidx, k.bdofs, k.boff, k.bexcl, k.bcell, k.nblk, a.G,
+                     a.tB, a.tD, a.Xv, a.tP, a.tW, a.accumulate);
   return hipGetLastError();
 }
 
 // Returns hipErrorNotSupported for an (n, q) pair outside SF_FOR_EACH_NQ.
-// 3D: block-assembled (k needed); 2D: atomic scatter.  Both add into y:
-// the caller zeroes y for y = A x (3D with accumulate = 0 stores the dofs
-// exclusive to a block instead of adding them).
+// Block-assembled in 2D and 3D (k needed).  The shared dofs are added into
+// y: the caller zeroes y for y = A x (with accumulate = 0 the dofs
+// exclusive to a block are stored, not added).
 template <typename T, typename TG>
 hipError_t launch_apply(int dim, int n, int q, const ApplyArgs<T, TG>& a,
                         const BlockArgs* k, hipStream_t s) {
@@ -811,10 +934,11 @@ hipError_t launch_apply(int dim, int n, int q, const ApplyArgs<T, TG>& a,
         return hipErrorNotSupported;
     }
   } else if (dim == 2) {
+    if (!k) return hipErrorInvalidValue;
     switch (n * 16 + q) {
 #define SF_CASE(N_, Q_) \
   case N_ * 16 + Q_:    \
-    return launch_apply2d_nq<N_, Q_, T, TG>(a, s);
+    return launch_apply2d_nq<N_, Q_, T, TG>(a, *k, s);
       SF_FOR_EACH_NQ(SF_CASE)
 #undef SF_CASE
       default:

@@ -97,6 +97,52 @@ def test_full_blocks_every_kernel(pkg, oracle_mod, cuda):
     say("worst rel. l2 %.2e" % worst)
 
 
+def test_2d_blocks_every_kernel(pkg, oracle_mod, cuda, monkeypatch):
+    """2D meshes of m = cells per block + 3 cells per row, so blocks span
+    row ends and hold up to 64 cells (several vertex loads per thread):
+    every (n, q) of the block-assembled 2D apply -- fused geometry and
+    stored G in FP64, fused FP32, and y0 + A x (exclusive dofs
+    read-modify-written) -- against the oracle."""
+    worst = 0.0
+    for n, q in NQ:
+        p = n - 1
+        _, _, _, A0 = make(pkg, 2, 2, p, q)
+        eb = A0.info()["cells_per_block"]
+        A0.free()
+        m = eb + 3
+        say("2D blocks p=%d q=%d m=%d (%d cells per block)" % (p, q, m, eb))
+        model = pkg.CartesianModel(2, m, perturb=0.2, seed=2)
+        space = pkg.FESpace(model, p)
+        mp = pkg.SumFactorizationMap.with_points(2, p, q)
+        X = model.cell_vertices()
+        x = pkg.uniform(0, space.n_free, -1, 1)
+        y0 = pkg.uniform(1, space.n_free, -1, 1)
+        y_ref = oracle_mod.apply(2, n, q, space.cell_dofs, X, x, mp.tables)
+        xd = cuda.from_numpy(x).cuda()
+        errs = []
+        for fused in (True, False):
+            if not fused:
+                monkeypatch.setenv("SFHIP_FUSED_GEOMETRY", "0")
+            A = pkg.LazyMatrix(mp, space)
+            if not fused:
+                monkeypatch.delenv("SFHIP_FUSED_GEOMETRY")
+            assert A.info()["n_blocks"] == -(-space.n_cells // eb)
+            yd = cuda.full_like(xd, float("nan"))
+            A.mul(yd, xd)
+            ya = cuda.from_numpy(y0.copy()).cuda()
+            A.mul_add(ya, xd)
+            y32 = cuda.full_like(xd.float(), float("nan"))
+            A.mul_reduced(y32, xd.float(), "f32")
+            cuda.cuda.synchronize()
+            errs += [rel_l2(yd.cpu().numpy(), y_ref), rel_l2(ya.cpu().numpy(), y0 + y_ref)]
+            e32 = rel_l2(y32.cpu().numpy().astype(np.float64), y_ref)
+            assert e32 < 1e-5, (p, q, fused, e32)
+            A.free()
+        worst = max(worst, *errs)
+        assert max(errs) < TOL, (p, q, errs)
+    say("worst rel. l2 %.2e" % worst)
+
+
 @pytest.mark.parametrize("p,q,m,perturb", [(1, 3, 4, 0.0), (1, 3, 4, 0.2), (3, 5, 3, 0.2)])
 def test_demo_step_by_step_on_gpu(pkg, oracle_mod, cuda, p, q, m, perturb):
     """test/Demo_StepByStep.jl's case (2D, Q1, 4 x 4 cells, 3 Gauss points,
