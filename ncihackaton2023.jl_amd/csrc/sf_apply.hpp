@@ -97,6 +97,7 @@ struct Apply3Shape {
   // Elsewhere the compiler's own choice was as fast or faster.
   static constexpr int WPE =
       (sizeof(T) == 4 && N == 6 && Q <= 7) ? 5 :
+      (sizeof(T) == 8 && GEO && N == 5 && Q <= 6) ? 4 :
       (sizeof(T) == 8 && (N == 7 || (GEO && ((N == 8 && Q <= 9) || (N == 9 && Q <= 10))))) ? 3
                                                                                             : 1;
   // the apply kernel's LDS strides (SA1 / SB1 / QP above size EB and serve
@@ -116,7 +117,15 @@ struct Apply3Shape {
   static constexpr int RA = EB * CA;
   static constexpr int RX = EB * NNN;
   static constexpr int RC = LC ? cmax(EB * CB, RA + RX) : RA + cmax(EB * CB, RX);
-  static constexpr int LDS_BYTES = RC * (int)sizeof(T);
+  // FP64 fused geometry at N = 5 (Q <= 6) and N = 8 (Q <= 9): the block's
+  // dof ids wait for the write-back in LDS (4 bytes per distinct dof), not
+  // in KU registers.  At N = 5 that and a 4-wave cap (128 VGPRs, 2-3
+  // spilled) give a fourth workgroup per CU (Q4 -4%); at N = 8 it halves
+  // the spills (Q7 -3.5%).  Elsewhere the extra LDS costs a workgroup per
+  // CU (Q3 +8%, Q6 +20%; measured A/B/A/B).
+  static constexpr bool DVL =
+      sizeof(T) == 8 && GEO && ((N == 5 && Q <= 6) || (N == 8 && Q <= 9));
+  static constexpr int LDS_BYTES = RC * (int)sizeof(T) + (DVL ? RX * 4 : 0);
   // distinct dofs of a block handled per thread in the gather / write-back
   static constexpr int KU = (EB * NNN + NT - 1) / NT;
   static_assert(LDS_BYTES <= 65536, "apply3d: LDS per workgroup above 64 KiB");
@@ -242,6 +251,7 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
   static_assert(S::EB == Apply3Shape<N, Q, T>::EB && S::NT == Apply3Shape<N, Q, T>::NT,
                 "apply3d: one block structure for both forms");
   __shared__ T smem[S::RC + (GEO ? S::EB * 24 : 0)];
+  __shared__ int32_t sdof[S::DVL ? S::RX : 1];  // DVL: the block's dof ids
   T* const regV = smem + S::RC;  // GEO: the block's cell vertices
   T* const regA = smem;                               // x / y^T pass out
   T* const regB = S::LC ? smem : smem + S::RA;        // y pass out, z in place
@@ -334,7 +344,10 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
 #pragma unroll
   for (int k = 0; k < S::KU; ++k) {
     const int i = tid + k * S::NT;
-    if (i < nu) regX[i] = xv[k];
+    if (i < nu) {
+      regX[i] = xv[k];
+      if constexpr (S::DVL) sdof[i] = dv[k];
+    }
   }
   __syncthreads();
 
@@ -544,6 +557,13 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
   __syncthreads();
 
   // ---- write-back: one global access per distinct dof of the block --------
+  if constexpr (S::DVL) {
+#pragma unroll
+    for (int k = 0; k < S::KU; ++k) {
+      const int i = tid + k * S::NT;
+      dv[k] = i < nu ? sdof[i] : -1;
+    }
+  }
   T xr[DOT ? S::KU : 1];
   if constexpr (DOT) {
 #pragma unroll
