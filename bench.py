@@ -318,7 +318,7 @@ def mode_sweep(args, torch, pkg):
         n = p + 1
         x = torch.from_numpy(pkg.uniform(0, space.n_free, -1, 1)).cuda()
         y = torch.empty_like(x)
-        wall, kern = time_apply(torch, A, x, y, args.steps, args.warmup)
+        wall, kern, det = time_apply(torch, A, x, y, args.steps, args.warmup, detail=True)
         # fused-geometry FP64 apply: bound by the larger of its bytes and the
         # reference's FLOP model
         B64 = alg_bytes_fused(space.n_free, space.n_cells, n, 3)
@@ -328,7 +328,10 @@ def mode_sweep(args, torch, pkg):
         t_hbm, t_flop = B64 / (HBM_PEAK_GBS * 1e9), F / (FP64_PEAK_TFLOPS * 1e12)
         row = {"order": p, "nquad": q, "m": m, "n_free": space.n_free,
                "f64_gdofs": space.n_free / wall / 1e9, "f64_kernel_ms": kern * 1e3,
+               "f64_kernel_ms_median": det["kernel_ms"]["median"],
+               "f64_kernel_ms_min": det["kernel_ms"]["min"],
                "f64_roofline_frac": max(t_hbm, t_flop) / kern,
+               "f64_roofline_frac_median": max(t_hbm, t_flop) / (det["kernel_ms"]["median"] * 1e-3),
                "f64_bound": "fp64" if t_flop >= t_hbm else "hbm",
                "f64_tflops": F / kern / 1e12, "f64_GBs": B64 / kern / 1e9}
         for prec in ("f32", "f32_gbf16", "mfma_f32", "mfma_bf16x3", "mfma_bf16", "mfma_f16"):
