@@ -615,15 +615,22 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
 template <int N, int Q, typename T>
 struct Apply2Shape {
   static constexpr int NN = N * N, QQ = Q * Q;
-  static constexpr int QP = (Q % 2 == 0) ? Q + 1 : Q;  // odd row stride
+  // row stride of the [y][i] tensors: FP32 pads an even Q to the next odd
+  // length (the rows of consecutive lanes then start in different banks);
+  // FP64 keeps Q -- the padding's LDS cost a workgroup per CU at Q4 and
+  // Q6, and without it those orders were 8% and 17% faster (2-way bank
+  // conflicts in the stage-1 writes; measured A/B/A/B on 1024^2 meshes),
+  // while FP32 lost 3.5% at Q4
+  static constexpr int QPAD = (Q % 2 == 0) ? Q + 1 : Q;
+  static constexpr int QP = sizeof(T) == 8 ? Q : QPAD;
   static constexpr int SA1 = N * QP;                   // one tensor [y][i]
   static constexpr int SE = 2 * SA1;                   // per cell
   static constexpr int MAXT = cmax(N, Q);
   // cells per workgroup: one task per thread in the wider stage, 256 lanes,
-  // within 64 KiB of FP64 LDS (the same EB for every T: FP64 and FP32
-  // applies share the blocks)
+  // within 64 KiB of FP64 LDS at the padded stride (the same EB for every
+  // T: FP64 and FP32 applies share the blocks)
   static constexpr int EB_T = cmax(1, 256 / MAXT);
-  static constexpr int EB_L = 65536 / ((SE + NN + 8) * 8);
+  static constexpr int EB_L = 65536 / ((2 * N * QPAD + NN + 8) * 8);
   static constexpr int EB = EB_T < EB_L ? EB_T : EB_L;
   static constexpr int NT = round_up(EB * MAXT, 64);
   static constexpr int RX = EB * NN;  // distinct dofs of a block, at most
