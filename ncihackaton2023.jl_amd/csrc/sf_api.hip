@@ -472,6 +472,14 @@ int check_ready(const sf_plan* P) {
   return SF_OK;
 }
 
+// A system without free dofs (every dof Dirichlet) is solved as it stands:
+// the solvers return at once, converged after no iteration.
+bool empty_solve(const sf_plan* P, sf_solve_stats* stats) {
+  if (P->nfree > 0) return false;
+  if (stats) *stats = sf_solve_stats{0, 0, 1, 0.0, 0.0};
+  return true;
+}
+
 int read_flag(sf_plan* P, int* out) {
   int h = 0;
   SF_HIP(hipMemcpy(&h, P->flag.p, sizeof(int), hipMemcpyDeviceToHost));
@@ -2461,6 +2469,7 @@ int sf_cg(sf_plan* P, const double* b, double* x, double rtol, int maxit,
   if (rc) return rc;
   if ((!b || !x) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
   if (!(rtol >= 0.0) || maxit < 0) return fail(SF_ERR_ARG, "bad rtol / maxit");
+  if (empty_solve(P, stats)) return SF_OK;
   DeviceGuard g(P->device);
   hipStream_t s = (hipStream_t)stream;
   rc = ensure_solver_ws(P, false);
@@ -2476,6 +2485,7 @@ int sf_cg_noalloc(sf_plan* P, const double* b, double* x, double rtol, int maxit
   if (rc) return rc;
   if ((!b || !x) && P->nfree > 0) return fail(SF_ERR_ARG, "null vector");
   if (!(rtol >= 0.0) || maxit < 0) return fail(SF_ERR_ARG, "bad rtol / maxit");
+  if (empty_solve(P, stats)) return SF_OK;
   if (P->pc_kind != SF_PC_NONE && !P->pc_valid && P->nfree > 0)
     return fail(SF_ERR_NOT_READY,
                 "the preconditioner is not built yet (one sf_cg or sf_precond_apply call "
@@ -2529,6 +2539,7 @@ int sf_cg_mixed(sf_plan* P, const double* b, double* x, double rtol, int maxit,
     return fail(SF_ERR_ARG, "fp64_recurrence must be 0, 1 or 2 (got %d)", fp64_recurrence);
   if (!is_reduced(inner_precision))
     return fail(SF_ERR_UNSUPPORTED, "inner precision %d not available", inner_precision);
+  if (empty_solve(P, stats)) return SF_OK;
   DeviceGuard g(P->device);
   hipStream_t s = (hipStream_t)stream;
   rc = prepare(P, inner_precision);
@@ -2589,6 +2600,7 @@ int sf_cg_ir(sf_plan* P, const double* b, double* x, double rtol,
     return fail(SF_ERR_ARG, "bad tolerances / iteration limits");
   if (!is_reduced(inner_precision))
     return fail(SF_ERR_UNSUPPORTED, "inner precision %d not available for refinement", inner_precision);
+  if (empty_solve(P, stats)) return SF_OK;
   DeviceGuard g(P->device);
   hipStream_t s = (hipStream_t)stream;
   rc = prepare(P, inner_precision);

@@ -72,7 +72,7 @@ def cg_ir_(x, A, b, rtol=1e-10, inner="f32", inner_rtol=1e-3, inner_maxiter=None
     _set_pc(A, Pl)
     st = _lib.SolveStats()
     prec = _lib.PRECISIONS[inner] if isinstance(inner, str) else int(inner)
-    inner_maxiter = A.n_free if inner_maxiter is None else int(inner_maxiter)
+    inner_maxiter = max(1, A.n_free) if inner_maxiter is None else int(inner_maxiter)
     rc = A._L.sf_cg_ir(A.plan, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
                        float(rtol), int(max_outer), prec, float(inner_rtol), inner_maxiter,
                        ctypes.byref(st), _torch_stream(x))
@@ -90,8 +90,8 @@ def cg_mixed_(x, A, b, rtol=1e-10, inner="f32", delta=0.1, maxiter=None,
     b - A x whenever it has fallen by `delta` (search direction kept).
     Pl: the preconditioner, as in cg_.  recurrence None: 'f32' with the
     Gauss-Lobatto preconditioner (its well-conditioned system loses no
-    iterations to the FP32 recurrence, which moves the fewest bytes: 1.53x
-    FP64 CG at 32^3 Q7 against 1.34x for 'f64p32'), else 'f64p32' (on the
+    iterations to the FP32 recurrence, which moves the fewest bytes: 1.50x
+    FP64 CG at 32^3 Q7 against 1.25x for 'f64p32'), else 'f64p32' (on the
     unpreconditioned system the FP32 recurrence needs about 50% more
     iterations)."""
     _vec_ok(A, x)

@@ -143,6 +143,38 @@ def test_2d_blocks_every_kernel(pkg, oracle_mod, cuda, monkeypatch):
     say("worst rel. l2 %.2e" % worst)
 
 
+def test_tiny_meshes(pkg, oracle_mod, cuda):
+    """One cell, or two per direction, with and without Dirichlet boundaries
+    (down to no free dof at all): blocks holding a single partial cell, cells
+    whose every dof is Dirichlet, FP64 and FP32, 2D and 3D, against the
+    oracle; the solvers return at once for an empty system."""
+    for D in (2, 3):
+        for m, p, q in [(1, 1, 2), (1, 2, 3), (1, 3, 4), (2, 1, 2)]:
+            for dirichlet in (True, False):
+                say("tiny D=%d m=%d p=%d q=%d dirichlet=%d" % (D, m, p, q, dirichlet))
+                model, space, mp, A = make(pkg, D, m, p, q, perturb=0.2 if m > 1 else 0.0,
+                                           seed=3, dirichlet=dirichlet)
+                x = pkg.uniform(0, space.n_free, -1, 1)
+                y_ref = oracle_mod.apply(D, p + 1, q, space.cell_dofs, model.cell_vertices(), x,
+                                         mp.tables)
+                y = gpu_apply(cuda, A, x)
+                xd = cuda.from_numpy(x).cuda()
+                y32 = cuda.full_like(xd.float(), float("nan"))
+                A.mul_reduced(y32, xd.float(), "f32")
+                cuda.cuda.synchronize()
+                if space.n_free == 0:
+                    assert y.size == 0 and y32.numel() == 0
+                    b = cuda.zeros(0, dtype=cuda.float64, device="cuda")
+                    for st in (pkg.cg_(cuda.zeros_like(b), A, b, rtol=1e-10, maxiter=10),
+                               pkg.cg_mixed_(cuda.zeros_like(b), A, b, rtol=1e-10, maxiter=10),
+                               pkg.cg_ir_(cuda.zeros_like(b), A, b, rtol=1e-10)):
+                        assert st["iterations"] == 0 and st["converged"] == 1
+                else:
+                    assert rel_l2(y, y_ref) < TOL, (D, m, p, q, dirichlet)
+                    assert rel_l2(y32.cpu().numpy().astype(np.float64), y_ref) < 1e-5
+                A.free()
+
+
 @pytest.mark.parametrize("p,q,m,perturb", [(1, 3, 4, 0.0), (1, 3, 4, 0.2), (3, 5, 3, 0.2)])
 def test_demo_step_by_step_on_gpu(pkg, oracle_mod, cuda, p, q, m, perturb):
     """test/Demo_StepByStep.jl's case (2D, Q1, 4 x 4 cells, 3 Gauss points,
