@@ -252,6 +252,41 @@ def test_config2_full_size_vs_oracle(pkg, oracle_mod, cuda):
     A.free()
 
 
+@pytest.mark.parametrize("cfg,m,p,q,with_kappa,n_free", [
+    ("config 3", 32, 7, 9, False, 11089567),
+    ("config 4", 48, 5, 7, True, 13651919),
+    ("config 5", 24, 9, 11, True, 9938375)])
+def test_configs_full_size_vs_oracle(pkg, oracle_mod, cuda, cfg, m, p, q, with_kappa, n_free):
+    """The other 3D workloads at their full sizes (perturbed meshes; log-normal
+    kappa per quadrature point where the workload has one): the FP64 apply
+    against the oracle (1e-12), symmetry x.Av = v.Ax, and the FP32 apply
+    within 1e-5 of the oracle."""
+    say("%s full size: 3D %d^3 Q%d q=%d kappa=%d" % (cfg, m, p, q, with_kappa))
+    model = pkg.CartesianModel(3, m, perturb=0.2, seed=1)
+    space = pkg.FESpace(model, p)
+    assert space.n_free == n_free
+    mp = pkg.SumFactorizationMap.with_points(3, p, q)
+    kappa = np.exp(pkg.normal(3, space.n_cells * q ** 3)) if with_kappa else None
+    A = pkg.LazyMatrix(mp, space, kappa=kappa)
+    x = pkg.uniform(0, space.n_free, -1, 1)
+    v = pkg.uniform(9, space.n_free, -1, 1)
+    y = gpu_apply(cuda, A, x)
+    Av = gpu_apply(cuda, A, v)
+    xd = cuda.from_numpy(x).cuda()
+    y32 = cuda.full_like(xd.float(), float("nan"))
+    A.mul_reduced(y32, xd.float(), "f32")
+    cuda.cuda.synchronize()
+    y32 = y32.cpu().numpy().astype(np.float64)
+    say("GPU done; oracle (CPU) ...")
+    y_ref = oracle_mod.apply(3, p + 1, q, space.cell_dofs, model.cell_vertices(), x, mp.tables,
+                             kappa=kappa)
+    e, e32 = rel_l2(y, y_ref), rel_l2(y32, y_ref)
+    say("rel l2 FP64 %.2e, FP32 %.2e" % (e, e32))
+    assert e < TOL and e32 < 1e-5
+    assert abs(v @ y - x @ Av) < 1e-12 * abs(x @ y)
+    A.free()
+
+
 def test_patch_test_full_operator(pkg, cuda):
     """All dofs free (no Dirichlet): A 1 = 0 and interior rows of A (a.x + b)
     vanish on a perturbed mesh (exact quadrature, q = p + 2)."""
