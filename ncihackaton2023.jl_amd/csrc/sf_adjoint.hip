@@ -460,7 +460,7 @@ struct AdjB3Shape {
   // the y-pass outputs over both; then the y^T-pass output (two tensors per
   // cell slot) with the assembled values at the end
   static constexpr int RC = cmax(EB * CB, RA + 2 * RX);
-  static_assert((RC + EB * 24) * 8 <= 163840, "adjoint3d: LDS above 160 KiB");
+  static_assert((RC + EB * kGeoCoef3) * 8 <= 163840, "adjoint3d: LDS above 160 KiB");
   static_assert(NT >= EB * cmax(NN, cmax(NQ, QQ)), "one task per thread per stage");
 };
 
@@ -479,8 +479,8 @@ __attribute__((amdgpu_waves_per_eu(2)))
                     const double* __restrict__ tW, const double* __restrict__ kap,
                     const int32_t* __restrict__ corig) {
   using S = AdjB3Shape<N, Q>;
-  __shared__ double smem[S::RC + S::EB * 24];
-  double* const regV = smem + S::RC;          // the block's cell vertices
+  __shared__ double smem[S::RC + S::EB * kGeoCoef3];
+  double* const regV = smem + S::RC;          // the block's map coefficients
   double* const regA = smem;                  // x / y^T pass outputs
   double* const regB = smem;                  // y pass outputs, z^T in place
   double* const regXu = smem + S::RA;         // the block's u values
@@ -525,10 +525,10 @@ __attribute__((amdgpu_waves_per_eu(2)))
   const bool own3 = tid < S::EB * S::QQ;
   const int e3 = tid / S::QQ, r3 = tid - e3 * S::QQ;  // r3 = k*Q + i
   const bool act3 = own3 && e3 < ncb;
-  // ncb * 24 values: more than NT when a block holds more than NT / 24
-  // cells (up to 64 at the lowest orders)
-  for (int t = tid; t < ncb * 24; t += S::NT)
-    regV[t] = __builtin_nontemporal_load(Xv + base * 24 + t);
+  // ncb * 21 map coefficients: more than NT when a block holds more than
+  // NT / 21 cells (up to 64 at the lowest orders)
+  for (int t = tid; t < ncb * kGeoCoef3; t += S::NT)
+    regV[t] = __builtin_nontemporal_load(Xv + base * kGeoCoef3 + t);
   double kq[KAP ? Q : 1];
   if constexpr (KAP) {
     const double* kp = kap + (base + (act3 ? e3 : 0)) * S::QQQ + (act3 ? r3 : 0);
@@ -665,23 +665,8 @@ __attribute__((amdgpu_waves_per_eu(2)))
     // in the fused-geometry apply
     double E0[3], dE[3], F0[3], dF[3], H[3];
     const int qi = r3 % Q, qk = r3 / Q;
-    const double xi = tP[qi], et = tP[qk];
     const double wik = tW[qi] * tW[qk];
-    const double* V = regV + e3 * 24;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double e0 = (1 - et) * (V[3 + a] - V[a]) + et * (V[9 + a] - V[6 + a]);
-      const double e1 = (1 - et) * (V[15 + a] - V[12 + a]) + et * (V[21 + a] - V[18 + a]);
-      const double f0 = (1 - xi) * (V[6 + a] - V[a]) + xi * (V[9 + a] - V[3 + a]);
-      const double f1 = (1 - xi) * (V[18 + a] - V[12 + a]) + xi * (V[21 + a] - V[15 + a]);
-      const double h0 = (1 - xi) * (V[12 + a] - V[a]) + xi * (V[15 + a] - V[3 + a]);
-      const double h1 = (1 - xi) * (V[18 + a] - V[6 + a]) + xi * (V[21 + a] - V[9 + a]);
-      E0[a] = e0;
-      dE[a] = e1 - e0;
-      F0[a] = f0;
-      dF[a] = f1 - f0;
-      H[a] = (1 - et) * h0 + et * h1;
-    }
+    column_geom3<double>(regV + e3 * kGeoCoef3, tP[qi], tP[qk], E0, dE, F0, dF, H);
     const int64_t crow = base + (act3 ? e3 : 0);  // the plan's cell, the caller's row
     double* kout = dJdk + (corig ? (int64_t)corig[crow] : crow) * S::QQQ + r3;
     TableRows<double, S::TK> t_(tB, tD);

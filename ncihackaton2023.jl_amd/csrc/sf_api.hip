@@ -303,8 +303,11 @@ struct sf_plan {
   DevBuf<double> G, Gk, kappa;
   DevBuf<float> kappa32;  // FP32 copy for the FP32 fused-geometry apply
   bool k32_valid = false;
-  DevBuf<double> X;   // cell vertices (kept for the fused-geometry apply)
-  DevBuf<float> X32, Pts32, W32;  // FP32 copies for the FP32 fused apply
+  DevBuf<double> X;   // cell vertices (G is recomputed from them for new weights)
+  DevBuf<double> Xc;  // each cell's map coefficients (kGeoCoef3 / kGeoCoef2
+                      // values): what the fused-geometry kernels read
+  DevBuf<float> X32, Pts32, W32;  // FP32 copies (of Xc, points, weights) for
+                                  // the FP32 fused apply
   bool fused_geom = false;
   DevBuf<float> G32;
   DevBuf<bf16s> Gbf;
@@ -368,7 +371,7 @@ struct sf_plan {
     dB.release(); dD.release(); dW.release(); dPts.release();
     dB32.release(); dD32.release();
     cells.release(); G.release(); Gk.release(); kappa.release(); kappa32.release();
-    X.release();
+    X.release(); Xc.release();
     X32.release(); Pts32.release(); W32.release();
     G32.release(); Gbf.release(); Gm.release(); Gm32.release(); flag.release();
     pidx.release(); bdofs.release(); bexcl.release(); cell_block.release();
@@ -390,7 +393,7 @@ struct sf_plan {
 
   int64_t device_bytes() const {
     return dB.bytes() + dD.bytes() + dW.bytes() + dPts.bytes() + dB32.bytes() +
-           dD32.bytes() + cells.bytes() + G.bytes() + Gk.bytes() + X.bytes() +
+           dD32.bytes() + cells.bytes() + G.bytes() + Gk.bytes() + X.bytes() + Xc.bytes() +
            X32.bytes() + Pts32.bytes() + W32.bytes() +
            kappa.bytes() + kappa32.bytes() + G32.bytes() + Gbf.bytes() + Gm.bytes() + Gm32.bytes() + flag.bytes() +
            pidx.bytes() + bdofs.bytes() + bexcl.bytes() + cell_block.bytes() +
@@ -495,14 +498,14 @@ int clear_flag(sf_plan* P) {
 
 // The 3D fused-geometry applies (vertices kept, FP32 copies made): the
 // forms the fused CG step and the block-assembled adjoint build on.
-bool fused_f64(const sf_plan* P) { return P->fused_geom && P->dim == 3 && P->X.p; }
+bool fused_f64(const sf_plan* P) { return P->fused_geom && P->dim == 3 && P->Xc.p; }
 bool fused_f32(const sf_plan* P) {
   return P->fused_geom && P->dim == 3 && P->X32.p && (!P->has_kappa || P->k32_valid);
 }
 // Whether the apply recomputes G from the vertices: the 3D forms above, and
 // in 2D without kappa (a 2D kappa scales the stored G).
 bool geo_f64(const sf_plan* P) {
-  return P->dim == 3 ? fused_f64(P) : (P->fused_geom && P->X.p && !P->has_kappa);
+  return P->dim == 3 ? fused_f64(P) : (P->fused_geom && P->Xc.p && !P->has_kappa);
 }
 bool geo_f32(const sf_plan* P) {
   return P->dim == 3 ? fused_f32(P) : (P->fused_geom && P->X32.p && !P->has_kappa);
@@ -542,7 +545,7 @@ int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
   ApplyTimer timer(P, s);
   hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
                            P->dB.p, P->dD.p, P->ncell, P->blocks(), accumulate, s,
-                           geo ? P->X.p : nullptr, P->dPts.p, P->dW.p, dotp,
+                           geo ? P->Xc.p : nullptr, P->dPts.p, P->dW.p, dotp,
                            geo && P->dim == 3 && P->has_kappa ? P->kappa.p : nullptr);
   if (e == hipErrorNotSupported)
     return fail(SF_ERR_UNSUPPORTED, "no kernel for dim=%d n=%d q=%d", P->dim, P->n, P->q);
@@ -631,13 +634,14 @@ int prepare(sf_plan* P, int precision) {
     return SF_OK;
   }
   if (precision == SF_PREC_F32) {
-    if (P->fused_geom && !P->X32.p) {
-      // FP32 vertices and 1D quadrature for the FP32 fused-geometry apply
-      const int64_t nv = P->ncell * (P->dim == 3 ? 24 : 8);
+    if (P->fused_geom && P->Xc.p && !P->X32.p) {
+      // FP32 map coefficients and 1D quadrature for the FP32 fused-geometry
+      // apply
+      const int64_t nv = P->ncell * (P->dim == 3 ? kGeoCoef3 : kGeoCoef2);
       SF_HIP(P->X32.alloc(nv));
       SF_HIP(P->Pts32.alloc(P->q));
       SF_HIP(P->W32.alloc(P->q));
-      SF_HIP(launch_to_f32(P->X.p, nv, P->X32.p, nullptr));
+      SF_HIP(launch_to_f32(P->Xc.p, nv, P->X32.p, nullptr));
       std::vector<float> pw(2 * P->q);
       for (int i = 0; i < P->q; ++i) {
         pw[i] = (float)P->hPts[i];
@@ -2129,6 +2133,7 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
     P->has_geom = false;
     P->fused_geom = false;
     P->X.release();
+    P->Xc.release();
     P->X32.release();
     P->g32_valid = P->gbf_valid = P->gm_valid = P->gm32_valid = false;
   }
@@ -2163,6 +2168,41 @@ static int need_cells_for_geometry(const sf_plan* P) {
   return SF_OK;
 }
 
+// The coefficient vectors of each cell's multilinear map from its vertices
+// (vertex v = b0 + 2 b1 + 4 b2, D coordinates each), in the order the
+// fused-geometry kernels read them: 3D a e p b c g d, 2D a e b (the terms
+// xi, eta, zeta, xi eta, xi zeta, eta zeta, xi eta zeta of x - V0).
+static std::vector<double> map_coefficients(int dim, const double* X, int64_t ncell) {
+  const int nc = dim == 3 ? kGeoCoef3 : kGeoCoef2;
+  std::vector<double> C((size_t)(ncell * nc));
+  for (int64_t e = 0; e < ncell; ++e) {
+    double* c = C.data() + e * nc;
+    if (dim == 3) {
+      const double* V = X + e * 24;
+      for (int a = 0; a < 3; ++a) {
+        const double v0 = V[a], v1 = V[3 + a], v2 = V[6 + a], v3 = V[9 + a], v4 = V[12 + a],
+                     v5 = V[15 + a], v6 = V[18 + a], v7 = V[21 + a];
+        c[a] = v1 - v0;                                          // a: xi
+        c[3 + a] = v2 - v0;                                      // e: eta
+        c[6 + a] = v4 - v0;                                      // p: zeta
+        c[9 + a] = (v3 - v2) - (v1 - v0);                        // b: xi eta
+        c[12 + a] = (v5 - v4) - (v1 - v0);                       // c: xi zeta
+        c[15 + a] = (v6 - v4) - (v2 - v0);                       // g: eta zeta
+        c[18 + a] = ((v7 - v6) - (v5 - v4)) - ((v3 - v2) - (v1 - v0));  // d: xi eta zeta
+      }
+    } else {
+      const double* V = X + e * 8;
+      for (int a = 0; a < 2; ++a) {
+        const double v0 = V[a], v1 = V[2 + a], v2 = V[4 + a], v3 = V[6 + a];
+        c[a] = v1 - v0;                    // a: xi
+        c[2 + a] = v2 - v0;                // e: eta
+        c[4 + a] = (v3 - v2) - (v1 - v0);  // b: xi eta
+      }
+    }
+  }
+  return C;
+}
+
 int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
   SF_TRY
   if (check_plan(P)) return SF_ERR_ARG;
@@ -2175,6 +2215,7 @@ int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
   P->fused_geom = false;
   P->X32.release();
   SF_HIP(X.alloc(nv));
+  SF_HIP(P->Xc.alloc(P->ncell * (P->dim == 3 ? kGeoCoef3 : kGeoCoef2)));
   int rc = SF_OK;
   do {
     if (nv > 0) {
@@ -2182,6 +2223,9 @@ int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
       const double* Xp = to_plan_order(P, vertices, nv / P->ncell, tmp);
       hipError_t e = hipMemcpy(X.p, Xp, nv * 8, hipMemcpyHostToDevice);
       if (e != hipSuccess) { rc = fail(SF_ERR_HIP, "vertex upload: %s", hipGetErrorString(e)); break; }
+      const std::vector<double> C = map_coefficients(P->dim, Xp, P->ncell);
+      e = hipMemcpy(P->Xc.p, C.data(), C.size() * 8, hipMemcpyHostToDevice);
+      if (e != hipSuccess) { rc = fail(SF_ERR_HIP, "map coefficient upload: %s", hipGetErrorString(e)); break; }
     }
     hipError_t e = P->G.alloc(P->ncell * P->ng * P->nqc);
     if (e != hipSuccess) { rc = fail(SF_ERR_HIP, "G alloc: %s", hipGetErrorString(e)); break; }
@@ -2202,13 +2246,16 @@ int sf_plan_set_vertices(sf_plan* P, const double* vertices) {
     P->has_geom = true;
     rc = finish_geometry(P);
   } while (0);
-  // The apply recomputes G from the vertices (SFHIP_FUSED_GEOMETRY=0 reads
-  // the stored G instead; in 2D a kappa, which scales the stored G, does
-  // too).  Every form keeps them (24 or 8 values per cell): new quadrature
-  // weights (sf_plan_set_tables) recompute G from them.
+  // The apply recomputes G from the map coefficients (SFHIP_FUSED_GEOMETRY=0
+  // reads the stored G instead; in 2D a kappa, which scales the stored G,
+  // does too).  Every form keeps the vertices (24 or 8 values per cell): new
+  // quadrature weights (sf_plan_set_tables) recompute G from them.
   const char* fg = getenv("SFHIP_FUSED_GEOMETRY");
   P->fused_geom = rc == SF_OK && !(fg && fg[0] == '0');
-  if (rc != SF_OK) X.release();
+  if (rc != SF_OK) {
+    X.release();
+    P->Xc.release();
+  }
   return rc;
   SF_CATCH
 }
@@ -2243,6 +2290,7 @@ int sf_plan_set_geometry(sf_plan* P, const double* G) {
   P->has_geom = true;
   P->fused_geom = false;  // a caller-supplied G is what the apply must use
   P->X.release();
+  P->Xc.release();
   P->X32.release();
   return finish_geometry(P);
   SF_CATCH
@@ -2390,7 +2438,7 @@ int sf_adjoint(sf_plan* P, const double* u, const double* lam,
     if (!dJdkappa) return dJdu ? do_apply_f64(P, dJdu, lam, s) : SF_OK;
     if (dJdu) SF_HIP(launch_fill<double>(dJdu, P->nfree, 0.0, s));
     hipError_t e = adjoint3d_fused_f64(P->n, P->q, u, lam, dJdkappa, dJdu, &P->kargs,
-                                       P->dB.p, P->dD.p, P->X.p, P->dPts.p, P->dW.p,
+                                       P->dB.p, P->dD.p, P->Xc.p, P->dPts.p, P->dW.p,
                                        P->has_kappa ? P->kappa.p : nullptr, P->corig.p, s);
     if (e == hipErrorNotSupported)
       return fail(SF_ERR_UNSUPPORTED, "no adjoint kernel for dim=%d n=%d q=%d", P->dim, P->n,

@@ -171,6 +171,33 @@ __device__ __forceinline__ float recip_pos(float a) {
   return fmaf(r, fmaf(-a, r, 1.0f), r);
 }
 
+// The fused-geometry kernels read each cell's multilinear map as its
+// coefficient vectors (computed once per plan from the vertices, vertex
+// v = b0 + 2 b1 + 4 b2):
+//   3D  x(xi, eta, zeta) = V0 + xi a + eta e + zeta p + xi eta b + xi zeta c
+//                          + eta zeta g + xi eta zeta d,   stored a e p b c g d
+//   2D  x(xi, eta) = V0 + xi a + eta e + xi eta b,         stored a e b
+// so the derivatives along a quadrature column are a few FMAs each, not
+// bilinear interpolations of vertex differences.
+constexpr int kGeoCoef3 = 21, kGeoCoef2 = 6;
+
+// Edge vectors of the 3D map along quadrature column (xi, eta):
+// dx/dxi = E0 + zeta dE, dx/deta = F0 + zeta dF, dx/dzeta = H (constant).
+template <typename T>
+__device__ __forceinline__ void column_geom3(const T* C, T xi, T et, T (&E0)[3], T (&dE)[3],
+                                             T (&F0)[3], T (&dF)[3], T (&H)[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const T ca = C[a], ce = C[3 + a], cp = C[6 + a], cb = C[9 + a], cc = C[12 + a],
+            cg = C[15 + a], cd = C[18 + a];
+    E0[a] = fma(et, cb, ca);  // dx/dxi   = a + eta b + zeta (c + eta d)
+    dE[a] = fma(et, cd, cc);
+    F0[a] = fma(xi, cb, ce);  // dx/deta  = e + xi b + zeta (g + xi d)
+    dF[a] = fma(xi, cd, cg);
+    H[a] = fma(et, dF[a], fma(xi, cc, cp));  // dx/dzeta = p + xi c + eta (g + xi d)
+  }
+}
+
 // t = G g at one quadrature point from the columns of J, with G = w / |det J|
 // C^T C (C the cofactor matrix; the geometry kernel's formula) applied as
 // s C^T (C g): 9 + 9 FMAs and one scaling instead of forming the six G
@@ -217,11 +244,11 @@ __device__ __forceinline__ void apply_geom(const T (&j0)[3], const T (&j1)[3],
 // accumulate is set, in which case exclusive dofs are read-modify-written.
 //
 // GEO: the geometric factor is not read but recomputed at every quadrature
-// point from the cell's 8 vertices (Xv, 24 values per cell, trilinear map)
-// and the 1D points / weights (tP, tW): 192 bytes per cell instead of
-// 48 q^3.  The z-column thread of quadrature column (i, k) interpolates
-// dx/dxi and dx/deta linearly in zeta between edge vectors it forms once,
-// and dx/dzeta is constant along its column.
+// point from the cell's trilinear map (Xv: its 7 coefficient vectors, 21
+// values per cell) and the 1D points / weights (tP, tW): 168 bytes per cell
+// instead of 48 q^3.  The z-column thread of quadrature column (i, k) forms
+// dx/dxi and dx/deta as linear functions of zeta once (column_geom3), and
+// dx/dzeta is constant along its column.
 //
 // KAP (with GEO): a diffusion coefficient kap[cell][qp] (qp = l Q^2 + k Q
 // + i) scales the quadrature weight; it is read in the prologue, 8 (FP64)
@@ -250,9 +277,9 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
   static_assert(GEO || !KAP, "apply3d: kappa per point needs the fused geometry");
   static_assert(S::EB == Apply3Shape<N, Q, T>::EB && S::NT == Apply3Shape<N, Q, T>::NT,
                 "apply3d: one block structure for both forms");
-  __shared__ T smem[S::RC + (GEO ? S::EB * 24 : 0)];
+  __shared__ T smem[S::RC + (GEO ? S::EB * kGeoCoef3 : 0)];
   __shared__ int32_t sdof[S::DVL ? S::RX : 1];  // DVL: the block's dof ids
-  T* const regV = smem + S::RC;  // GEO: the block's cell vertices
+  T* const regV = smem + S::RC;  // GEO: the block's map coefficients
   T* const regA = smem;                               // x / y^T pass out
   T* const regB = S::LC ? smem : smem + S::RA;        // y pass out, z in place
   T* const regX = S::LC ? smem + S::RA : regB;        // the block's x values
@@ -304,11 +331,11 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
   T gq[GEO ? 1 : 6][GEO ? 1 : Q];
   T kq[KAP ? Q : 1];
   if constexpr (GEO) {
-    // the block's vertex coordinates into LDS (ncb * 24 consecutive values)
-    // ncb * 24 values: more than NT when a block holds more than NT / 24
-    // cells (up to 64 at the lowest orders)
-    for (int t = tid; t < ncb * 24; t += S::NT)
-      regV[t] = __builtin_nontemporal_load(Xv + base * 24 + t);
+    // the block's map coefficients into LDS (ncb * 21 consecutive values:
+    // more than NT when a block holds more than NT / 21 cells, up to 64 at
+    // the lowest orders)
+    for (int t = tid; t < ncb * kGeoCoef3; t += S::NT)
+      regV[t] = __builtin_nontemporal_load(Xv + base * kGeoCoef3 + t);
     if constexpr (KAP) {
       // kappa of column r3 of cell e3 at every z level l
       const T* kp = kap + (base + (act3 ? e3 : 0)) * S::QQQ + (act3 ? r3 : 0);
@@ -424,26 +451,8 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
     T E0[3], dE[3], F0[3], dF[3], H[3], wik = T(0);
     if constexpr (GEO) {
       const int qi = r3 % Q, qk = r3 / Q;
-      const T xi = tP[qi], et = tP[qk];
       wik = tW[qi] * tW[qk];
-      const T* V = regV + e3 * 24;  // vertex v = b0 + 2 b1 + 4 b2, 3 coords
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        // dx/dxi on the planes zeta = 0, 1 (interpolated in eta)
-        const T e0 = (T(1) - et) * (V[1 * 3 + a] - V[0 * 3 + a]) + et * (V[3 * 3 + a] - V[2 * 3 + a]);
-        const T e1 = (T(1) - et) * (V[5 * 3 + a] - V[4 * 3 + a]) + et * (V[7 * 3 + a] - V[6 * 3 + a]);
-        // dx/deta on the planes zeta = 0, 1 (interpolated in xi)
-        const T f0 = (T(1) - xi) * (V[2 * 3 + a] - V[0 * 3 + a]) + xi * (V[3 * 3 + a] - V[1 * 3 + a]);
-        const T f1 = (T(1) - xi) * (V[6 * 3 + a] - V[4 * 3 + a]) + xi * (V[7 * 3 + a] - V[5 * 3 + a]);
-        // dx/dzeta, bilinear in (xi, eta)
-        const T h0 = (T(1) - xi) * (V[4 * 3 + a] - V[0 * 3 + a]) + xi * (V[5 * 3 + a] - V[1 * 3 + a]);
-        const T h1 = (T(1) - xi) * (V[6 * 3 + a] - V[2 * 3 + a]) + xi * (V[7 * 3 + a] - V[3 * 3 + a]);
-        E0[a] = e0;
-        dE[a] = e1 - e0;
-        F0[a] = f0;
-        dF[a] = f1 - f0;
-        H[a] = (T(1) - et) * h0 + et * h1;
-      }
+      column_geom3<T>(regV + e3 * kGeoCoef3, tP[qi], tP[qk], E0, dE, F0, dF, H);
     }
     TableRows<T, S::TK> t_(tB, tD);
 #pragma unroll
@@ -606,11 +615,12 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
 // shared ones with one atomic each (accumulate: exclusive dofs
 // read-modify-written).
 //
-// GEO: G is recomputed at every quadrature point from the cell's 4 vertices
-// (bilinear map, 8 values per cell instead of 24 Q^2 bytes): along the
-// column of xi = t_i, dx/dxi = E0 + eta dE is linear in eta and dx/deta is
-// constant, so a point costs two FMAs for J, its determinant, one
-// reciprocal and t = (w / |det J|) C^T (C g) with C the cofactor matrix.
+// GEO: G is recomputed at every quadrature point from the cell's bilinear
+// map (its 3 coefficient vectors, 6 values per cell instead of 24 Q^2
+// bytes): along the column of xi = t_i, dx/dxi = a + eta b is linear in eta
+// and dx/deta = e + xi b is constant, so a point costs two FMAs for J, its
+// determinant, one reciprocal and t = (w / |det J|) C^T (C g) with C the
+// cofactor matrix.
 // Otherwise G is read, [cell][entry][k Q + i].
 template <int N, int Q, typename T>
 struct Apply2Shape {
@@ -636,7 +646,7 @@ struct Apply2Shape {
   static constexpr int RX = EB * NN;  // distinct dofs of a block, at most
   static constexpr int KU = (RX + NT - 1) / NT;
   static constexpr int TK = 1;  // table rows per scalar-load group
-  static constexpr int LDS_BYTES = (EB * SE + RX + EB * 8) * (int)sizeof(T);
+  static constexpr int LDS_BYTES = (EB * SE + RX + EB * kGeoCoef2) * (int)sizeof(T);
   static_assert(LDS_BYTES <= 65536, "apply2d: LDS per workgroup above 64 KiB");
   static_assert(NT >= EB * MAXT, "one task per thread per stage");
 };
@@ -653,10 +663,10 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
               const T* __restrict__ tD, const T* __restrict__ Xv,
               const T* __restrict__ tP, const T* __restrict__ tW, int accumulate) {
   using S = Apply2Shape<N, Q, T>;
-  __shared__ T smem[S::EB * S::SE + S::RX + (GEO ? S::EB * 8 : 0)];
+  __shared__ T smem[S::EB * S::SE + S::RX + (GEO ? S::EB * kGeoCoef2 : 0)];
   T* const regA = smem;                    // per cell [2][N][QP]
   T* const regX = smem + S::EB * S::SE;    // the block's x values, later y
-  T* const regV = regX + S::RX;            // GEO: the block's cell vertices
+  T* const regV = regX + S::RX;            // GEO: the block's map coefficients
   const int b = xcd_remap(blockIdx.x, nblk);
   const int64_t base = bcell[b];
   const int ncb = (int)(bcell[b + 1] - base);  // cells in this block, <= EB
@@ -696,8 +706,8 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
   const bool act2 = own2 && e2 < ncb;
   T gq[GEO ? 1 : 3][GEO ? 1 : Q];
   if constexpr (GEO) {
-    for (int t = tid; t < ncb * 8; t += S::NT)
-      regV[t] = __builtin_nontemporal_load(Xv + base * 8 + t);
+    for (int t = tid; t < ncb * kGeoCoef2; t += S::NT)
+      regV[t] = __builtin_nontemporal_load(Xv + base * kGeoCoef2 + t);
   } else {
     const TG* g = G + (base + (act2 ? e2 : 0)) * 3 * S::QQ + i2;
 #pragma unroll
@@ -753,18 +763,18 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
       a[j] = sA[j * S::QP];
       d[j] = sA[S::SA1 + j * S::QP];
     }
-    // GEO: J[c][0] = dx_c/dxi = E0[c] + eta dE[c], J[c][1] = dx_c/deta = H[c]
-    // (vertex v = b0 + 2 b1, 2 coordinates)
+    // GEO: J[c][0] = dx_c/dxi = E0[c] + eta dE[c] (= a + eta b),
+    // J[c][1] = dx_c/deta = H[c] (= e + xi b) on the column xi = t_i2
     T E0[2], dE[2], H[2], wi = T(0);
     if constexpr (GEO) {
       const T xi = tP[i2];
       wi = tW[i2];
-      const T* V = regV + (act2 ? e2 : 0) * 8;
+      const T* C = regV + (act2 ? e2 : 0) * kGeoCoef2;  // a, e, b
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        E0[c] = V[1 * 2 + c] - V[0 * 2 + c];
-        dE[c] = (V[3 * 2 + c] - V[2 * 2 + c]) - E0[c];
-        H[c] = (T(1) - xi) * (V[2 * 2 + c] - V[0 * 2 + c]) + xi * (V[3 * 2 + c] - V[1 * 2 + c]);
+        E0[c] = C[c];
+        dE[c] = C[4 + c];
+        H[c] = fma(xi, C[4 + c], C[2 + c]);
       }
     }
     T tx[Q], ty[Q];
@@ -864,8 +874,8 @@ struct ApplyArgs {
   const T* tD;
   int64_t ncell;
   int accumulate;  // 0 -> y zeroed by the caller, 1 -> y += A x
-  // cell vertices, 1D points and weights; set -> G recomputed in the
-  // kernel (the G argument is then unused)
+  // cell map coefficients (kGeoCoef3 / kGeoCoef2 per cell), 1D points and
+  // weights; set -> G recomputed in the kernel (the G argument is then unused)
   const T* Xv = nullptr;
   const T* tP = nullptr;
   const T* tW = nullptr;
