@@ -43,9 +43,10 @@ def alg_bytes(n_free, n_cells, n, q, D, s_vec=8, s_g=8):
 
 def alg_bytes_fused(n_free, n_cells, n, D, s_vec=8):
     """Algorithmic HBM bytes of one apply that recomputes the geometric
-    factor from the cell vertices: x, y, cell dofs, 2^D D coordinates per
-    cell."""
-    return (s_vec + s_vec) * n_free + 4 * n ** D * n_cells + s_vec * D * 2 ** D * n_cells
+    factor from each cell's multilinear map: x, y, cell dofs, the map's
+    coefficient vectors (7 x 3 values per cell in 3D, 3 x 2 in 2D)."""
+    ncoef = 21 if D == 3 else 6
+    return (s_vec + s_vec) * n_free + 4 * n ** D * n_cells + s_vec * ncoef * n_cells
 
 
 STORED_G_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "apply3d_f64_q4_64cube_traffic.txt")
