@@ -301,6 +301,7 @@ struct sf_plan {
   DevBuf<float> dB32, dD32;
   DevBuf<int32_t> cells;
   DevBuf<double> G, Gk, kappa;
+  bool gk_valid = false;  // Gk = kappa * G is current (built on first use)
   DevBuf<float> kappa32;  // FP32 copy for the FP32 fused-geometry apply
   bool k32_valid = false;
   DevBuf<double> X;   // cell vertices (G is recomputed from them for new weights)
@@ -364,6 +365,7 @@ struct sf_plan {
   size_t nrec = 0;
   hipStream_t own_stream = nullptr;
 
+  // the stored geometric factor the stored-G forms read (after ensure_gk)
   const double* active_G() const { return has_kappa ? Gk.p : G.p; }
   const BlockArgs* blocks() const { return &kargs; }
 
@@ -532,6 +534,24 @@ struct ApplyTimer {
   }
 };
 
+// kappa * G for the forms that read a stored geometric factor (the stored-G
+// applies, the reduced-precision G copies, the diagonal), built on first use
+// after kappa or the geometry changed: the fused-geometry kernels read kappa
+// per point, so setting kappa on such a plan (the rrule's forward call, a
+// finite-difference loop) costs no pass over the stored G.
+int ensure_gk(sf_plan* P) {
+  if (!P->has_kappa || P->gk_valid || !P->has_geom) return SF_OK;
+  SF_HIP(P->Gk.alloc(P->ncell * P->ng * P->nqc));
+  if (P->dim == 3)
+    SF_HIP(launch_scale_kappa3(P->G.p, P->kappa.p, P->q, P->ncell, P->cell_block.p,
+                               P->bcell.p, P->Gk.p, nullptr));
+  else
+    SF_HIP(launch_scale_kappa(P->G.p, P->kappa.p, P->ng, P->nqc, P->ncell, P->Gk.p, nullptr));
+  SF_HIP(hipDeviceSynchronize());
+  P->gk_valid = true;
+  return SF_OK;
+}
+
 // y = A x (FP64) on stream s.  accumulate: y += A x.  zeroed: the caller
 // guarantees y == 0 already (the CG direction update clears it), so the
 // fill is skipped.
@@ -542,6 +562,10 @@ int do_apply_f64(sf_plan* P, double* y, const double* x, hipStream_t s,
   // the fused-geometry kernel when the plan has vertices (3D: kappa, if
   // set, read per quadrature point)
   const bool geo = geo_f64(P);
+  if (!geo) {
+    const int rc = ensure_gk(P);
+    if (rc) return rc;
+  }
   ApplyTimer timer(P, s);
   hipError_t e = apply_f64(P->dim, P->n, P->q, x, y, P->cells.p, P->active_G(),
                            P->dB.p, P->dD.p, P->ncell, P->blocks(), accumulate, s,
@@ -603,6 +627,8 @@ int prepare(sf_plan* P, int precision) {
     if (P->dim != 3)
       return fail(SF_ERR_UNSUPPORTED, "the MFMA apply is 3D only (dim=%d)", P->dim);
     if (!P->gm32_valid) {
+      const int rc = ensure_gk(P);
+      if (rc) return rc;
       SF_HIP(P->Gm32.alloc(P->ncell * P->nqc * 6));
       SF_HIP(launch_g3_to_mfma32(P->active_G(), P->Gm32.p, P->q, P->ncell, P->cell_block.p,
                                  P->bcell.p, nullptr));
@@ -620,6 +646,8 @@ int prepare(sf_plan* P, int precision) {
                   "the FP16 GEMM apply is built for n = 8, q = 8..10 (got n=%d q=%d)", P->n,
                   P->q);
     if (!P->gm_valid) {
+      const int rc = ensure_gk(P);
+      if (rc) return rc;
       SF_HIP(P->Gm.alloc(P->ncell * P->nqc * 3));
       SF_HIP(launch_g3_to_mfma(P->active_G(), P->Gm.p, P->q, P->ncell,
                                P->cell_block.p, P->bcell.p, nullptr));
@@ -657,7 +685,10 @@ int prepare(sf_plan* P, int precision) {
       SF_HIP(hipDeviceSynchronize());
       P->k32_valid = true;
     }
-    if (!P->g32_valid) {
+    // the stored FP32 G only where the FP32 apply reads it (no fused geometry)
+    if (!P->g32_valid && !geo_f32(P)) {
+      const int rc = ensure_gk(P);
+      if (rc) return rc;
       SF_HIP(P->G32.alloc(ng));
       if (P->dim == 3)
         SF_HIP(launch_g3_to_f32_pairs(P->active_G(), P->G32.p, P->q, P->ncell,
@@ -671,6 +702,8 @@ int prepare(sf_plan* P, int precision) {
   }
   if (precision == SF_PREC_F32_GBF16) {
     if (!P->gbf_valid) {
+      const int rc = ensure_gk(P);
+      if (rc) return rc;
       SF_HIP(P->Gbf.alloc(ng));
       if (P->dim == 3)
         SF_HIP(launch_g3_to_bf16_pairs(P->active_G(), P->Gbf.p, P->q, P->ncell,
@@ -1027,6 +1060,8 @@ int ensure_solver_ws(sf_plan* P, bool mixed) {
 // diag = diag(A) on the free dofs (FP64, overwritten) on stream s.
 int compute_diagonal(sf_plan* P, double* diag, hipStream_t s) {
   if (P->nfree == 0) return SF_OK;
+  const int rc = ensure_gk(P);
+  if (rc) return rc;
   SF_HIP(launch_fill<double>(diag, P->nfree, 0.0, s));
   hipError_t e = launch_diagonal(P->dim, P->n, P->q, P->active_G(), P->dB.p, P->dD.p,
                                  P->cells.p, P->cell_block.p, P->bcell.p, P->ncell, diag, s);
@@ -1048,6 +1083,8 @@ int ensure_pc(sf_plan* P, hipStream_t s, const float** dinv, bool* gll = nullptr
     return fail(SF_ERR_UNSUPPORTED, "the Gauss-Lobatto preconditioner is not available here");
   if (!P->pc_valid) {
     MarkerRange mark("sfhip preconditioner set-up");
+    const int rk = ensure_gk(P);  // the diagonal kernels read kappa * G
+    if (rk) return rk;
     DevBuf<double> d, tb, td;
     SF_HIP(d.alloc(P->nfree));
     SF_HIP(P->dinv.alloc(P->nfree));
@@ -2148,16 +2185,11 @@ int sf_plan_set_cells(sf_plan* P, const int32_t* cell_dofs, int index_base,
 static int finish_geometry(sf_plan* P) {
   P->g32_valid = P->gbf_valid = P->gm_valid = P->gm32_valid = false;
   P->pc_valid = false;
-  if (P->has_kappa) {
-    SF_HIP(P->Gk.alloc(P->ncell * P->ng * P->nqc));
-    if (P->dim == 3)
-      SF_HIP(launch_scale_kappa3(P->G.p, P->kappa.p, P->q, P->ncell, P->cell_block.p,
-                                 P->bcell.p, P->Gk.p, nullptr));
-    else
-      SF_HIP(launch_scale_kappa(P->G.p, P->kappa.p, P->ng, P->nqc, P->ncell, P->Gk.p, nullptr));
-    SF_HIP(hipDeviceSynchronize());
-  }
-  return SF_OK;
+  P->gk_valid = false;
+  // a plan whose FP64 apply reads the stored G gets kappa * G now (so its
+  // applies, sf_cg_noalloc's included, never build it); with the fused
+  // geometry it is built on first use
+  return geo_f64(P) ? SF_OK : ensure_gk(P);
 }
 
 // 3D geometry lives in the block-interleaved layout of the apply, which
