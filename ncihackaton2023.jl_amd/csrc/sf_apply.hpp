@@ -100,6 +100,15 @@ struct Apply3Shape {
       (sizeof(T) == 8 && GEO && N == 5 && Q <= 6) ? 4 :
       (sizeof(T) == 8 && (N == 7 || (GEO && ((N == 8 && Q <= 9) || (N == 9 && Q <= 10))))) ? 3
                                                                                             : 1;
+  // fused geometry, z stage: SPLIT forms all q gradients of the column
+  // first and then applies G point by point in place, so the geometry's
+  // registers are never live beside the 3N contraction inputs (FP64 at N =
+  // 6: 126 instead of 152 VGPRs, a fourth wave per SIMD; at N = 8 no
+  // spills instead of 7).  Otherwise G is applied as each gradient is
+  // formed, which gives the scheduler both to interleave: where the
+  // registers do not change the waves (FP64 at N = 5 and 7, FP32 below
+  // N = 6) that was 2-3% faster (measured A/B/A/B).
+  static constexpr bool SPLIT = GEO && (sizeof(T) == 8 ? (N != 5 && N != 7) : N >= 6);
   // the apply kernel's LDS strides (SA1 / SB1 / QP above size EB and serve
   // the adjoint kernel's dense layout).  The FP64 table assumes the shared
   // region from N = 5 on; the stored-G kernel at N = 5 (split regions)
@@ -445,11 +454,15 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
       cy[z] = sB[S::TB + z * S::ZB];
       cz[z] = sB[2 * S::TB + z * S::ZB];
     }
-    T tx[Q], ty[Q], tz[Q];
     // GEO: edge vectors of the trilinear map for this quadrature column
-    // (i, k): dx/dxi = E0 + zeta dE, dx/deta = F0 + zeta dF, dx/dzeta = H
+    // (i, k): dx/dxi = E0 + zeta dE, dx/deta = F0 + zeta dF, dx/dzeta = H;
+    // formed before the gradients, or after them (SPLIT)
+    T tx[Q], ty[Q], tz[Q];
     T E0[3], dE[3], F0[3], dF[3], H[3], wik = T(0);
-    if constexpr (GEO) {
+    // the FP32 CG form at N = 8 keeps the interleaved order: there the
+    // split frees registers without adding a wave (2% slower per iteration)
+    constexpr bool SPLIT = S::SPLIT && !(DOT && sizeof(T) == 4 && N == 8);
+    if constexpr (GEO && !SPLIT) {
       const int qi = r3 % Q, qk = r3 / Q;
       wik = tW[qi] * tW[qk];
       column_geom3<T>(regV + e3 * kGeoCoef3, tP[qi], tP[qk], E0, dE, F0, dF, H);
@@ -465,7 +478,11 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
         gy = fma(t_.b[l * N + z], cy[z], gy);
         gz = fma(t_.d[l * N + z], cz[z], gz);
       }
-      if constexpr (GEO) {
+      if constexpr (SPLIT) {
+        tx[l] = gx;
+        ty[l] = gy;
+        tz[l] = gz;
+      } else if constexpr (GEO) {
         const T ze = tP[l];
         T j0[3], j1[3];
 #pragma unroll
@@ -479,6 +496,23 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
         tx[l] = gq[0][l] * gx + gq[1][l] * gy + gq[2][l] * gz;
         ty[l] = gq[1][l] * gx + gq[3][l] * gy + gq[4][l] * gz;
         tz[l] = gq[2][l] * gx + gq[4][l] * gy + gq[5][l] * gz;
+      }
+    }
+    if constexpr (SPLIT) {
+      const int qi = r3 % Q, qk = r3 / Q;
+      wik = tW[qi] * tW[qk];
+      column_geom3<T>(regV + e3 * kGeoCoef3, tP[qi], tP[qk], E0, dE, F0, dF, H);
+#pragma unroll
+      for (int l = 0; l < Q; ++l) {
+        const T ze = tP[l];
+        T j0[3], j1[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          j0[a] = fma(ze, dE[a], E0[a]);
+          j1[a] = fma(ze, dF[a], F0[a]);
+        }
+        const T wl = KAP ? wik * tW[l] * kq[l] : wik * tW[l];
+        apply_geom<T>(j0, j1, H, wl, tx[l], ty[l], tz[l], tx[l], ty[l], tz[l]);
       }
     }
     TableRows<T, S::TK> tt_(tB + Q * N, tD + Q * N);  // transposed tables
