@@ -71,8 +71,10 @@ struct Apply3Shape {
   // shared A / B region (N >= 6, and N = 5 with the fused geometry; the
   // stored-G kernel at N = 5 is faster without the extra barriers): LDS per
   // cell is the y-pass output, or the x-pass output next to the cell's x
-  // values
-  static constexpr bool LC = N >= 6 || (N == 5 && GEO);
+  // values.  FP64 fused geometry at N = 3 too: 35 -> 24 KB per workgroup,
+  // five workgroups per CU instead of four (Q2 -9%; at N = 4 and in FP32
+  // it measured no better)
+  static constexpr bool LC = N >= 6 || (N == 5 && GEO) || (N == 3 && GEO && sizeof(T) == 8);
   static constexpr int SC = LC ? cmax(3 * SB1, 2 * SA1 + NNN) : SE;
   // cells per workgroup: fill ~256 lanes in the widest stage, within 64 KiB
   // of FP64 LDS (the same EB for every T, so FP64 and FP32 applies share the
