@@ -677,7 +677,17 @@ struct Apply2Shape {
   // T: FP64 and FP32 applies share the blocks)
   static constexpr int EB_T = cmax(1, 256 / MAXT);
   static constexpr int EB_L = 65536 / ((2 * N * QPAD + NN + 8) * 8);
-  static constexpr int EB = EB_T < EB_L ? EB_T : EB_L;
+  static constexpr int EB_TL = EB_T < EB_L ? EB_T : EB_L;
+  // at (n, q) = (5, 4) and (8, 7) (the reference's 2D benchmark at Q4 and
+  // Q7, q = p) a few cells fewer per block give one more FP64 workgroup per
+  // CU (160 KiB of LDS): 51 -> 48 and 32 -> 28 cells, FP64 -5% and -7%
+  // (A/B/A/B).  At (6, 5) the same step (42 -> 40 cells, 4 -> 5 workgroups)
+  // was 4.5% slower, so it is not a general rule.
+  static constexpr int CELL_B = (2 * N * Q + NN + kGeoCoef2) * 8;
+  static constexpr int W0 = 163840 / (EB_TL * CELL_B);
+  static constexpr int EB_O = 163840 / ((W0 + 1) * CELL_B);
+  static constexpr bool MORE_WG = (N == 5 && Q == 4) || (N == 8 && Q == 7);
+  static constexpr int EB = MORE_WG ? EB_O : EB_TL;
   static constexpr int NT = round_up(EB * MAXT, 64);
   static constexpr int RX = EB * NN;  // distinct dofs of a block, at most
   static constexpr int KU = (RX + NT - 1) / NT;
