@@ -134,8 +134,11 @@ struct Apply3Shape {
   // spilled) give a fourth workgroup per CU (Q4 -4%); at N = 8 it halves
   // the spills (Q7 -3.5%).  Elsewhere the extra LDS costs a workgroup per
   // CU (Q3 +8%, Q6 +20%; measured A/B/A/B).
+  // FP32 at N = 8 (Q <= 9), with the split z stage: the CG form goes from
+  // 115 to 95 VGPRs, five waves per SIMD, the config-3 mixed iteration -2%.
   static constexpr bool DVL =
-      sizeof(T) == 8 && GEO && ((N == 5 && Q <= 6) || (N == 8 && Q <= 9));
+      (sizeof(T) == 8 && GEO && ((N == 5 && Q <= 6) || (N == 8 && Q <= 9))) ||
+      (sizeof(T) == 4 && GEO && N == 8 && Q <= 9);
   static constexpr int LDS_BYTES = RC * (int)sizeof(T) + (DVL ? RX * 4 : 0);
   // distinct dofs of a block handled per thread in the gather / write-back
   static constexpr int KU = (EB * NNN + NT - 1) / NT;
@@ -461,9 +464,7 @@ __attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
     // formed before the gradients, or after them (SPLIT)
     T tx[Q], ty[Q], tz[Q];
     T E0[3], dE[3], F0[3], dF[3], H[3], wik = T(0);
-    // the FP32 CG form at N = 8 keeps the interleaved order: there the
-    // split frees registers without adding a wave (2% slower per iteration)
-    constexpr bool SPLIT = S::SPLIT && !(DOT && sizeof(T) == 4 && N == 8);
+    constexpr bool SPLIT = S::SPLIT;
     if constexpr (GEO && !SPLIT) {
       const int qi = r3 % Q, qk = r3 / Q;
       wik = tW[qi] * tW[qk];
