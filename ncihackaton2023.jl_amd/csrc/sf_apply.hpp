@@ -89,14 +89,18 @@ struct Apply3Shape {
   static constexpr int NT = round_up(EB * MAXT, 64);
   // minimum waves per SIMD the compiler must fit the FP64 kernel into.  At
   // N = 7 and, fused geometry only, (N = 8, Q <= 9) and (N = 9, Q <= 10) it
-  // would use 170-206 VGPRs (2 waves per SIMD) while the LDS allows 3
-  // workgroups per CU; held to 168 (a few spills from N = 8) the third wave
-  // wins (measured: Q6 -13%, Q7 -6%, Q8 -12%).  The stored-G kernel spills
-  // more: with the cap it is 2% faster at Q6 but 14% slower at Q7 and 26%
-  // slower at Q8.  At N = 10 the spills cost more than the wave gains
-  // (Q9 with q = 11: +14%).  The FP32 kernel at N = 6, Q <= 7 fits five
-  // waves without spills (Q5 -7%; at N = 7, 8 the same cap was +1% / +14%).
-  // Elsewhere the compiler's own choice was as fast or faster.
+  // used 170-206 VGPRs (2 waves per SIMD) while the LDS allows 3
+  // workgroups per CU; held to 168 the third wave won (measured: Q6 -13%,
+  // Q7 -6%, Q8 -12%).  Since the split z stage (SPLIT below) the fused
+  // kernels at N = 8, 9 fit 168 by themselves (144-167 VGPRs, no spills);
+  // the cap stays for N = 7, which keeps the interleaved order.  The
+  // stored-G kernel spills more: with the cap it is 2% faster at Q6 but 14%
+  // slower at Q7 and 26% slower at Q8.  At N = 10 the spills cost more than
+  // the wave gains (Q9 with q = 11: +14%).  The FP32 kernel at N = 6, Q <= 7
+  // fits five waves without spills (Q5 -7%; at N = 7, 8 the same cap was
+  // +1% / +14%).  A 4-wave cap on the FP64 kappa apply at N = 6 (140 VGPRs,
+  // 10 spilled under the cap) was 2% slower.  Elsewhere the compiler's own
+  // choice was as fast or faster.
   static constexpr int WPE =
       (sizeof(T) == 4 && N == 6 && Q <= 7) ? 5 :
       (sizeof(T) == 8 && GEO && N == 5 && Q <= 6) ? 4 :
