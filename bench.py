@@ -502,9 +502,10 @@ def mode_adjoint(args, torch, pkg):
         torch.cuda.synchronize()
         t = (time.perf_counter() - t0) / args.steps
         if form == "fused":
-            # u, r read and dJ/du written; cell dofs; 24 vertex coordinates
-            # per cell; kappa read and dJ/dkappa written per quadrature point
-            B = 8 * 3 * nf + 4 * n ** 3 * E + 8 * 24 * E + 8 * 2 * nq * E
+            # u, r read and dJ/du written; cell dofs; the trilinear map's 21
+            # coefficients per cell; kappa read and dJ/dkappa written per
+            # quadrature point
+            B = 8 * 3 * nf + 4 * n ** 3 * E + 8 * 21 * E + 8 * 2 * nq * E
         else:
             # the same with the six G entries per quadrature point
             B = 8 * 3 * nf + 4 * n ** 3 * E + 8 * (6 + 1 + 1) * nq * E
@@ -551,7 +552,8 @@ def mode_q9(args, torch, pkg):
     y = torch.empty_like(x)
     wall, kern = time_apply(torch, A, x, y, args.steps, args.warmup)
     # fused-geometry apply reading kappa per quadrature point: bytes of the
-    # vectors, cell dofs, vertices and kappa; FLOPs of the reference's model
+    # vectors, cell dofs, map coefficients and kappa; FLOPs of the reference's
+    # model
     B64 = alg_bytes_fused(nf, E, n, D) + 8 * nq * E
     F = pkg.count_manual_flops_poisson_matrix_free(E, D, (q,) * D, (n,) * D,
                                                    count_only_fma_ops=False,
@@ -625,8 +627,9 @@ def mode_q9(args, torch, pkg):
         call()
     torch.cuda.synchronize()
     ta = (time.perf_counter() - t0) / 10
-    # fused-geometry adjoint: vectors, cell dofs, vertices, kappa, dJ/dkappa
-    Badj = 8 * 3 * nf + 4 * n ** 3 * E + 8 * 24 * E + 8 * 2 * nq * E
+    # fused-geometry adjoint: vectors, cell dofs, map coefficients, kappa,
+    # dJ/dkappa
+    Badj = 8 * 3 * nf + 4 * n ** 3 * E + 8 * 21 * E + 8 * 2 * nq * E
     res["adjoint"] = {"ms": ta * 1e3, "hbm_frac": Badj / ta / 1e9 / HBM_PEAK_GBS,
                       "over_f64_apply": ta / kern}
     log("adjoint", res["adjoint"])
