@@ -71,10 +71,12 @@ struct Apply3Shape {
   // shared A / B region (N >= 6, and N = 5 with the fused geometry; the
   // stored-G kernel at N = 5 is faster without the extra barriers): LDS per
   // cell is the y-pass output, or the x-pass output next to the cell's x
-  // values.  FP64 fused geometry at N = 3 too: 35 -> 24 KB per workgroup,
-  // five workgroups per CU instead of four (Q2 -9%; at N = 4 and in FP32
+  // values.  FP64 fused geometry at N = 3 and 4 too: 35 -> 24 KB and
+  // 40 -> 28 KB per workgroup, five workgroups per CU instead of four (Q2
+  // -9%; Q3 -4% with the dof ids in LDS and a 5-wave cap, below; in FP32
   // it measured no better)
-  static constexpr bool LC = N >= 6 || (N == 5 && GEO) || (N == 3 && GEO && sizeof(T) == 8);
+  static constexpr bool LC =
+      N >= 6 || (N == 5 && GEO) || ((N == 3 || N == 4) && GEO && sizeof(T) == 8);
   static constexpr int SC = LC ? cmax(3 * SB1, 2 * SA1 + NNN) : SE;
   // cells per workgroup: fill ~256 lanes in the widest stage, within 64 KiB
   // of FP64 LDS (the same EB for every T, so FP64 and FP32 applies share the
@@ -99,11 +101,15 @@ struct Apply3Shape {
   // the wave gains (Q9 with q = 11: +14%).  The FP32 kernel at N = 6, Q <= 7
   // fits five waves without spills (Q5 -7%; at N = 7, 8 the same cap was
   // +1% / +14%).  A 4-wave cap on the FP64 kappa apply at N = 6 (140 VGPRs,
-  // 10 spilled under the cap) was 2% slower.  Elsewhere the compiler's own
-  // choice was as fast or faster.
+  // 10 spilled under the cap) was 2% slower.  FP64 fused geometry at N = 4
+  // (Q <= 5): five waves (96 VGPRs, 4 spilled) now that the shared region
+  // and the ids in LDS let five workgroups fit; the kappa apply there (12
+  // spills under that cap) keeps the compiler's choice.  Elsewhere the
+  // compiler's own choice was as fast or faster.
   static constexpr int WPE =
       (sizeof(T) == 4 && N == 6 && Q <= 7) ? 5 :
       (sizeof(T) == 8 && GEO && N == 5 && Q <= 6) ? 4 :
+      (sizeof(T) == 8 && GEO && N == 4 && Q <= 5) ? 5 :
       (sizeof(T) == 8 && (N == 7 || (GEO && ((N == 8 && Q <= 9) || (N == 9 && Q <= 10))))) ? 3
                                                                                             : 1;
   // fused geometry, z stage: SPLIT forms all q gradients of the column
@@ -138,10 +144,12 @@ struct Apply3Shape {
   // spilled) give a fourth workgroup per CU (Q4 -4%); at N = 8 it halves
   // the spills (Q7 -3.5%).  Elsewhere the extra LDS costs a workgroup per
   // CU (Q3 +8%, Q6 +20%; measured A/B/A/B).
+  // FP64 at N = 4 (Q <= 5), with the shared region and a 5-wave cap: 96
+  // VGPRs with 4 spilled (7 without the ids in LDS), Q3 -4%.
   // FP32 at N = 8 (Q <= 9), with the split z stage: the CG form goes from
   // 115 to 95 VGPRs, five waves per SIMD, the config-3 mixed iteration -2%.
   static constexpr bool DVL =
-      (sizeof(T) == 8 && GEO && ((N == 5 && Q <= 6) || (N == 8 && Q <= 9))) ||
+      (sizeof(T) == 8 && GEO && ((N == 4 && Q <= 5) || (N == 5 && Q <= 6) || (N == 8 && Q <= 9))) ||
       (sizeof(T) == 4 && GEO && N == 8 && Q <= 9);
   static constexpr int LDS_BYTES = RC * (int)sizeof(T) + (DVL ? RX * 4 : 0);
   // distinct dofs of a block handled per thread in the gather / write-back
@@ -280,7 +288,7 @@ __device__ __forceinline__ void apply_geom(const T (&j0)[3], const T (&j1)[3],
 // reading p and q again.
 template <int N, int Q, typename T, typename TG, bool GEO, bool DOT, bool KAP = false>
 __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
-__attribute__((amdgpu_waves_per_eu(Apply3Shape<N, Q, T, GEO>::WPE)))
+__attribute__((amdgpu_waves_per_eu((KAP && sizeof(T) == 8 && N == 4) ? 1 : Apply3Shape<N, Q, T, GEO>::WPE)))
     k_apply3d(const T* __restrict__ x, T* __restrict__ y,
               const uint16_t* __restrict__ pidx,
               const int32_t* __restrict__ bdofs,
