@@ -545,7 +545,7 @@ def mode_q9(args, torch, pkg):
     kappa = np.exp(pkg.normal(3, space.n_cells * nq))
     A = pkg.LazyMatrix(mp, space, kappa=kappa)
     nf, E = space.n_free, space.n_cells
-    res = {"metric": "config 5: 3D 24^3 Q9 q=11 perturbed, log-normal kappa", "n_free": nf,
+    res = {"metric": "3D 24^3 Q9 q=11 perturbed, log-normal kappa", "n_free": nf,
            "n_cells": E, "n_gpus": 1}
     xh = pkg.uniform(0, nf, -1.0, 1.0)
     x = torch.from_numpy(xh).cuda()

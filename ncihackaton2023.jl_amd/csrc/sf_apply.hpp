@@ -147,7 +147,7 @@ struct Apply3Shape {
   // FP64 at N = 4 (Q <= 5), with the shared region and a 5-wave cap: 96
   // VGPRs with 4 spilled (7 without the ids in LDS), Q3 -4%.
   // FP32 at N = 8 (Q <= 9), with the split z stage: the CG form goes from
-  // 115 to 95 VGPRs, five waves per SIMD, the config-3 mixed iteration -2%.
+  // 115 to 95 VGPRs, five waves per SIMD, the 32^3 Q7 mixed PCG iteration -2%.
   static constexpr bool DVL =
       (sizeof(T) == 8 && GEO && ((N == 4 && Q <= 5) || (N == 5 && Q <= 6) || (N == 8 && Q <= 9))) ||
       (sizeof(T) == 4 && GEO && N == 8 && Q <= 9);

@@ -1,5 +1,5 @@
 """GPU parity of the HIP apply (FP64) against the CPU oracle, through the
-C ABI.  Bar: 1e-12 relative l2 (the north-star FP64 tolerance)."""
+C ABI.  Bar: 1e-12 relative l2 (the FP64 parity tolerance)."""
 import ctypes
 import sys
 
@@ -200,10 +200,10 @@ def test_demo_step_by_step_on_gpu(pkg, oracle_mod, cuda, p, q, m, perturb):
     A.free()
 
 
-def test_config1_2d_q4_vs_oracle_and_assembled(pkg, oracle_mod, brute, cuda):
+def test_2d_64sq_q4_vs_oracle_and_assembled(pkg, oracle_mod, brute, cuda):
     """2D unit square, 64x64 Cartesian quads, Q4, 6 Gauss points, u ~ U[-1,1]
     seed 0: against the oracle and the assembled CSR SpMV."""
-    say("config 1: 2D 64^2 Q4 q=6")
+    say("2D 64^2 Q4 q=6")
     model, space, mp, A = make(pkg, 2, 64, 4, 6, perturb=0.0)
     assert space.n_free == 65025
     x = pkg.uniform(0, space.n_free, -1, 1)
@@ -230,9 +230,9 @@ def test_reduced_configs_vs_oracle(pkg, oracle_mod, cuda, D, m, p, q):
     assert rel_l2(y, y_ref) < TOL
 
 
-def test_config2_full_size_vs_oracle(pkg, oracle_mod, cuda):
+def test_3d_64cube_q4_full_size_vs_oracle(pkg, oracle_mod, cuda):
     """3D 64^3 perturbed hexes, Q4, 6 points (16.6M dofs), u ~ U[-1,1]."""
-    say("config 2 full size: 3D 64^3 Q4 q=6 perturbed")
+    say("full size: 3D 64^3 Q4 q=6 perturbed")
     model, space, mp, A = make(pkg, 3, 64, 4, 6)
     assert space.n_free == 16581375
     x = pkg.uniform(0, space.n_free, -1, 1)
@@ -253,15 +253,15 @@ def test_config2_full_size_vs_oracle(pkg, oracle_mod, cuda):
 
 
 @pytest.mark.parametrize("cfg,m,p,q,with_kappa,n_free", [
-    ("config 3", 32, 7, 9, False, 11089567),
-    ("config 4", 48, 5, 7, True, 13651919),
-    ("config 5", 24, 9, 11, True, 9938375)])
-def test_configs_full_size_vs_oracle(pkg, oracle_mod, cuda, cfg, m, p, q, with_kappa, n_free):
+    ("32cube_q7", 32, 7, 9, False, 11089567),
+    ("48cube_q5_kappa", 48, 5, 7, True, 13651919),
+    ("24cube_q9_kappa", 24, 9, 11, True, 9938375)])
+def test_large_meshes_full_size_vs_oracle(pkg, oracle_mod, cuda, cfg, m, p, q, with_kappa, n_free):
     """The other 3D workloads at their full sizes (perturbed meshes; log-normal
     kappa per quadrature point where the workload has one): the FP64 apply
     against the oracle (1e-12), symmetry x.Av = v.Ax, and the FP32 apply
     within 1e-5 of the oracle."""
-    say("%s full size: 3D %d^3 Q%d q=%d kappa=%d" % (cfg, m, p, q, with_kappa))
+    say("full size: %s (m=%d p=%d q=%d kappa=%d)" % (cfg, m, p, q, with_kappa))
     model = pkg.CartesianModel(3, m, perturb=0.2, seed=1)
     space = pkg.FESpace(model, p)
     assert space.n_free == n_free

@@ -1,4 +1,4 @@
-"""GPU tests of the north-star extensions, through the C ABI:
+"""GPU tests of the extensions beyond the reference's apply, through the C ABI:
 reduced-precision applies (error against the FP64 oracle), the fused
 adjoint kernel (oracle gradient + central finite differences), and the
 device-resident CG / mixed-precision iterative refinement."""
@@ -193,7 +193,7 @@ def test_adjoint_every_fused_kernel(pkg, oracle_mod, cuda, monkeypatch):
     say("worst rel. l2 %.2e" % worst)
 
 
-def test_adjoint_homogeneity_identity_config4_size(pkg, cuda):
+def test_adjoint_homogeneity_identity_48cube_q5(pkg, cuda):
     """Size-independent properties of the pullback at the differentiation
     workload's full size (48^3 Q5, q = 7, log-normal kappa): A(kappa) is
     linear in kappa, so sum_q kappa_q dJdk_q = lam . A(kappa) u for the

@@ -276,10 +276,10 @@ def test_flop_model_unstructured_hand_values(pkg):
     roofline uses, against values worked out by hand from the reference's
     formula: in 3D the weights-and-Jacobian step is (D-1) D^D + D^2 = 63
     FMA-only operations per quadrature point, 2*63 + 27 + 6 = 159 in all;
-    in 2D 8 and 2*8 + 4 + 4 = 24.  The contractions of config 2 (Q4,
-    q = 6) are 3 (6*125 + 36*25 + 216*5) = 8190 FMAs each way per cell."""
+    in 2D 8 and 2*8 + 4 + 4 = 24.  The contractions of the 64^3 Q4 workload
+    (q = 6) are 3 (6*125 + 36*25 + 216*5) = 8190 FMAs each way per cell."""
     f = pkg.count_manual_flops_poisson_matrix_free
-    # config 2: 64^3 cells, q = 6, n = 5; 16380 + 159*216 + 16380 = 67104 per cell
+    # 64^3 cells, q = 6, n = 5: 16380 + 159*216 + 16380 = 67104 per cell
     assert f(64 ** 3, 3, (6,) * 3, (5,) * 3, count_only_fma_ops=False,
              structured_mesh_opt=False) == 17_590_910_976
     assert f(1, 3, (6,) * 3, (5,) * 3, count_only_fma_ops=True,
