@@ -221,8 +221,8 @@ def test_2d_64sq_q4_vs_oracle_and_assembled(pkg, oracle_mod, brute, cuda):
 
 @pytest.mark.parametrize("D,m,p,q", [(3, 8, 4, 6), (3, 16, 4, 6), (3, 4, 7, 9), (3, 6, 5, 7),
                                      (3, 3, 9, 11), (3, 8, 2, 4), (3, 8, 3, 5), (3, 5, 6, 8)])
-def test_reduced_configs_vs_oracle(pkg, oracle_mod, cuda, D, m, p, q):
-    say("reduced config D=%d m=%d p=%d q=%d" % (D, m, p, q))
+def test_small_meshes_vs_oracle(pkg, oracle_mod, cuda, D, m, p, q):
+    say("small mesh D=%d m=%d p=%d q=%d" % (D, m, p, q))
     model, space, mp, A = make(pkg, D, m, p, q)
     x = pkg.uniform(0, space.n_free, -1, 1)
     y = gpu_apply(cuda, A, x)
