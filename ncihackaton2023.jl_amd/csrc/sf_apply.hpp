@@ -303,6 +303,8 @@ __attribute__((amdgpu_waves_per_eu((KAP && sizeof(T) == 8 && N == 4) ? 1 : Apply
   static_assert(GEO || !KAP, "apply3d: kappa per point needs the fused geometry");
   static_assert(S::EB == Apply3Shape<N, Q, T>::EB && S::NT == Apply3Shape<N, Q, T>::NT,
                 "apply3d: one block structure for both forms");
+  static_assert(S::EB == Apply3Shape<N, Q, double>::EB,
+                "apply3d: the host builds the blocks of the FP64 shape for every form");
   __shared__ T smem[S::RC + (GEO ? S::EB * kGeoCoef3 : 0)];
   __shared__ int32_t sdof[S::DVL ? S::RX : 1];  // DVL: the block's dof ids
   T* const regV = smem + S::RC;  // GEO: the block's map coefficients
@@ -722,6 +724,8 @@ __global__ void __launch_bounds__((Apply2Shape<N, Q, T>::NT))
               const T* __restrict__ tD, const T* __restrict__ Xv,
               const T* __restrict__ tP, const T* __restrict__ tW, int accumulate) {
   using S = Apply2Shape<N, Q, T>;
+  static_assert(S::EB == Apply2Shape<N, Q, double>::EB,
+                "apply2d: the host builds the blocks of the FP64 shape for every form");
   __shared__ T smem[S::EB * S::SE + S::RX + (GEO ? S::EB * kGeoCoef2 : 0)];
   T* const regA = smem;                    // per cell [2][N][QP]
   T* const regX = smem + S::EB * S::SE;    // the block's x values, later y
