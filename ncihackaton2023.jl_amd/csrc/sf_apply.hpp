@@ -283,8 +283,10 @@ __device__ __forceinline__ void apply_geom(const T (&j0)[3], const T (&j1)[3],
 // DOT (with GEO): wave w of workgroup b also writes dotp[b * NT / 64 + w],
 // its share of x.Ax = sum over blocks of x . (the block's contribution):
 // in the write-back each lane adds x_d times the assembled value of its
-// distinct dofs d (FP32: x kept in registers from the prologue; FP64: x
-// re-read, which costs fewer registers).  A CG step then gets p.Ap without
+// distinct dofs d (FP32, and FP64 at N = 8 without kappa: x kept in
+// registers from the prologue, 147 -> 158 VGPRs there, still 3 waves, the
+// 32^3 Q7 PCG iteration -2.6%; otherwise FP64 re-reads x, which costs fewer
+// registers).  A CG step then gets p.Ap without
 // reading p and q again.
 template <int N, int Q, typename T, typename TG, bool GEO, bool DOT, bool KAP = false>
 __global__ void __launch_bounds__((Apply3Shape<N, Q, T>::NT))
@@ -628,7 +630,7 @@ __attribute__((amdgpu_waves_per_eu((KAP && sizeof(T) == 8 && N == 4) ? 1 : Apply
   if constexpr (DOT) {
 #pragma unroll
     for (int k = 0; k < S::KU; ++k)
-      xr[k] = sizeof(T) == 4 ? xv[k] : (dv[k] >= 0 ? x[dv[k]] : T(0));
+      xr[k] = (sizeof(T) == 4 || (N == 8 && !KAP)) ? xv[k] : (dv[k] >= 0 ? x[dv[k]] : T(0));
   }
   double ep = 0.0;
 #pragma unroll
