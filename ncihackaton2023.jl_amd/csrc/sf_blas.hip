@@ -14,9 +14,11 @@ namespace sf {
 
 namespace {
 
-// Block sum over kRedThreads threads; result valid on thread 0.
+// Block sum over NT threads (kRedThreads unless given); result valid on
+// thread 0.
+template <int NT = kRedThreads>
 __device__ double block_sum(double v) {
-  __shared__ double ws[kRedThreads / 64];
+  __shared__ double ws[NT / 64];
   v = wave_sum_f64(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __syncthreads();  // ws may still be read by a previous call
@@ -25,10 +27,15 @@ __device__ double block_sum(double v) {
   double s = 0.0;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int w = 0; w < kRedThreads / 64; ++w) s += ws[w];
+    for (int w = 0; w < NT / 64; ++w) s += ws[w];
   }
   return s;
 }
+
+// Threads of the one-workgroup CG finish: the shares of a streaming grid
+// (up to 4096 per sum) are read four per thread in one round of loads,
+// where 256 threads needed 16 dependent rounds.
+constexpr int kFinThreads = 1024;
 
 // Publish this block's partial; true in every thread of the last block.
 __device__ bool arrive(Reducer red, double partial) {
@@ -304,17 +311,18 @@ __global__ void __launch_bounds__(256)
 // One workgroup: res2 = r.r and rr_new = r.z (r.r without a
 // preconditioner) from the np shares, beta = rr_new / rr, the stop test on
 // res2.
-__global__ void __launch_bounds__(kRedThreads)
+__global__ void __launch_bounds__(kFinThreads)
     k_cg_resid_finish(const double* __restrict__ part, int np, int pc,
                       CGScalars* __restrict__ sc) {
   if (sc->done) return;
   double s = 0.0, sz = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+#pragma unroll 4
+  for (int i = threadIdx.x; i < np; i += kFinThreads) {
     s += part[i];
     if (pc) sz += part[np + i];
   }
-  s = block_sum(s);
-  if (pc) sz = block_sum(sz);
+  s = block_sum<kFinThreads>(s);
+  if (pc) sz = block_sum<kFinThreads>(sz);
   if (threadIdx.x == 0) {
     const double rr_new = pc ? sz : s;
     sc->res2 = s;
@@ -618,7 +626,7 @@ hipError_t update_two_phase(V* x, V* r, const V* p, const W* q, const float* din
                        n, red.part, sc);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_cg_resid_finish, dim3(1), dim3(kRedThreads), 0, s, red.part,
+  hipLaunchKernelGGL(k_cg_resid_finish, dim3(1), dim3(kFinThreads), 0, s, red.part,
                      (int)g, dinv ? 1 : 0, sc);
   return hipGetLastError();
 }
@@ -667,7 +675,7 @@ hipError_t resid_two_phase(V* r, const W* q, const float* dinv, int64_t n, Reduc
                        red.part, sc);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_cg_resid_finish, dim3(1), dim3(kRedThreads), 0, s, red.part,
+  hipLaunchKernelGGL(k_cg_resid_finish, dim3(1), dim3(kFinThreads), 0, s, red.part,
                      (int)g, dinv ? 1 : 0, sc);
   return hipGetLastError();
 }
