@@ -90,8 +90,8 @@ def cg_mixed_(x, A, b, rtol=1e-10, inner="f32", delta=0.1, maxiter=None,
     b - A x whenever it has fallen by `delta` (search direction kept).
     Pl: the preconditioner, as in cg_.  recurrence None: 'f32' with the
     Gauss-Lobatto preconditioner (its well-conditioned system loses no
-    iterations to the FP32 recurrence, which moves the fewest bytes: 1.46x
-    FP64 CG at 32^3 Q7 against 1.22x for 'f64p32'), else 'f64p32' (on the
+    iterations to the FP32 recurrence, which moves the fewest bytes: 1.48x
+    FP64 CG at 32^3 Q7 against 1.24x for 'f64p32'), else 'f64p32' (on the
     unpreconditioned system the FP32 recurrence needs about 50% more
     iterations)."""
     _vec_ok(A, x)
